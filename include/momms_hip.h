@@ -1,0 +1,118 @@
+/*
+ * momms_hip.h — C ABI of the MI355X (gfx950) GEMM engine.
+ *
+ * This is the boundary a momms GemmNode calls into. In the reference the
+ * hot path ends in a C micro-kernel reached through Rust FFI,
+ *   bli_dgemm_haswell_asm_6x8(k, alpha*, a*, b*, beta*, c*, rs_c, cs_c, ...)
+ *   (ukernels/bli_gemm_haswell_asm_d6x8.c:666-676,
+ *    bound at src/kern/ukernel_wrapper.rs:40-53 and :73-90),
+ * called once per 6x8 tile by KernelNM (src/kern/kernel_nm.rs:31-106) under
+ * the Goto tree (src/exper/l3_packing.rs:58-66). A per-tile GPU launch is not
+ * viable, so these entry points replace the whole subtree below
+ * GemmNode::run (src/composables/mod.rs:31-36) for Matrix<T> operands.
+ * They keep that FFI's conventions:
+ *   - scalars by pointer, C := beta*C + alpha*A*B, beta == 0 never reads C
+ *     (d6x8.c:929-931, :1076-1193);
+ *   - every operand has a row stride and a column stride (Matrix<T>'s
+ *     get_row_stride/get_column_stride, src/matrix/general_stride.rs:42-43);
+ *     column-major (rs == 1), row-major (cs == 1) and general strides work;
+ *   - all pointers are borrowed; matrix pointers are DEVICE pointers
+ *     (hipMalloc'd or otherwise GPU-visible), scalars are host pointers.
+ * Return value: 0 on success, -i when argument i (1-based) is invalid (as
+ * xerbla reports for the Fortran dgemm_ of src/util.rs:14-22), and a
+ * positive hipError_t for a HIP runtime failure. Nothing aborts. The text of
+ * the last failure on the calling thread is momms_last_error_string().
+ * `stream` is a hipStream_t (NULL = the default stream); every entry point
+ * only enqueues work on it and returns without synchronizing.
+ */
+#ifndef MOMMS_HIP_H
+#define MOMMS_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Text of the last error reported on this thread ("" if none). */
+const char* momms_last_error_string(void);
+
+/* Plain FP64 GEMM on FP64 matrix cores: C := beta*C + alpha*A*B, A m x k,
+ * B k x n, C m x n. Replaces the Goto subtree SpawnThreads/PartN ... KernelNM
+ * -> bli_dgemm_haswell_asm_6x8 (src/exper/l3_packing.rs:58-66,
+ * d6x8.c:666-1221). k == 0 gives C := beta*C. */
+int momms_hip_dgemm(int64_t m, int64_t n, int64_t k, const double* alpha, const double* A,
+                    int64_t rsA, int64_t csA, const double* B, int64_t rsB, int64_t csB,
+                    const double* beta, double* C, int64_t rsC, int64_t csC, void* stream);
+
+/* momms_hip_dgemm with an explicit tile configuration (0 = 128x128,
+ * 1 = 64x64, 2 = 128x64 per workgroup) instead of the built-in choice; for
+ * tuning. Argument 15 is tile_cfg. Same semantics as momms_hip_dgemm. */
+int momms_hip_dgemm_tiled(int64_t m, int64_t n, int64_t k, const double* alpha,
+                          const double* A, int64_t rsA, int64_t csA, const double* B,
+                          int64_t rsB, int64_t csB, const double* beta, double* C,
+                          int64_t rsC, int64_t csC, int tile_cfg, void* stream);
+
+/* ---- FP64-accurate GEMM from BF16 slices (Ozaki scheme I) -----------------
+ * No reference counterpart: the precision addition beside the Goto path.
+ * C := beta*C + alpha*A*B where each row of A and column of B is scaled by a
+ * power of two and cut into `slices` (1..8) BF16 integer slices, the slice
+ * products with t+u <= slices+1 run on BF16 matrix cores with exact FP32
+ * accumulation per k-chunk, and the levels are folded and rescaled in FP64
+ * on the GPU. The result equals, bit for bit, the CPU restatement in
+ * oracle/ozaki.c. A row or column holding Inf/NaN yields non-finite outputs
+ * in the entries it touches.
+ * `workspace` is caller-owned device memory of at least
+ * momms_hip_split_bf16_workspace_size(m, n, k, slices) bytes; it holds the
+ * packed slices and exponents between the prepare and multiply stages.
+ * Calls that share a workspace must be ordered on one stream. */
+size_t momms_hip_split_bf16_workspace_size(int64_t m, int64_t n, int64_t k, int slices);
+
+/* Whole split GEMM (prepare + multiply). Arguments 1-14 as momms_hip_dgemm,
+ * then slices (15), workspace (16), workspace_bytes (17), stream. */
+int momms_hip_dgemm_split_bf16(int64_t m, int64_t n, int64_t k, const double* alpha,
+                               const double* A, int64_t rsA, int64_t csA, const double* B,
+                               int64_t rsB, int64_t csB, const double* beta, double* C,
+                               int64_t rsC, int64_t csC, int slices, void* workspace,
+                               size_t workspace_bytes, void* stream);
+
+/* Stage 1: exponents and packed BF16 slices of A (rows) and B (columns) into
+ * the workspace: the device form of PackA/PackB (src/composables/pack.rs:311-419). */
+int momms_hip_split_bf16_prepare(int64_t m, int64_t n, int64_t k, const double* A, int64_t rsA,
+                                 int64_t csA, const double* B, int64_t rsB, int64_t csB,
+                                 int slices, void* workspace, size_t workspace_bytes,
+                                 void* stream);
+
+/* Stage 2: the fused slice-product multiply and FP64 reconstruction into C,
+ * from a workspace filled by stage 1 with the same m, n, k, slices. */
+int momms_hip_split_bf16_multiply(int64_t m, int64_t n, int64_t k, const double* alpha,
+                                  const double* beta, double* C, int64_t rsC, int64_t csC,
+                                  int slices, const void* workspace, size_t workspace_bytes,
+                                  void* stream);
+
+/* ---- accuracy statement -----------------------------------------------------
+ * Recomputes `samples` entries of beta*C0 + alpha*A*B in double-double on the
+ * GPU (Dot2, Ogita-Rump-Oishi 2005) and returns, on the host, the largest
+ * scaled error |C - C*| / (|alpha||A||B| + |beta||C0|) and relative error
+ * |C - C*| / |C*| of C against them. Entry e is (i, j) =
+ * (momms_sample_index(seed, e, 0, m), momms_sample_index(seed, e, 1, n)).
+ * ref_out (optional, device, `samples` doubles) receives C*. C0 may be NULL
+ * when beta == 0. Synchronizes `stream` before returning. The reference's
+ * counterpart is the unasserted residual of src/util.rs:55-85. */
+int momms_hip_gemm_check_sampled(int64_t m, int64_t n, int64_t k, const double* alpha,
+                                 const double* A, int64_t rsA, int64_t csA, const double* B,
+                                 int64_t rsB, int64_t csB, const double* beta, const double* C0,
+                                 const double* C, int64_t rsC, int64_t csC, int64_t samples,
+                                 uint64_t seed, double* ref_out, double* max_scaled,
+                                 double* max_rel, void* stream);
+
+/* Sample position used by momms_hip_gemm_check_sampled: SplitMix64 of
+ * seed + 2e (+1 for the column, which != 0) modulo extent. Host only. */
+uint64_t momms_sample_index(uint64_t seed, uint64_t e, int which, uint64_t extent);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MOMMS_HIP_H */

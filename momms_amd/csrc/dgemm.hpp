@@ -1,0 +1,28 @@
+// Internal launch interface of the FP64 matrix-core GEMM (used by the C ABI
+// entry points and by the batched and split-precision paths).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace momms {
+
+struct DgemmArgs {
+  int64_t m, n, k;
+  double alpha;
+  const double* A;
+  int64_t rsA, csA;
+  const double* B;
+  int64_t rsB, csB;
+  double beta;
+  double* C;
+  int64_t rsC, csC;
+  hipStream_t stream;
+};
+
+// Tile configurations (BM x BN, 4 wavefronts): 0 = 128x128, 1 = 64x64, 2 = 128x64.
+constexpr int kDgemmTileConfigs = 3;
+
+int dgemm_pick_tile(int64_t m, int64_t n, int64_t k);
+hipError_t dgemm_launch(const DgemmArgs& g, int tile_cfg);
+
+}  // namespace momms

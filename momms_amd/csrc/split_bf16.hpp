@@ -1,0 +1,28 @@
+// Internal interface of the split-BF16 FP64-accurate GEMM (used by the C ABI
+// entry points and the batched path).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace momms {
+
+// Workspace layout for one (m, n, k, s) problem: packed slices of A and B,
+// per-row / per-column exponents and the max |x| scratch they come from.
+struct SplitLayout {
+  int s;
+  int64_t mpad, npad, kpad;
+  size_t off_aq, off_bq, off_ea, off_eb, off_maxa, off_maxb, bytes;
+};
+
+SplitLayout split_layout(int64_t m, int64_t n, int64_t k, int s);
+int split_chunk(int s);
+
+hipError_t split_prepare(const SplitLayout& L, int64_t m, int64_t n, int64_t k, const double* A,
+                         int64_t rsA, int64_t csA, const double* B, int64_t rsB, int64_t csB,
+                         char* ws, hipStream_t st);
+hipError_t split_multiply_launch(const SplitLayout& L, int64_t m, int64_t n, int64_t k,
+                                 double alpha, double beta, double* C, int64_t rsC, int64_t csC,
+                                 const char* ws, hipStream_t st);
+
+}  // namespace momms

@@ -1,0 +1,51 @@
+/*
+ * ORACLE — test infrastructure only. Only tests/, the smoke check and
+ * bench.py's CPU-baseline leg may load liboracle.so; the product never does.
+ *
+ * Functions and what pins each (see the file headers for details):
+ *   goto.c   oracle_goto_dgemm     restated Goto tree + 6x8 kernel (reference path)
+ *            oracle_goto_scalar    same rounding order, scalar, no packing
+ *            oracle_ukernel_6x8    the micro-kernel alone
+ *            oracle_triple_loop    TripleLoop node
+ *   dd.c     oracle_dd_entries / oracle_dd_gemm   double-double (Dot2) values
+ *            oracle_residual       randomized residual check
+ *   ozaki.c  oracle_ozaki_*        split-BF16 algorithm restatement
+ */
+#ifndef MOMMS_ORACLE_H
+#define MOMMS_ORACLE_H
+#include <stdint.h>
+
+int oracle_goto_dgemm(int64_t m, int64_t n, int64_t k, const double* alpha, const double* A,
+                      int64_t rsA, int64_t csA, const double* B, int64_t rsB, int64_t csB,
+                      const double* beta, double* C, int64_t rsC, int64_t csC, int nthreads);
+void oracle_goto_scalar(int64_t m, int64_t n, int64_t k, const double* alpha, const double* A,
+                        int64_t rsA, int64_t csA, const double* B, int64_t rsB, int64_t csB,
+                        const double* beta, double* C, int64_t rsC, int64_t csC);
+void oracle_ukernel_6x8(int64_t k, const double* alpha, const double* a, const double* b,
+                        const double* beta, double* c, int64_t rs_c, int64_t cs_c);
+void oracle_triple_loop(int64_t m, int64_t n, int64_t k, const double* A, int64_t rsA,
+                        int64_t csA, const double* B, int64_t rsB, int64_t csB, double* C,
+                        int64_t rsC, int64_t csC);
+
+void oracle_dd_entries(int64_t k, const double* alpha, const double* A, int64_t rsA,
+                       int64_t csA, const double* B, int64_t rsB, int64_t csB,
+                       const double* beta, const double* C0, int64_t rsC, int64_t csC,
+                       int64_t count, const int64_t* rows, const int64_t* cols, double* out,
+                       double* mag);
+void oracle_dd_gemm(int64_t m, int64_t n, int64_t k, const double* alpha, const double* A,
+                    int64_t rsA, int64_t csA, const double* B, int64_t rsB, int64_t csB,
+                    const double* beta, const double* C0, int64_t rsC, int64_t csC,
+                    double* out, double* mag);
+double oracle_residual(int64_t m, int64_t n, int64_t k, const double* A, int64_t rsA,
+                       int64_t csA, const double* B, int64_t rsB, int64_t csB,
+                       const double* C, const double* C0, int64_t rsC, int64_t csC,
+                       double alpha, double beta, const double* w);
+
+int oracle_ozaki_chunk(int s);
+void oracle_ozaki_split(int64_t count, int64_t k, const double* X, int64_t stride_vec,
+                        int64_t stride_elem, int s, double* q, int* exps);
+void oracle_ozaki_gemm(int64_t m, int64_t n, int64_t k, const double* alpha, const double* A,
+                       int64_t rsA, int64_t csA, const double* B, int64_t rsB, int64_t csB,
+                       const double* beta, double* C, int64_t rsC, int64_t csC, int s);
+
+#endif
