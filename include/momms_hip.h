@@ -54,6 +54,21 @@ int momms_hip_dgemm_tiled(int64_t m, int64_t n, int64_t k, const double* alpha,
                           int64_t rsB, int64_t csB, const double* beta, double* C,
                           int64_t rsC, int64_t csC, int tile_cfg, void* stream);
 
+/* ---- FP32-accumulating GEMMs ------------------------------------------------
+ * C := beta*C + alpha*A*B with float C, alpha, beta; arguments as
+ * momms_hip_dgemm. momms_hip_sgemm takes float A, B (FP32 matrix cores,
+ * v_mfma_f32_32x32x2_f32): the counterpart of the reference's KNM sgemm leaf
+ * (src/kern/knm_kernel.rs:8-81 -> sgemm_knm_asm_16x24,
+ * ukernels/sgemm_knm_asm_24x16.c:30-38). momms_hip_gemm_bf16 takes A, B as
+ * bfloat16 bit patterns (uint16_t) and accumulates in FP32 on BF16 matrix
+ * cores (no reference counterpart). Any strides, as for momms_hip_dgemm. */
+int momms_hip_sgemm(int64_t m, int64_t n, int64_t k, const float* alpha, const float* A,
+                    int64_t rsA, int64_t csA, const float* B, int64_t rsB, int64_t csB,
+                    const float* beta, float* C, int64_t rsC, int64_t csC, void* stream);
+int momms_hip_gemm_bf16(int64_t m, int64_t n, int64_t k, const float* alpha, const uint16_t* A,
+                        int64_t rsA, int64_t csA, const uint16_t* B, int64_t rsB, int64_t csB,
+                        const float* beta, float* C, int64_t rsC, int64_t csC, void* stream);
+
 /* ---- FP64-accurate GEMM from BF16 slices (Ozaki scheme I) -----------------
  * No reference counterpart: the precision addition beside the Goto path.
  * C := beta*C + alpha*A*B where each row of A and column of B is scaled by a
@@ -90,6 +105,55 @@ int momms_hip_split_bf16_multiply(int64_t m, int64_t n, int64_t k, const double*
                                   const double* beta, double* C, int64_t rsC, int64_t csC,
                                   int slices, const void* workspace, size_t workspace_bytes,
                                   void* stream);
+
+/* ---- batched uneven GEMMs -----------------------------------------------------
+ * No reference counterpart: the batching addition. Problem i of a batch is
+ * C_i := beta*C_i + alpha*A_i*B_i with column-major operands at element
+ * offsets into three device arenas A, B, C (leading dimensions lda >= m,
+ * ldb >= k, ldc >= m). Output regions must not overlap. */
+typedef struct momms_gemm_desc {
+  int64_t m, n, k;
+  int64_t a_off, b_off, c_off;
+  int64_t lda, ldb, ldc;
+} momms_gemm_desc;
+
+typedef struct momms_batch_plan momms_batch_plan;
+
+/* Builds a plan for `count` problems (descs is a HOST array, copied).
+ * mode 0: plain FP64 matrix cores; mode 1: split-BF16 FP64-accurate with
+ * `slices` slices (0 = chosen by the autotuner: the smallest count whose
+ * sampled error is within 2x plain FP64's, 7 without autotuning).
+ * autotune != 0 times the candidate tile shapes per size bucket on this
+ * batch (reading A and B, writing only plan-owned scratch) and remembers the
+ * choice for the process. A and B may be NULL when autotune == 0.
+ * The plan owns device tables and, in mode 1, the slice workspace. */
+int momms_hip_batch_plan_create(const momms_gemm_desc* descs, int64_t count, int mode,
+                                int slices, int autotune, const double* A, const double* B,
+                                momms_batch_plan** out, void* stream);
+
+/* Enqueues the whole batch on `stream` (reset kernel, slicing in mode 1,
+ * persistent multiply kernels). */
+int momms_hip_batch_run(momms_batch_plan* plan, const double* alpha, const double* A,
+                        const double* B, const double* beta, double* C, void* stream);
+
+/* Captures the same sequence once into a hipGraph owned by the plan (on a
+ * plan-owned stream; the captured graph holds kernel nodes only), for
+ * momms_hip_batch_launch. Re-capturing replaces the graph. */
+int momms_hip_batch_capture(momms_batch_plan* plan, const double* alpha, const double* A,
+                            const double* B, const double* beta, double* C);
+
+/* One hipGraphLaunch of the captured batch on `stream`. */
+int momms_hip_batch_launch(momms_batch_plan* plan, void* stream);
+
+/* JSON text describing the plan's choices (tile per bucket, slices). */
+const char* momms_hip_batch_plan_describe(const momms_batch_plan* plan);
+
+int momms_hip_batch_plan_destroy(momms_batch_plan* plan);
+
+/* The comparison baseline: one momms_hip_dgemm launch per problem. */
+int momms_hip_dgemm_batched_loop(const momms_gemm_desc* descs, int64_t count,
+                                 const double* alpha, const double* A, const double* B,
+                                 const double* beta, double* C, void* stream);
 
 /* ---- accuracy statement -----------------------------------------------------
  * Recomputes `samples` entries of beta*C0 + alpha*A*B in double-double on the

@@ -56,10 +56,23 @@ _SIGS: dict[str, list] = {
                                      _c_vp, _c_i64, _c_i64, _c_dp, _c_vp, _c_vp, _c_i64, _c_i64,
                                      _c_i64, ctypes.c_uint64, _c_vp, _c_dp, _c_dp, _c_vp],
     "momms_sample_index": [ctypes.c_uint64, ctypes.c_uint64, _c_int, ctypes.c_uint64],
+    "momms_hip_sgemm": [_c_i64, _c_i64, _c_i64, _c_fp, _c_vp, _c_i64, _c_i64, _c_vp, _c_i64,
+                        _c_i64, _c_fp, _c_vp, _c_i64, _c_i64, _c_vp],
+    "momms_hip_batch_plan_create": [_c_vp, _c_i64, _c_int, _c_int, _c_int, _c_vp, _c_vp,
+                                    ctypes.POINTER(_c_vp), _c_vp],
+    "momms_hip_batch_run": [_c_vp, _c_dp, _c_vp, _c_vp, _c_dp, _c_vp, _c_vp],
+    "momms_hip_batch_capture": [_c_vp, _c_dp, _c_vp, _c_vp, _c_dp, _c_vp],
+    "momms_hip_batch_launch": [_c_vp, _c_vp],
+    "momms_hip_batch_plan_describe": [_c_vp],
+    "momms_hip_batch_plan_destroy": [_c_vp],
+    "momms_hip_dgemm_batched_loop": [_c_vp, _c_i64, _c_dp, _c_vp, _c_vp, _c_dp, _c_vp, _c_vp],
+    "momms_hip_gemm_bf16": [_c_i64, _c_i64, _c_i64, _c_fp, _c_vp, _c_i64, _c_i64, _c_vp, _c_i64,
+                            _c_i64, _c_fp, _c_vp, _c_i64, _c_i64, _c_vp],
 }
 _RESTYPES = {"momms_last_error_string": ctypes.c_char_p,
              "momms_hip_split_bf16_workspace_size": ctypes.c_size_t,
-             "momms_sample_index": ctypes.c_uint64}
+             "momms_sample_index": ctypes.c_uint64,
+             "momms_hip_batch_plan_describe": ctypes.c_char_p}
 
 
 def lib() -> ctypes.CDLL:
@@ -132,6 +145,128 @@ def dgemm(A, B, C, alpha: float = 1.0, beta: float = 1.0, stream=None, tile_cfg:
                                          ctypes.byref(be), c, rsc, csc, int(tile_cfg), s)
         check("momms_hip_dgemm_tiled", st)
     return C
+
+
+def _gemm_f32acc(fn: str, A, B, C, alpha, beta, in_dtype, stream):
+    import torch
+
+    for t, nm in ((A, "A"), (B, "B")):
+        if t.dtype != in_dtype:
+            raise TypeError(f"{nm} must be {in_dtype}")
+    if C.dtype != torch.float32:
+        raise TypeError("C must be float32")
+    m, k = A.shape
+    n = B.shape[1]
+    a, rsa, csa = _mat(A, m, k, "A")
+    b, rsb, csb = _mat(B, k, n, "B")
+    c, rsc, csc = _mat(C, m, n, "C")
+    al, be = ctypes.c_float(alpha), ctypes.c_float(beta)
+    st = getattr(lib(), fn)(m, n, k, ctypes.byref(al), a, rsa, csa, b, rsb, csb, ctypes.byref(be),
+                            c, rsc, csc, stream_handle(stream))
+    check(fn, st)
+    return C
+
+
+def sgemm(A, B, C, alpha: float = 1.0, beta: float = 1.0, stream=None):
+    """C := beta*C + alpha*A*B, float32 operands, FP32 matrix cores."""
+    import torch
+
+    return _gemm_f32acc("momms_hip_sgemm", A, B, C, alpha, beta, torch.float32, stream)
+
+
+def gemm_bf16(A, B, C, alpha: float = 1.0, beta: float = 1.0, stream=None):
+    """C := beta*C + alpha*A*B, bfloat16 A and B, float32 accumulation and C."""
+    import torch
+
+    return _gemm_f32acc("momms_hip_gemm_bf16", A, B, C, alpha, beta, torch.bfloat16, stream)
+
+
+class GemmDesc(ctypes.Structure):
+    """momms_gemm_desc: one column-major problem at element offsets into the arenas."""
+    _fields_ = [(f, ctypes.c_int64) for f in
+                ("m", "n", "k", "a_off", "b_off", "c_off", "lda", "ldb", "ldc")]
+
+
+def desc_array(rows):
+    """ctypes array of GemmDesc from (m, n, k, a_off, b_off, c_off, lda, ldb, ldc) tuples."""
+    arr = (GemmDesc * len(rows))()
+    for i, r in enumerate(rows):
+        arr[i] = GemmDesc(*[int(x) for x in r])
+    return arr
+
+
+class BatchPlan:
+    """A plan for one batch table (momms_hip_batch_plan_*)."""
+
+    def __init__(self, rows, mode: int = 0, slices: int = 0, autotune: bool = False, A=None,
+                 B=None, stream=None):
+        self.descs = desc_array(rows)
+        self.count = len(rows)
+        h = ctypes.c_void_p()
+        st = lib().momms_hip_batch_plan_create(
+            ctypes.cast(self.descs, _c_vp), self.count, mode, slices, int(bool(autotune)),
+            A.data_ptr() if A is not None else None, B.data_ptr() if B is not None else None,
+            ctypes.byref(h), stream_handle(stream))
+        check("momms_hip_batch_plan_create", st)
+        self.handle = h
+
+    def describe(self) -> dict:
+        import json
+
+        return json.loads(lib().momms_hip_batch_plan_describe(self.handle).decode())
+
+    def run(self, A, B, C, alpha=1.0, beta=1.0, stream=None):
+        al, be = ctypes.c_double(alpha), ctypes.c_double(beta)
+        st = lib().momms_hip_batch_run(self.handle, ctypes.byref(al), A.data_ptr(), B.data_ptr(),
+                                       ctypes.byref(be), C.data_ptr(), stream_handle(stream))
+        check("momms_hip_batch_run", st)
+
+    def capture(self, A, B, C, alpha=1.0, beta=1.0):
+        al, be = ctypes.c_double(alpha), ctypes.c_double(beta)
+        st = lib().momms_hip_batch_capture(self.handle, ctypes.byref(al), A.data_ptr(),
+                                           B.data_ptr(), ctypes.byref(be), C.data_ptr())
+        check("momms_hip_batch_capture", st)
+
+    def launch(self, stream=None):
+        check("momms_hip_batch_launch", lib().momms_hip_batch_launch(self.handle, stream_handle(stream)))
+
+    def close(self):
+        if getattr(self, "handle", None):
+            lib().momms_hip_batch_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def dgemm_batched_loop(rows_or_descs, A, B, C, alpha=1.0, beta=1.0, stream=None):
+    d = rows_or_descs if isinstance(rows_or_descs, ctypes.Array) else desc_array(rows_or_descs)
+    al, be = ctypes.c_double(alpha), ctypes.c_double(beta)
+    st = lib().momms_hip_dgemm_batched_loop(ctypes.cast(d, _c_vp), len(d), ctypes.byref(al),
+                                            A.data_ptr(), B.data_ptr(), ctypes.byref(be),
+                                            C.data_ptr(), stream_handle(stream))
+    check("momms_hip_dgemm_batched_loop", st)
+
+
+def uneven_batch(count: int, lo: int = 16, hi: int = 512, seed: int = 0x6D6F6D6D73):
+    """The uneven batch workload: m, n, k each floor(exp(U[ln lo, ln(hi+1)))) clamped to
+    [lo, hi], packed back to back column-major. Returns (rows, a_len, b_len, c_len)."""
+    import numpy as np
+
+    rng = np.random.default_rng(seed)
+    dims = np.floor(np.exp(rng.uniform(np.log(lo), np.log(hi + 1), (count, 3)))).astype(np.int64)
+    dims = np.clip(dims, lo, hi)
+    rows = []
+    ao = bo = co = 0
+    for m, n, k in dims:
+        rows.append((m, n, k, ao, bo, co, m, k, m))
+        ao += m * k
+        bo += k * n
+        co += m * n
+    return rows, ao, bo, co
 
 
 def has(name: str) -> bool:

@@ -1,0 +1,727 @@
+// Thousands of independent GEMMs of uneven shape in one launch.
+//
+// No reference counterpart: the batching addition of the engine. A batch is a
+// table of column-major problems (momms_gemm_desc: m, n, k, element offsets
+// into shared A/B/C arenas, leading dimensions). A plan, built once per table,
+//   - sorts every problem into a size bucket (by max(m, n)) and gives each
+//     bucket a tile shape (plain FP64) chosen by timing the candidates on the
+//     batch itself (the autotuner), or a slice count (split-BF16) chosen as
+//     the smallest one whose sampled error stays within 2x plain FP64's;
+//   - flattens the work into tile items, largest k first (longest job first
+//     keeps the tail of the queue short);
+//   - runs persistent workgroups that pull items from a device counter until
+//     the queue is empty: one workgroup per CU slot, no per-problem launch.
+// The whole sequence (counter reset by a kernel, slicing for the split mode,
+// the persistent multiplies) can be captured once into a hipGraph and then
+// replayed with one hipGraphLaunch.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "dgemm.hpp"
+#include "dgemm_tile.hpp"
+#include "mfma.hpp"
+#include "momms_hip.h"
+#include "split_bf16.hpp"
+#include "split_tile.hpp"
+#include "status.hpp"
+
+namespace momms {
+
+struct BatchItem {
+  int32_t prob, tm, tn, pad;
+};
+
+// Per-problem workspace of the split mode (byte offsets into the arena).
+struct SplitProb {
+  int64_t aq, bq, ea, eb;  // byte offsets
+  int32_t mpad, npad, kpad, pad;
+};
+
+// Slicing task: 64 vectors (rows of A: kind 0, columns of B: kind 1) of one
+// problem, and for the slice pass one 64-wide block of k.
+struct SliceTask {
+  int32_t prob, kind, vblock, pblock;
+};
+
+constexpr int kBatchPanel = 64;  // split mode: 64 x 64 tiles, panels of 64
+
+__global__ void reset_counters(unsigned* c, int n) {
+  if ((int)threadIdx.x < n) c[threadIdx.x] = 0u;
+}
+
+// Pulls the next item index for the whole workgroup.
+__device__ __forceinline__ int next_item(unsigned* counter) {
+  __shared__ int s_item;
+  if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
+  __syncthreads();
+  const int it = s_item;
+  __syncthreads();
+  return it;
+}
+
+template <int BM, int BN>
+__global__ void __launch_bounds__(256)
+batched_dgemm_kernel(const momms_gemm_desc* __restrict__ d, const BatchItem* __restrict__ items,
+                     int nitems, unsigned* counter, double alpha, const double* __restrict__ A,
+                     const double* __restrict__ B, double beta, double* __restrict__ C) {
+  for (;;) {
+    const int it = next_item(counter);
+    if (it >= nitems) return;
+    const BatchItem w = items[it];
+    const momms_gemm_desc p = d[w.prob];
+    dgemm_tile<BM, BN, 16, 2, 2, kCol, kCol>(p.m, p.n, p.k, alpha, A + p.a_off, 1, p.lda,
+                                             B + p.b_off, 1, p.ldb, beta, C + p.c_off, 1, p.ldc,
+                                             (int64_t)w.tm * BM, (int64_t)w.tn * BN);
+  }
+}
+
+// Exponents of 64 vectors per task: 4 threads per vector share its k range.
+__global__ void __launch_bounds__(256)
+batched_exponents(const momms_gemm_desc* __restrict__ d, const SplitProb* __restrict__ sp,
+                  const SliceTask* __restrict__ tasks, const double* __restrict__ A,
+                  const double* __restrict__ B, char* __restrict__ ws) {
+  __shared__ uint64_t part[4][64];
+  const SliceTask t = tasks[blockIdx.x];
+  const momms_gemm_desc p = d[t.prob];
+  const int tid = threadIdx.x;
+  // kind 0 (rows of A): adjacent rows are adjacent in memory -> vector = tid % 64
+  // kind 1 (columns of B): k is contiguous -> 4 adjacent lanes per vector
+  const int vv = t.kind == 0 ? (tid & 63) : (tid >> 2);
+  const int q = t.kind == 0 ? (tid >> 6) : (tid & 3);
+  const int64_t v = (int64_t)t.vblock * 64 + vv;
+  const int64_t count = t.kind == 0 ? p.m : p.n;
+  uint64_t best = 0;
+  if (v < count) {
+    const double* x = t.kind == 0 ? A + p.a_off + v : B + p.b_off + v * p.ldb;
+    const int64_t se = t.kind == 0 ? p.lda : 1;
+    for (int64_t e = q; e < p.k; e += 4) {
+      const uint64_t b = (uint64_t)__double_as_longlong(x[e * se]) & 0x7fffffffffffffffull;
+      best = b > best ? b : best;
+    }
+  }
+  part[q][vv] = best;
+  __syncthreads();
+  if (tid < 64) {
+    uint64_t b = part[0][tid];
+    for (int i = 1; i < 4; ++i) b = part[i][tid] > b ? part[i][tid] : b;
+    const int64_t w = (int64_t)t.vblock * 64 + tid;
+    const SplitProb s = sp[t.prob];
+    int* ex = reinterpret_cast<int*>(ws + (t.kind == 0 ? s.ea : s.eb));
+    const int64_t vpad = t.kind == 0 ? s.mpad : s.npad;
+    if (w < vpad) {
+      int E = 0;
+      if (w < count) {
+        if (b >= 0x7ff0000000000000ull) E = kNonFinite;
+        else if (b != 0) frexp(__longlong_as_double((long long)b), &E);
+      }
+      ex[w] = E;
+    }
+  }
+}
+
+template <int S>
+__global__ void __launch_bounds__(256)
+batched_slice(const momms_gemm_desc* __restrict__ d, const SplitProb* __restrict__ sp,
+              const SliceTask* __restrict__ tasks, const double* __restrict__ A,
+              const double* __restrict__ B, char* __restrict__ ws) {
+  const SliceTask t = tasks[blockIdx.x];
+  const momms_gemm_desc p = d[t.prob];
+  const SplitProb s = sp[t.prob];
+  if (t.kind == 0) {
+    slice_block<S, kBatchPanel>(A + p.a_off, p.m, p.k, 1, p.lda,
+                                reinterpret_cast<const int*>(ws + s.ea), s.mpad, s.kpad,
+                                reinterpret_cast<uint16_t*>(ws + s.aq), (int64_t)t.vblock * 64,
+                                (int64_t)t.pblock * 64);
+  } else {
+    slice_block<S, kBatchPanel>(B + p.b_off, p.n, p.k, p.ldb, 1,
+                                reinterpret_cast<const int*>(ws + s.eb), s.npad, s.kpad,
+                                reinterpret_cast<uint16_t*>(ws + s.bq), (int64_t)t.vblock * 64,
+                                (int64_t)t.pblock * 64);
+  }
+}
+
+template <int S>
+__global__ void __launch_bounds__(256)
+batched_split_kernel(const momms_gemm_desc* __restrict__ d, const SplitProb* __restrict__ sp,
+                     const BatchItem* __restrict__ items, int nitems, unsigned* counter,
+                     double alpha, double beta, double* __restrict__ C,
+                     const char* __restrict__ ws, int slabs_per_chunk) {
+  for (;;) {
+    const int it = next_item(counter);
+    if (it >= nitems) return;
+    const BatchItem w = items[it];
+    const momms_gemm_desc p = d[w.prob];
+    const SplitProb s = sp[w.prob];
+    split_tile<S, 2, 2>(p.m, p.n, p.k > 0 ? s.kpad : 0, alpha, beta, C + p.c_off, 1, p.ldc,
+                        reinterpret_cast<const uint16_t*>(ws + s.aq),
+                        reinterpret_cast<const uint16_t*>(ws + s.bq),
+                        reinterpret_cast<const int*>(ws + s.ea),
+                        reinterpret_cast<const int*>(ws + s.eb), s.mpad / kBatchPanel,
+                        s.npad / kBatchPanel, slabs_per_chunk, w.tm, w.tn);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+// Tile shapes of the plain mode (BM = BN).
+constexpr int kTileCfgs = 3;
+static const int kTileEdge[kTileCfgs] = {32, 64, 128};
+constexpr int kBuckets = 5;
+static int bucket_of(const momms_gemm_desc& p) {
+  const int64_t e = std::max(p.m, p.n);
+  if (e <= 48) return 0;
+  if (e <= 96) return 1;
+  if (e <= 192) return 2;
+  if (e <= 384) return 3;
+  return 4;
+}
+static const int kDefaultTile[kBuckets] = {0, 1, 1, 2, 2};
+
+// Tuned tile per bucket, remembered for the process (keyed by bucket and a
+// coarse k class), so plans for similar batches reuse the measurement.
+static std::mutex g_tune_mu;
+static std::map<int, int> g_tuned;
+
+static int occupancy(const void* kernel, int threads, int cus) {
+  int per = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess || per < 1)
+    per = 1;
+  return per * cus;
+}
+
+struct Launchable {
+  int cfg;  // plain: tile config; split: unused
+  int nitems;
+  BatchItem* items;
+  int grid;
+};
+
+}  // namespace momms
+
+struct momms_batch_plan {
+  int mode = 0, slices = 0;
+  int64_t count = 0;
+  int cus = 256;
+  std::vector<momms_gemm_desc> descs;
+  momms_gemm_desc* d_desc = nullptr;
+  unsigned* d_counters = nullptr;
+  std::vector<momms::Launchable> classes;  // plain mode, one per used tile config
+  // split mode
+  std::vector<momms::SplitProb> sp;
+  momms::SplitProb* d_sp = nullptr;
+  char* d_ws = nullptr;
+  size_t ws_bytes = 0;
+  momms::SliceTask* d_exp_tasks = nullptr;
+  int n_exp_tasks = 0;
+  momms::SliceTask* d_slice_tasks = nullptr;
+  int n_slice_tasks = 0;
+  momms::Launchable mul{};
+  // graph
+  hipStream_t cap_stream = nullptr;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  std::string info;
+  int64_t total_tiles = 0;
+  double flops = 0;
+};
+
+namespace momms {
+
+static hipError_t upload(void** dst, const void* src, size_t bytes) {
+  hipError_t e = hipMalloc(dst, std::max<size_t>(bytes, 16));
+  if (e != hipSuccess) return e;
+  if (bytes) e = hipMemcpy(*dst, src, bytes, hipMemcpyHostToDevice);
+  return e;
+}
+
+static std::vector<BatchItem> tile_items(const std::vector<momms_gemm_desc>& d,
+                                         const std::vector<int>& probs, int edge) {
+  std::vector<std::pair<int64_t, BatchItem>> v;
+  for (int p : probs) {
+    const int64_t tm = ceil_div(d[p].m, edge), tn = ceil_div(d[p].n, edge);
+    for (int64_t i = 0; i < tm; ++i)
+      for (int64_t j = 0; j < tn; ++j) v.push_back({d[p].k, BatchItem{p, (int32_t)i, (int32_t)j, 0}});
+  }
+  std::stable_sort(v.begin(), v.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+  std::vector<BatchItem> out;
+  out.reserve(v.size());
+  for (auto& x : v) out.push_back(x.second);
+  return out;
+}
+
+static hipError_t launch_plain_class(const momms_batch_plan* P, const Launchable& L, int slot,
+                                     double alpha, const double* A, const double* B, double beta,
+                                     double* C, hipStream_t st) {
+  dim3 block(256), grid((unsigned)L.grid);
+  unsigned* ctr = P->d_counters + slot;
+  switch (L.cfg) {
+    case 0:
+      hipLaunchKernelGGL((batched_dgemm_kernel<32, 32>), grid, block, 0, st, P->d_desc, L.items,
+                         L.nitems, ctr, alpha, A, B, beta, C);
+      break;
+    case 1:
+      hipLaunchKernelGGL((batched_dgemm_kernel<64, 64>), grid, block, 0, st, P->d_desc, L.items,
+                         L.nitems, ctr, alpha, A, B, beta, C);
+      break;
+    default:
+      hipLaunchKernelGGL((batched_dgemm_kernel<128, 128>), grid, block, 0, st, P->d_desc, L.items,
+                         L.nitems, ctr, alpha, A, B, beta, C);
+      break;
+  }
+  return hipGetLastError();
+}
+
+static const void* plain_kernel_ptr(int cfg) {
+  switch (cfg) {
+    case 0: return reinterpret_cast<const void*>(&batched_dgemm_kernel<32, 32>);
+    case 1: return reinterpret_cast<const void*>(&batched_dgemm_kernel<64, 64>);
+    default: return reinterpret_cast<const void*>(&batched_dgemm_kernel<128, 128>);
+  }
+}
+
+static Launchable make_class(const std::vector<momms_gemm_desc>& d, const std::vector<int>& probs,
+                             int cfg, int cus, hipError_t& err) {
+  Launchable L{cfg, 0, nullptr, 0};
+  auto items = tile_items(d, probs, kTileEdge[cfg]);
+  L.nitems = (int)items.size();
+  err = upload((void**)&L.items, items.data(), items.size() * sizeof(BatchItem));
+  const int slots = occupancy(plain_kernel_ptr(cfg), 256, cus);
+  L.grid = std::max(1, std::min(L.nitems, slots));
+  return L;
+}
+
+#define MOMMS_SPLIT_SWITCH(S_, CALL) \
+  switch (S_) {                      \
+    case 1: CALL(1); break;          \
+    case 2: CALL(2); break;          \
+    case 3: CALL(3); break;          \
+    case 4: CALL(4); break;          \
+    case 5: CALL(5); break;          \
+    case 6: CALL(6); break;          \
+    case 7: CALL(7); break;          \
+    default: CALL(8); break;         \
+  }
+
+static hipError_t enqueue(const momms_batch_plan* P, double alpha, const double* A,
+                          const double* B, double beta, double* C, hipStream_t st) {
+  const int nctr = P->mode == 0 ? (int)P->classes.size() : 1;
+  hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, st, P->d_counters, nctr);
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return e;
+  if (P->mode == 0) {
+    for (size_t c = 0; c < P->classes.size(); ++c) {
+      if (!P->classes[c].nitems) continue;
+      if ((e = launch_plain_class(P, P->classes[c], (int)c, alpha, A, B, beta, C, st)) != hipSuccess)
+        return e;
+    }
+    return hipSuccess;
+  }
+  if (P->n_exp_tasks) {
+    hipLaunchKernelGGL(batched_exponents, dim3((unsigned)P->n_exp_tasks), dim3(256), 0, st,
+                       P->d_desc, P->d_sp, P->d_exp_tasks, A, B, P->d_ws);
+  }
+  if (P->n_slice_tasks) {
+#define MOMMS_BS(S) hipLaunchKernelGGL((batched_slice<S>), dim3((unsigned)P->n_slice_tasks), dim3(256), 0, st, P->d_desc, P->d_sp, P->d_slice_tasks, A, B, P->d_ws)
+    MOMMS_SPLIT_SWITCH(P->slices, MOMMS_BS)
+#undef MOMMS_BS
+  }
+  if (P->mul.nitems) {
+    const int spc = split_chunk(P->slices) / kSlab;
+#define MOMMS_BM(S) hipLaunchKernelGGL((batched_split_kernel<S>), dim3((unsigned)P->mul.grid), dim3(256), 0, st, P->d_desc, P->d_sp, P->mul.items, P->mul.nitems, P->d_counters, alpha, beta, C, P->d_ws, spc)
+    MOMMS_SPLIT_SWITCH(P->slices, MOMMS_BM)
+#undef MOMMS_BM
+  }
+  return hipGetLastError();
+}
+
+static const void* split_kernel_ptr(int s) {
+#define MOMMS_KP(S) return reinterpret_cast<const void*>(&batched_split_kernel<S>)
+  MOMMS_SPLIT_SWITCH(s, MOMMS_KP)
+#undef MOMMS_KP
+  return nullptr;
+}
+
+static void free_plan(momms_batch_plan* P) {
+  if (!P) return;
+  for (auto& c : P->classes) (void)hipFree(c.items);
+  (void)hipFree(P->mul.items);
+  (void)hipFree(P->d_desc);
+  (void)hipFree(P->d_counters);
+  (void)hipFree(P->d_sp);
+  (void)hipFree(P->d_ws);
+  (void)hipFree(P->d_exp_tasks);
+  (void)hipFree(P->d_slice_tasks);
+  if (P->exec) (void)hipGraphExecDestroy(P->exec);
+  if (P->graph) (void)hipGraphDestroy(P->graph);
+  if (P->cap_stream) (void)hipStreamDestroy(P->cap_stream);
+  delete P;
+}
+
+// Builds the split-mode tables for slice count s (replacing earlier ones).
+static hipError_t build_split(momms_batch_plan* P, int s) {
+  (void)hipFree(P->d_sp);
+  (void)hipFree(P->d_ws);
+  (void)hipFree(P->d_exp_tasks);
+  (void)hipFree(P->d_slice_tasks);
+  (void)hipFree(P->mul.items);
+  P->d_sp = nullptr;
+  P->d_ws = nullptr;
+  P->d_exp_tasks = P->d_slice_tasks = nullptr;
+  P->mul = Launchable{};
+  P->slices = s;
+  const auto& d = P->descs;
+  P->sp.assign(d.size(), SplitProb{});
+  size_t off = 0;
+  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+  std::vector<SliceTask> et, st;
+  std::vector<int> probs;
+  for (size_t i = 0; i < d.size(); ++i) {
+    SplitProb& q = P->sp[i];
+    q.mpad = (int32_t)(ceil_div(d[i].m, kBatchPanel) * kBatchPanel);
+    q.npad = (int32_t)(ceil_div(d[i].n, kBatchPanel) * kBatchPanel);
+    q.kpad = (int32_t)(ceil_div(std::max<int64_t>(d[i].k, 1), kSlab) * kSlab);
+    q.aq = (int64_t)off;
+    off = al(off + (size_t)s * q.mpad * q.kpad * 2);
+    q.bq = (int64_t)off;
+    off = al(off + (size_t)s * q.npad * q.kpad * 2);
+    q.ea = (int64_t)off;
+    off = al(off + (size_t)q.mpad * 4);
+    q.eb = (int64_t)off;
+    off = al(off + (size_t)q.npad * 4);
+    if (d[i].m == 0 || d[i].n == 0) continue;
+    probs.push_back((int)i);
+    for (int kind = 0; kind < 2; ++kind) {
+      const int vb = (kind == 0 ? q.mpad : q.npad) / 64;
+      const int pb = (int)ceil_div(q.kpad, 64);
+      for (int v = 0; v < vb; ++v) {
+        et.push_back({(int32_t)i, kind, v, 0});
+        for (int pq = 0; pq < pb; ++pq) st.push_back({(int32_t)i, kind, v, pq});
+      }
+    }
+  }
+  P->ws_bytes = std::max<size_t>(off, 256);
+  hipError_t e;
+  if ((e = hipMalloc((void**)&P->d_ws, P->ws_bytes)) != hipSuccess) return e;
+  if ((e = upload((void**)&P->d_sp, P->sp.data(), P->sp.size() * sizeof(SplitProb))) != hipSuccess)
+    return e;
+  if ((e = upload((void**)&P->d_exp_tasks, et.data(), et.size() * sizeof(SliceTask))) != hipSuccess)
+    return e;
+  if ((e = upload((void**)&P->d_slice_tasks, st.data(), st.size() * sizeof(SliceTask))) != hipSuccess)
+    return e;
+  P->n_exp_tasks = (int)et.size();
+  P->n_slice_tasks = (int)st.size();
+  auto items = tile_items(d, probs, kBatchPanel);
+  P->mul.nitems = (int)items.size();
+  if ((e = upload((void**)&P->mul.items, items.data(), items.size() * sizeof(BatchItem))) != hipSuccess)
+    return e;
+  P->mul.grid = std::max(1, std::min(P->mul.nitems, occupancy(split_kernel_ptr(s), 256, P->cus)));
+  return hipSuccess;
+}
+
+static float time_enqueue(momms_batch_plan* P, const double* A, const double* B, double* C,
+                          hipStream_t st, int reps) {
+  hipEvent_t e0, e1;
+  (void)hipEventCreate(&e0);
+  (void)hipEventCreate(&e1);
+  float best = 1e30f;
+  for (int r = 0; r < reps; ++r) {
+    (void)hipEventRecord(e0, st);
+    (void)enqueue(P, 1.0, A, B, 0.0, C, st);
+    (void)hipEventRecord(e1, st);
+    (void)hipEventSynchronize(e1);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, e0, e1);
+    best = std::min(best, ms);
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  return best;
+}
+
+static size_t c_extent(const std::vector<momms_gemm_desc>& d) {
+  size_t e = 0;
+  for (auto& p : d)
+    if (p.m && p.n) e = std::max(e, (size_t)(p.c_off + (p.n - 1) * p.ldc + p.m));
+  return e;
+}
+
+}  // namespace momms
+
+using namespace momms;
+
+extern "C" int momms_hip_batch_plan_create(const momms_gemm_desc* descs, int64_t count, int mode,
+                                           int slices, int autotune, const double* A,
+                                           const double* B, momms_batch_plan** out,
+                                           void* stream) {
+  static const char* fn = "momms_hip_batch_plan_create";
+  if (descs == nullptr && count > 0) return bad_arg(fn, 1, "descs is null");
+  if (count < 0 || count > (1 << 24)) return bad_arg(fn, 2, "count out of range");
+  if (mode != 0 && mode != 1) return bad_arg(fn, 3, "mode must be 0 (FP64) or 1 (split-BF16)");
+  if (mode == 1 && (slices < 0 || slices > kMaxSlices))
+    return bad_arg(fn, 4, "slices must be 0 (tuned) or 1..8");
+  if (autotune && (A == nullptr || B == nullptr)) return bad_arg(fn, 6, "autotune needs A and B");
+  if (out == nullptr) return bad_arg(fn, 8, "out is null");
+  for (int64_t i = 0; i < count; ++i) {
+    const auto& p = descs[i];
+    if (p.m < 0 || p.n < 0 || p.k < 0 || p.m > (1 << 24) || p.n > (1 << 24) ||
+        p.a_off < 0 || p.b_off < 0 || p.c_off < 0 || p.lda < std::max<int64_t>(p.m, 1) ||
+        p.ldb < std::max<int64_t>(p.k, 1) || p.ldc < std::max<int64_t>(p.m, 1)) {
+      char buf[160];
+      std::snprintf(buf, sizeof(buf), "descs[%lld] is invalid", (long long)i);
+      return bad_arg(fn, 1, buf);
+    }
+  }
+  hipStream_t st = (hipStream_t)stream;
+  auto* P = new momms_batch_plan();
+  P->mode = mode;
+  P->count = count;
+  P->descs.assign(descs, descs + count);
+  int dev = 0;
+  (void)hipGetDevice(&dev);
+  (void)hipDeviceGetAttribute(&P->cus, hipDeviceAttributeMultiprocessorCount, dev);
+  for (auto& p : P->descs) P->flops += 2.0 * (double)p.m * (double)p.n * (double)p.k;
+  hipError_t e;
+#define MOMMS_FAIL(E)        \
+  do {                       \
+    free_plan(P);            \
+    return hip_status(fn, E); \
+  } while (0)
+  if ((e = upload((void**)&P->d_desc, P->descs.data(), P->descs.size() * sizeof(momms_gemm_desc))) != hipSuccess)
+    MOMMS_FAIL(e);
+  if ((e = hipMalloc((void**)&P->d_counters, 64 * sizeof(unsigned))) != hipSuccess) MOMMS_FAIL(e);
+
+  double* scratchC = nullptr;
+  const size_t cext = c_extent(P->descs);
+  if (autotune && cext) {
+    if ((e = hipMalloc((void**)&scratchC, cext * sizeof(double))) != hipSuccess) MOMMS_FAIL(e);
+  }
+
+  char line[256];
+  if (mode == 0) {
+    std::vector<int> bucket_probs[kBuckets];
+    for (int64_t i = 0; i < count; ++i)
+      if (P->descs[i].m && P->descs[i].n) bucket_probs[bucket_of(P->descs[i])].push_back((int)i);
+    int choice[kBuckets];
+    for (int b = 0; b < kBuckets; ++b) {
+      choice[b] = kDefaultTile[b];
+      if (bucket_probs[b].empty()) continue;
+      int64_t ksum = 0;
+      for (int p : bucket_probs[b]) ksum += P->descs[p].k;
+      const int kcls = (int)std::min<int64_t>(8, (int64_t)std::log2(1.0 + (double)ksum / bucket_probs[b].size()));
+      const int key = b * 16 + kcls;
+      {
+        std::lock_guard<std::mutex> g(g_tune_mu);
+        auto it = g_tuned.find(key);
+        if (it != g_tuned.end()) {
+          choice[b] = it->second;
+          continue;
+        }
+      }
+      if (!autotune) continue;
+      float best = 1e30f;
+      for (int cfg = 0; cfg < kTileCfgs; ++cfg) {
+        momms_batch_plan T;
+        T.mode = 0;
+        T.cus = P->cus;
+        T.d_desc = P->d_desc;
+        T.d_counters = P->d_counters;
+        T.classes.push_back(make_class(P->descs, bucket_probs[b], cfg, P->cus, e));
+        if (e != hipSuccess) {
+          (void)hipFree(T.classes[0].items);
+          T.d_desc = nullptr;
+          T.d_counters = nullptr;
+          MOMMS_FAIL(e);
+        }
+        (void)time_enqueue(&T, A, B, scratchC, st, 1);  // warm up
+        const float ms = time_enqueue(&T, A, B, scratchC, st, 3);
+        if (ms < best) {
+          best = ms;
+          choice[b] = cfg;
+        }
+        (void)hipFree(T.classes[0].items);
+        T.classes.clear();
+        T.d_desc = nullptr;  // owned by P
+        T.d_counters = nullptr;
+      }
+      std::lock_guard<std::mutex> g(g_tune_mu);
+      g_tuned[key] = choice[b];
+    }
+    // one launch class per tile config in use, largest tile first
+    for (int cfg = kTileCfgs - 1; cfg >= 0; --cfg) {
+      std::vector<int> probs;
+      for (int b = 0; b < kBuckets; ++b)
+        if (choice[b] == cfg) probs.insert(probs.end(), bucket_probs[b].begin(), bucket_probs[b].end());
+      if (probs.empty()) continue;
+      P->classes.push_back(make_class(P->descs, probs, cfg, P->cus, e));
+      if (e != hipSuccess) MOMMS_FAIL(e);
+      P->total_tiles += P->classes.back().nitems;
+    }
+    P->info = "{\"mode\":\"fp64\",\"buckets\":[";
+    for (int b = 0; b < kBuckets; ++b) {
+      std::snprintf(line, sizeof(line), "%s{\"bucket\":%d,\"problems\":%zu,\"tile\":%d}", b ? "," : "",
+                    b, bucket_probs[b].size(), kTileEdge[choice[b]]);
+      P->info += line;
+    }
+    P->info += "]}";
+  } else {
+    int s = slices;
+    if (s == 0 && autotune) {
+      // smallest slice count whose sampled error is within 2x plain FP64's
+      if ((e = build_split(P, 8)) != hipSuccess) MOMMS_FAIL(e);
+      momms_batch_plan T;
+      T.mode = 0;
+      T.cus = P->cus;
+      T.d_desc = P->d_desc;
+      T.d_counters = P->d_counters;
+      std::vector<int> all;
+      for (int64_t i = 0; i < count; ++i)
+        if (P->descs[i].m && P->descs[i].n) all.push_back((int)i);
+      T.classes.push_back(make_class(P->descs, all, 1, P->cus, e));
+      double* plainC = nullptr;
+      if (e == hipSuccess) e = hipMalloc((void**)&plainC, std::max<size_t>(cext, 1) * sizeof(double));
+      if (e == hipSuccess) e = enqueue(&T, 1.0, A, B, 0.0, plainC, st);
+      if (e == hipSuccess) e = hipStreamSynchronize(st);
+      double plain_err = 0;
+      const int sample_probs = (int)std::min<size_t>(all.size(), 64);
+      auto max_err = [&](const double* Cm) {
+        double worst = 0;
+        for (int q = 0; q < sample_probs; ++q) {
+          const auto& p = P->descs[all[(size_t)q * all.size() / sample_probs]];
+          const double one = 1.0, zero = 0.0;
+          double ms = 0, mr = 0;
+          if (momms_hip_gemm_check_sampled(p.m, p.n, p.k, &one, A + p.a_off, 1, p.lda, B + p.b_off,
+                                           1, p.ldb, &zero, nullptr, Cm + p.c_off, 1, p.ldc, 64,
+                                           0x6D6F6D6D73ull + q, nullptr, &ms, &mr, st) == 0)
+            worst = std::max(worst, ms);
+        }
+        return worst;
+      };
+      if (e == hipSuccess) plain_err = max_err(plainC);
+      s = 8;
+      for (int cand = 4; cand <= 8 && e == hipSuccess; ++cand) {
+        if ((e = build_split(P, cand)) != hipSuccess) break;
+        if ((e = enqueue(P, 1.0, A, B, 0.0, scratchC, st)) != hipSuccess) break;
+        if ((e = hipStreamSynchronize(st)) != hipSuccess) break;
+        if (max_err(scratchC) <= 2.0 * plain_err) {
+          s = cand;
+          break;
+        }
+      }
+      (void)hipFree(T.classes[0].items);
+      T.classes.clear();
+      T.d_desc = nullptr;
+      T.d_counters = nullptr;
+      (void)hipFree(plainC);
+      if (e != hipSuccess) {
+        (void)hipFree(scratchC);
+        MOMMS_FAIL(e);
+      }
+    } else if (s == 0) {
+      s = 7;
+    }
+    if ((e = build_split(P, s)) != hipSuccess) {
+      (void)hipFree(scratchC);
+      MOMMS_FAIL(e);
+    }
+    P->total_tiles = P->mul.nitems;
+    std::snprintf(line, sizeof(line),
+                  "{\"mode\":\"split_bf16\",\"slices\":%d,\"tile\":%d,\"tiles\":%d,\"workspace_bytes\":%zu}",
+                  s, kBatchPanel, P->mul.nitems, P->ws_bytes);
+    P->info = line;
+  }
+  (void)hipFree(scratchC);
+  if ((e = hipStreamCreate(&P->cap_stream)) != hipSuccess) MOMMS_FAIL(e);
+#undef MOMMS_FAIL
+  *out = P;
+  return 0;
+}
+
+extern "C" int momms_hip_batch_run(momms_batch_plan* plan, const double* alpha, const double* A,
+                                   const double* B, const double* beta, double* C, void* stream) {
+  static const char* fn = "momms_hip_batch_run";
+  if (!plan) return bad_arg(fn, 1, "plan is null");
+  if (!alpha) return bad_arg(fn, 2, "alpha is null");
+  if (!A && plan->count) return bad_arg(fn, 3, "A is null");
+  if (!B && plan->count) return bad_arg(fn, 4, "B is null");
+  if (!beta) return bad_arg(fn, 5, "beta is null");
+  if (!C && plan->count) return bad_arg(fn, 6, "C is null");
+  return hip_status(fn, enqueue(plan, *alpha, A, B, *beta, C, (hipStream_t)stream));
+}
+
+extern "C" int momms_hip_batch_capture(momms_batch_plan* plan, const double* alpha,
+                                       const double* A, const double* B, const double* beta,
+                                       double* C) {
+  static const char* fn = "momms_hip_batch_capture";
+  if (!plan) return bad_arg(fn, 1, "plan is null");
+  if (!alpha) return bad_arg(fn, 2, "alpha is null");
+  if (!A && plan->count) return bad_arg(fn, 3, "A is null");
+  if (!B && plan->count) return bad_arg(fn, 4, "B is null");
+  if (!beta) return bad_arg(fn, 5, "beta is null");
+  if (!C && plan->count) return bad_arg(fn, 6, "C is null");
+  if (plan->exec) {
+    (void)hipGraphExecDestroy(plan->exec);
+    plan->exec = nullptr;
+  }
+  if (plan->graph) {
+    (void)hipGraphDestroy(plan->graph);
+    plan->graph = nullptr;
+  }
+  MOMMS_HIP_TRY(fn, hipStreamBeginCapture(plan->cap_stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = enqueue(plan, *alpha, A, B, *beta, C, plan->cap_stream);
+  hipGraph_t g = nullptr;
+  hipError_t e2 = hipStreamEndCapture(plan->cap_stream, &g);
+  if (e != hipSuccess) {
+    if (g) (void)hipGraphDestroy(g);
+    return hip_status(fn, e);
+  }
+  if (e2 != hipSuccess) return hip_status(fn, e2);
+  plan->graph = g;
+  MOMMS_HIP_TRY(fn, hipGraphInstantiate(&plan->exec, g, nullptr, nullptr, 0));
+  return 0;
+}
+
+extern "C" int momms_hip_batch_launch(momms_batch_plan* plan, void* stream) {
+  static const char* fn = "momms_hip_batch_launch";
+  if (!plan) return bad_arg(fn, 1, "plan is null");
+  if (!plan->exec) {
+    set_error("momms_hip_batch_launch: plan has no captured graph (call momms_hip_batch_capture)");
+    return -1;
+  }
+  return hip_status(fn, hipGraphLaunch(plan->exec, (hipStream_t)stream));
+}
+
+extern "C" const char* momms_hip_batch_plan_describe(const momms_batch_plan* plan) {
+  return plan ? plan->info.c_str() : "";
+}
+
+extern "C" int momms_hip_batch_plan_destroy(momms_batch_plan* plan) {
+  free_plan(plan);
+  return 0;
+}
+
+extern "C" int momms_hip_dgemm_batched_loop(const momms_gemm_desc* descs, int64_t count,
+                                            const double* alpha, const double* A,
+                                            const double* B, const double* beta, double* C,
+                                            void* stream) {
+  static const char* fn = "momms_hip_dgemm_batched_loop";
+  if (descs == nullptr && count > 0) return bad_arg(fn, 1, "descs is null");
+  if (count < 0) return bad_arg(fn, 2, "count < 0");
+  if (!alpha) return bad_arg(fn, 3, "alpha is null");
+  if (!beta) return bad_arg(fn, 6, "beta is null");
+  for (int64_t i = 0; i < count; ++i) {
+    const auto& p = descs[i];
+    const int st = momms_hip_dgemm(p.m, p.n, p.k, alpha, A + p.a_off, 1, p.lda, B + p.b_off, 1,
+                                   p.ldb, beta, C + p.c_off, 1, p.ldc, stream);
+    if (st != 0) return st;
+  }
+  return 0;
+}

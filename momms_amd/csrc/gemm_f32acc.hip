@@ -1,0 +1,228 @@
+// Reduced-precision GEMMs with FP32 accumulation and FP32 output:
+//   momms_hip_sgemm       FP32 in  -> v_mfma_f32_32x32x2_f32
+//   momms_hip_gemm_bf16   BF16 in  -> v_mfma_f32_32x32x16_bf16
+// C := beta*C + alpha*A*B, C float. The FP32 form is the counterpart of the
+// reference's KNM sgemm leaf (KnmKernel -> sgemm_knm_asm_16x24,
+// src/kern/knm_kernel.rs:8-81, ukernels/sgemm_knm_asm_24x16.c:30-146).
+//
+// One template serves both: a slab of BK k-values is staged in LDS as
+// [ks][kh][row][V] where V is the elements per lane of one MFMA operand
+// (1 for f32, 8 for bf16) and kh = lane/32 picks the operand's k half, so
+// every lane reads its fragment with one contiguous load. Workgroup 256
+// threads (2 x 2 wavefronts), tile 128 x 128, each wavefront 64 x 64 as
+// 2 x 2 blocks of 32 x 32. Operands are swapped at the MFMA (B slab as the
+// "A" operand) so stores to column-major C are coalesced, as in dgemm.hip.
+#include "mfma.hpp"
+#include "status.hpp"
+
+namespace momms {
+
+template <class TIn>
+struct MfmaTraits;
+template <>
+struct MfmaTraits<float> {
+  static constexpr int V = 1;      // elements per lane per operand
+  static constexpr int KSTEP = 2;  // k per MFMA
+  typedef float frag;
+  __device__ static v16f32 mma(frag a, frag b, v16f32 c) { return mfma_f32_32x32x2(a, b, c); }
+};
+template <>
+struct MfmaTraits<__bf16> {
+  static constexpr int V = 8;
+  static constexpr int KSTEP = 16;
+  typedef v8bf16 frag;
+  __device__ static v16f32 mma(frag a, frag b, v16f32 c) { return mfma_bf16_32x32x16(a, b, c); }
+};
+
+template <class TIn, int BK, int LA, int LB>
+__global__ void __launch_bounds__(256)
+gemm_f32acc_kernel(int64_t m, int64_t n, int64_t k, float alpha, const TIn* __restrict__ A,
+                   int64_t rsA, int64_t csA, const TIn* __restrict__ B, int64_t rsB, int64_t csB,
+                   float beta, float* __restrict__ C, int64_t rsC, int64_t csC) {
+  typedef MfmaTraits<TIn> Tr;
+  constexpr int BM = 128, BN = 128, NT = 256;
+  constexpr int V = Tr::V;
+  constexpr int KS = BK / Tr::KSTEP;  // MFMA steps per slab
+  constexpr int A_PER = BM * BK / NT, B_PER = BN * BK / NT;
+  static_assert(BK % Tr::KSTEP == 0, "slab");
+  __shared__ __attribute__((aligned(16))) TIn As[2][BM * BK];
+  __shared__ __attribute__((aligned(16))) TIn Bs[2][BN * BK];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int64_t tiles_m = ceil_div(m, BM), tiles_n = ceil_div(n, BN);
+  const int64_t t = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
+  int64_t tm, tn;
+  grouped_tile(t, tiles_m, tiles_n, 8, tm, tn);
+  const int64_t row0 = tm * BM, col0 = tn * BN;
+  const bool m_full = row0 + BM <= m, n_full = col0 + BN <= n;
+  const int64_t ktiles = ceil_div(k, BK);
+
+  // LDS offset of slab element (r, kk): [ks][kh][r][e]
+  auto lds_off = [](int r, int kk, int rows) {
+    const int ks = kk / (2 * V), kh = (kk % (2 * V)) / V, e = kk % V;
+    return ((ks * 2 + kh) * rows + r) * V + e;
+  };
+
+  TIn ra[A_PER], rb[B_PER];
+  auto load_slab = [&](int64_t kt) {
+    const int64_t k0 = kt * BK;
+    const bool k_full = k0 + BK <= k;
+#pragma unroll
+    for (int p = 0; p < A_PER; ++p) {
+      const int e = tid + NT * p;
+      const int i = (LA == kRow) ? e / BK : e % BM;
+      const int kk = (LA == kRow) ? e % BK : e / BM;
+      const int64_t gi = row0 + i, gk = k0 + kk;
+      TIn v = TIn(0.0f);
+      if ((m_full && k_full) || (gi < m && gk < k)) v = A[gi * rsA + gk * csA];
+      ra[p] = v;
+    }
+#pragma unroll
+    for (int p = 0; p < B_PER; ++p) {
+      const int e = tid + NT * p;
+      const int j = (LB == kRow) ? e % BN : e / BK;
+      const int kk = (LB == kRow) ? e / BN : e % BK;
+      const int64_t gj = col0 + j, gk = k0 + kk;
+      TIn v = TIn(0.0f);
+      if ((n_full && k_full) || (gj < n && gk < k)) v = B[gk * rsB + gj * csB];
+      rb[p] = v;
+    }
+  };
+  auto store_slab = [&](int buf) {
+#pragma unroll
+    for (int p = 0; p < A_PER; ++p) {
+      const int e = tid + NT * p;
+      const int i = (LA == kRow) ? e / BK : e % BM;
+      const int kk = (LA == kRow) ? e % BK : e / BM;
+      As[buf][lds_off(i, kk, BM)] = ra[p];
+    }
+#pragma unroll
+    for (int p = 0; p < B_PER; ++p) {
+      const int e = tid + NT * p;
+      const int j = (LB == kRow) ? e % BN : e / BK;
+      const int kk = (LB == kRow) ? e / BN : e % BK;
+      Bs[buf][lds_off(j, kk, BN)] = rb[p];
+    }
+  };
+
+  v16f32 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
+
+  const int kh = lane >> 5, lr = lane & 31;
+  if (ktiles > 0) {
+    load_slab(0);
+    store_slab(0);
+  }
+  __syncthreads();
+  for (int64_t kt = 0; kt < ktiles; ++kt) {
+    const int cur = (int)(kt & 1);
+    if (kt + 1 < ktiles) load_slab(kt + 1);
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+      typename Tr::frag af[2], bf[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+        af[a] = *reinterpret_cast<const typename Tr::frag*>(
+            &As[cur][((ks * 2 + kh) * BM + wm * 64 + a * 32 + lr) * V]);
+#pragma unroll
+      for (int b = 0; b < 2; ++b)
+        bf[b] = *reinterpret_cast<const typename Tr::frag*>(
+            &Bs[cur][((ks * 2 + kh) * BN + wn * 64 + b * 32 + lr) * V]);
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = Tr::mma(bf[b], af[a], acc[a][b]);
+    }
+    if (kt + 1 < ktiles) store_slab(cur ^ 1);
+    __syncthreads();
+  }
+
+  // lane holds row i = l%32 of each block, columns 8(r/4) + 4(l/32) + r%4
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const int64_t gi = row0 + wm * 64 + a * 32 + lr;
+    if (gi >= m) continue;
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t gj = col0 + wn * 64 + b * 32 + 8 * (r >> 2) + 4 * kh + (r & 3);
+        if (gj < n) {
+          float* cp = C + gi * rsC + gj * csC;
+          const float s = alpha * acc[a][b][r];
+          *cp = (beta == 0.0f) ? s : __builtin_fmaf(beta, *cp, s);
+        }
+      }
+  }
+}
+
+static int layout_of(int64_t rs, int64_t cs, int64_t h, int64_t w) {
+  if (rs == 1 || w == 1) return kCol;
+  if (cs == 1 || h == 1) return kRow;
+  return kGen;
+}
+
+template <class TIn, int BK>
+static hipError_t launch_f32acc(int64_t m, int64_t n, int64_t k, float alpha, const TIn* A,
+                                int64_t rsA, int64_t csA, const TIn* B, int64_t rsB, int64_t csB,
+                                float beta, float* C, int64_t rsC, int64_t csC, hipStream_t st) {
+  const int la = layout_of(rsA, csA, m, k) == kRow ? kRow : kCol;
+  const int lb = layout_of(rsB, csB, k, n) == kRow ? kRow : kCol;
+  dim3 grid((unsigned)(ceil_div(m, 128) * ceil_div(n, 128))), block(256);
+#define MOMMS_F(LA_, LB_) hipLaunchKernelGGL((gemm_f32acc_kernel<TIn, BK, LA_, LB_>), grid, block, 0, st, m, n, k, alpha, A, rsA, csA, B, rsB, csB, beta, C, rsC, csC)
+  if (la == kCol && lb == kCol) MOMMS_F(kCol, kCol);
+  else if (la == kCol) MOMMS_F(kCol, kRow);
+  else if (lb == kCol) MOMMS_F(kRow, kCol);
+  else MOMMS_F(kRow, kRow);
+#undef MOMMS_F
+  return hipGetLastError();
+}
+
+}  // namespace momms
+
+using namespace momms;
+
+template <class T>
+static int f32acc_entry(const char* fn, int64_t m, int64_t n, int64_t k, const float* alpha,
+                        const T* A, int64_t rsA, int64_t csA, const T* B, int64_t rsB,
+                        int64_t csB, const float* beta, float* C, int64_t rsC, int64_t csC,
+                        void* stream) {
+  if (m < 0) return bad_arg(fn, 1, "m < 0");
+  if (n < 0) return bad_arg(fn, 2, "n < 0");
+  if (k < 0) return bad_arg(fn, 3, "k < 0");
+  if (alpha == nullptr) return bad_arg(fn, 4, "alpha is null");
+  int st;
+  if ((st = check_operand(fn, 5, A, m, k, rsA, csA))) return st;
+  if ((st = check_operand(fn, 8, B, k, n, rsB, csB))) return st;
+  if (beta == nullptr) return bad_arg(fn, 11, "beta is null");
+  if ((st = check_operand(fn, 12, C, m, n, rsC, csC))) return st;
+  if (m == 0 || n == 0) return 0;
+  constexpr int BK = std::is_same<T, float>::value ? 16 : 32;
+  return hip_status(fn, launch_f32acc<T, BK>(m, n, k, *alpha, A, rsA, csA, B, rsB, csB, *beta, C,
+                                             rsC, csC, (hipStream_t)stream));
+}
+
+extern "C" int momms_hip_sgemm(int64_t m, int64_t n, int64_t k, const float* alpha,
+                               const float* A, int64_t rsA, int64_t csA, const float* B,
+                               int64_t rsB, int64_t csB, const float* beta, float* C,
+                               int64_t rsC, int64_t csC, void* stream) {
+  return f32acc_entry<float>("momms_hip_sgemm", m, n, k, alpha, A, rsA, csA, B, rsB, csB, beta,
+                             C, rsC, csC, stream);
+}
+
+extern "C" int momms_hip_gemm_bf16(int64_t m, int64_t n, int64_t k, const float* alpha,
+                                   const uint16_t* A, int64_t rsA, int64_t csA,
+                                   const uint16_t* B, int64_t rsB, int64_t csB,
+                                   const float* beta, float* C, int64_t rsC, int64_t csC,
+                                   void* stream) {
+  return f32acc_entry<__bf16>("momms_hip_gemm_bf16", m, n, k, alpha,
+                              reinterpret_cast<const __bf16*>(A), rsA, csA,
+                              reinterpret_cast<const __bf16*>(B), rsB, csB, beta, C, rsC, csC,
+                              stream);
+}

@@ -27,14 +27,10 @@
 
 #include "mfma.hpp"
 #include "split_bf16.hpp"
+#include "split_tile.hpp"
 #include "status.hpp"
 
 namespace momms {
-
-constexpr int kPanelA = 128;  // rows of A per packed panel (= multiply tile BM)
-constexpr int kPanelB = 64;   // columns of B per packed panel (= multiply tile BN)
-constexpr int kSlab = 16;     // k per slab (= one v_mfma_f32_32x32x16_bf16 step)
-constexpr int kMaxSlices = 8;
 
 int split_chunk(int s) {
   int kch = 16 * (1024 / (16 * (s + 2)));
@@ -111,9 +107,8 @@ __global__ void absmax_contig(const double* __restrict__ X, int64_t count, int64
   if (lane == 0) mx[v] = best;
 }
 
-// Exponent E with max < 2^E (frexp convention); 0 for an all-zero vector.
-// A vector holding Inf or NaN gets the flag value kNonFinite.
-constexpr int kNonFinite = 0x7fffffff;
+// Exponent E with max < 2^E (frexp convention); 0 for an all-zero vector;
+// kNonFinite for a vector holding Inf or NaN.
 __global__ void exponents(const uint64_t* __restrict__ mx, int64_t count, int64_t padded,
                           int* __restrict__ e) {
   const int64_t v = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -134,96 +129,13 @@ __global__ void exponents(const uint64_t* __restrict__ mx, int64_t count, int64_
   }
 }
 
-// ---------------------------------------------------------------------------
-// 2. slicing into packed panels.
-//
-// Vector v (a row of A or a column of B), element p lives at
-// X[v*sv + p*sp]. Output for slice t: panel pv = v / PANEL, slab ks = p / 16,
-//   Q[((t*npanels + pv)*nslabs + ks)*(PANEL*16) + kh*PANEL*8 + (v%PANEL)*8 + p%8]
-// with kh = (p % 16) / 8. Padding (v >= count or p >= len) is written as 0.
-// A 64 x 64 tile of (v, p) goes through LDS so that both the global read
-// (along whichever index is unit-stride) and the packed write are coalesced.
-
 template <int S, int PANEL>
 __global__ void __launch_bounds__(256)
 slice_pack(const double* __restrict__ X, int64_t count, int64_t len, int64_t sv, int64_t sp,
            const int* __restrict__ ex, int64_t vpad, int64_t lpad, uint16_t* __restrict__ Q) {
-  __shared__ double tile[64][65];
-  const int tid = threadIdx.x;
-  const int64_t v0 = (int64_t)blockIdx.x * 64;
-  const int64_t p0 = (int64_t)blockIdx.y * 64;
-  const bool v_unit = (sv == 1);
-  // load: 64 x 64 doubles, 16 per thread
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int e = tid + 256 * r;
-    const int a = e & 63, b = e >> 6;  // a runs fastest
-    const int vv = v_unit ? a : b;
-    const int pp = v_unit ? b : a;
-    const int64_t gv = v0 + vv, gp = p0 + pp;
-    double x = 0.0;
-    if (gv < count && gp < len) x = X[gv * sv + gp * sp];
-    tile[vv][pp] = x;
-  }
-  __syncthreads();
-  const int64_t nslabs = lpad / 16;
-  const int64_t npanels = vpad / PANEL;
-  // write: each thread produces 8 consecutive p (one 16-byte chunk) of one v
-  // per step; 64 v x 8 chunks = 512 chunks, 2 per thread.
-#pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    const int c = tid + 256 * r;
-    const int vv = c & 63;
-    const int pc = c >> 6;  // chunk of 8 along p
-    const int64_t gv = v0 + vv;
-    const int64_t gp = p0 + pc * 8;
-    if (gv >= vpad || gp >= lpad) continue;
-    const int E = (gv < count) ? ex[gv] : 0;
-    uint16_t q[S][8];
-#pragma unroll
-    for (int e = 0; e < 8; ++e) {
-      const double x = tile[vv][pc * 8 + e];
-      if (E == kNonFinite) {
-        const uint32_t fb = __float_as_uint((float)x);
-        q[0][e] = (uint16_t)(fb >> 16);
-#pragma unroll
-        for (int t = 1; t < S; ++t) q[t][e] = 0;
-      } else {
-        double rr = ldexp(x, -E);
-#pragma unroll
-        for (int t = 1; t <= S; ++t) {
-          const double qt = rint(ldexp(rr, 8 * t));
-          rr -= ldexp(qt, -8 * t);
-          // qt is an integer with |qt| <= 256: exact in fp32 and bf16
-          q[t - 1][e] = (uint16_t)(__float_as_uint((float)qt) >> 16);
-        }
-      }
-    }
-    const int64_t pv = gv / PANEL, ks = gp / 16;
-    const int kh = (int)((gp % 16) / 8);
-    const int64_t within = (int64_t)kh * PANEL * 8 + (gv % PANEL) * 8;
-#pragma unroll
-    for (int t = 0; t < S; ++t) {
-      const int64_t base = (((int64_t)t * npanels + pv) * nslabs + ks) * (PANEL * 16) + within;
-      uint4 w;
-      w.x = (uint32_t)q[t][0] | ((uint32_t)q[t][1] << 16);
-      w.y = (uint32_t)q[t][2] | ((uint32_t)q[t][3] << 16);
-      w.z = (uint32_t)q[t][4] | ((uint32_t)q[t][5] << 16);
-      w.w = (uint32_t)q[t][6] | ((uint32_t)q[t][7] << 16);
-      *reinterpret_cast<uint4*>(Q + base) = w;
-    }
-  }
+  slice_block<S, PANEL>(X, count, len, sv, sp, ex, vpad, lpad, Q, (int64_t)blockIdx.x * 64,
+                        (int64_t)blockIdx.y * 64);
 }
-
-// ---------------------------------------------------------------------------
-// 3. fused multi-slice multiply.
-//
-// Workgroup: 8 wavefronts (4 x 2), tile 128 x 64 of C; each wavefront one
-// 32 x 32 block. Per 16-deep slab every wavefront loads S A-fragments and S
-// B-fragments (16 bytes per lane each) and issues the S(S+1)/2 products
-// t+u <= S+1 into S fp32 accumulators, one per level d = t+u. The MFMA takes
-// the B slice as its "A" operand, so lane l holds column j = 8(r/4)+4(l/32)+r%4
-// and row i = l%32 of the block: column-major stores are 256-byte coalesced.
 
 template <int S>
 __global__ void __launch_bounds__(512)
@@ -232,128 +144,13 @@ split_multiply(int64_t m, int64_t n, int64_t kpad, double alpha, double beta,
                const uint16_t* __restrict__ Aq, const uint16_t* __restrict__ Bq,
                const int* __restrict__ EA, const int* __restrict__ EB, int64_t mpad, int64_t npad,
                int slabs_per_chunk) {
-  constexpr int BM = kPanelA, BN = kPanelB;
-  constexpr int ABLK = BM * kSlab;  // bf16 per A slice-slab block (4 KiB)
-  constexpr int BBLK = BN * kSlab;  // (2 KiB)
-  constexpr int CH_A = S * ABLK / 8;  // 16-byte chunks per slab
-  constexpr int CH_B = S * BBLK / 8;
-  constexpr int CH = CH_A + CH_B;
-  constexpr int PER = (CH + 511) / 512;
-  __shared__ __attribute__((aligned(16))) uint16_t As[2][S * ABLK];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][S * BBLK];
-
-  const int tid = threadIdx.x;
-  const int lane = tid & 63;
-  const int wave = tid >> 6;
-  const int wm = wave & 3;   // 4 waves along M
-  const int wn = wave >> 2;  // 2 along N
-
-  const int64_t tiles_m = mpad / BM;
-  const int64_t tiles_n = npad / BN;
+  const int64_t tiles_m = mpad / kPanelA;
+  const int64_t tiles_n = npad / kPanelB;
   const int64_t t = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
   int64_t tm, tn;
   grouped_tile(t, tiles_m, tiles_n, 8, tm, tn);
-
-  const int64_t nslabs = kpad / kSlab;
-  // slab block of slice t: A panel tm, slab ks
-  const uint16_t* Abase = Aq + tm * nslabs * ABLK;
-  const uint16_t* Bbase = Bq + tn * nslabs * BBLK;
-  const int64_t a_slice_stride = tiles_m * nslabs * ABLK;
-  const int64_t b_slice_stride = tiles_n * nslabs * BBLK;
-
-  uint4 stage[PER];
-  auto load_slab = [&](int64_t ks) {
-#pragma unroll
-    for (int r = 0; r < PER; ++r) {
-      const int c = tid + 512 * r;
-      if (c < CH_A) {
-        const int sl = c / (ABLK / 8), w = c % (ABLK / 8);
-        stage[r] = *reinterpret_cast<const uint4*>(Abase + sl * a_slice_stride + ks * ABLK + w * 8);
-      } else if (c < CH) {
-        const int cb = c - CH_A;
-        const int sl = cb / (BBLK / 8), w = cb % (BBLK / 8);
-        stage[r] = *reinterpret_cast<const uint4*>(Bbase + sl * b_slice_stride + ks * BBLK + w * 8);
-      }
-    }
-  };
-  auto store_slab = [&](int buf) {
-#pragma unroll
-    for (int r = 0; r < PER; ++r) {
-      const int c = tid + 512 * r;
-      if (c < CH_A) *reinterpret_cast<uint4*>(&As[buf][c * 8]) = stage[r];
-      else if (c < CH) *reinterpret_cast<uint4*>(&Bs[buf][(c - CH_A) * 8]) = stage[r];
-    }
-  };
-
-  v16f32 lev[S];
-#pragma unroll
-  for (int d = 0; d < S; ++d)
-#pragma unroll
-    for (int r = 0; r < 16; ++r) lev[d][r] = 0.0f;
-  double acc[16];
-#pragma unroll
-  for (int r = 0; r < 16; ++r) acc[r] = 0.0;
-
-  // fragment offsets inside a slice-slab block: [kh][row][8]
-  const int kh = lane >> 5;
-  const int a_off = kh * BM * 8 + (wm * 32 + (lane & 31)) * 8;
-  const int b_off = kh * BN * 8 + (wn * 32 + (lane & 31)) * 8;
-
-  auto fold = [&]() {
-#pragma unroll
-    for (int d = S - 1; d >= 0; --d) {  // level d+2, smallest first
-      const double sc = ldexp(1.0, -8 * (d + 2));
-#pragma unroll
-      for (int r = 0; r < 16; ++r) {
-        acc[r] = __builtin_fma((double)lev[d][r], sc, acc[r]);
-        lev[d][r] = 0.0f;
-      }
-    }
-  };
-
-  if (nslabs > 0) {
-    load_slab(0);
-    store_slab(0);
-  }
-  __syncthreads();
-  int in_chunk = 0;
-  for (int64_t ks = 0; ks < nslabs; ++ks) {
-    const int cur = (int)(ks & 1);
-    if (ks + 1 < nslabs) load_slab(ks + 1);
-    v8bf16 af[S], bf[S];
-#pragma unroll
-    for (int s = 0; s < S; ++s) {
-      af[s] = *reinterpret_cast<const v8bf16*>(&As[cur][s * ABLK + a_off]);
-      bf[s] = *reinterpret_cast<const v8bf16*>(&Bs[cur][s * BBLK + b_off]);
-    }
-#pragma unroll
-    for (int d = 0; d < S; ++d)       // level d+2
-#pragma unroll
-      for (int ta = 0; ta <= d; ++ta)  // A slice ta+1, B slice d-ta+1
-        lev[d] = mfma_bf16_32x32x16(bf[d - ta], af[ta], lev[d]);
-    if (ks + 1 < nslabs) store_slab(cur ^ 1);
-    if (++in_chunk == slabs_per_chunk) {
-      fold();
-      in_chunk = 0;
-    }
-    __syncthreads();
-  }
-  if (in_chunk) fold();
-
-  // epilogue
-  const int64_t gi = tm * BM + wm * 32 + (lane & 31);
-  if (gi >= m) return;
-  const int ei = EA[gi];
-#pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int64_t gj = tn * BN + wn * 32 + 8 * (r >> 2) + 4 * kh + (r & 3);
-    if (gj < n) {
-      double v = alpha * acc[r];
-      if (ei != kNonFinite && EB[gj] != kNonFinite) v = ldexp(v, ei + EB[gj]);
-      double* cp = C + gi * rsC + gj * csC;
-      *cp = (beta == 0.0) ? v : __builtin_fma(beta, *cp, v);
-    }
-  }
+  split_tile<S, kPanelA / 32, kPanelB / 32>(m, n, kpad, alpha, beta, C, rsC, csC, Aq, Bq, EA, EB,
+                                            tiles_m, tiles_n, slabs_per_chunk, tm, tn);
 }
 
 // ---------------------------------------------------------------------------

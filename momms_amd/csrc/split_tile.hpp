@@ -1,0 +1,242 @@
+// Device building blocks of the split-BF16 FP64-accurate GEMM (the algorithm
+// is described in split_bf16.hip): packed-panel slicing of one 64 x 64 block
+// and the fused multi-slice multiply of one tile. Used by the one-problem
+// kernels (split_bf16.hip) and the persistent batched kernels (batched.hip).
+#pragma once
+#include <math.h>
+
+#include "mfma.hpp"
+
+namespace momms {
+
+constexpr int kPanelA = 128;  // rows of A per packed panel (= multiply tile BM)
+constexpr int kPanelB = 64;   // columns of B per packed panel (= multiply tile BN)
+constexpr int kSlab = 16;     // k per slab (= one v_mfma_f32_32x32x16_bf16 step)
+constexpr int kMaxSlices = 8;
+
+// Exponent flag of a vector holding Inf or NaN (its values pass unsliced).
+constexpr int kNonFinite = 0x7fffffff;
+
+// ---------------------------------------------------------------------------
+// 2. slicing into packed panels.
+//
+// Vector v (a row of A or a column of B), element p lives at
+// X[v*sv + p*sp]. Output for slice t: panel pv = v / PANEL, slab ks = p / 16,
+//   Q[((t*npanels + pv)*nslabs + ks)*(PANEL*16) + kh*PANEL*8 + (v%PANEL)*8 + p%8]
+// with kh = (p % 16) / 8. Padding (v >= count or p >= len) is written as 0.
+// A 64 x 64 tile of (v, p) goes through LDS so that both the global read
+// (along whichever index is unit-stride) and the packed write are coalesced.
+
+template <int S, int PANEL>
+__device__ __forceinline__ void slice_block(const double* __restrict__ X, int64_t count,
+                                            int64_t len, int64_t sv, int64_t sp,
+                                            const int* __restrict__ ex, int64_t vpad,
+                                            int64_t lpad, uint16_t* __restrict__ Q, int64_t v0,
+                                            int64_t p0) {
+  __shared__ double tile[64][65];
+  const int tid = threadIdx.x;
+  const bool v_unit = (sv == 1);
+  // load: 64 x 64 doubles, 16 per thread
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int e = tid + 256 * r;
+    const int a = e & 63, b = e >> 6;  // a runs fastest
+    const int vv = v_unit ? a : b;
+    const int pp = v_unit ? b : a;
+    const int64_t gv = v0 + vv, gp = p0 + pp;
+    double x = 0.0;
+    if (gv < count && gp < len) x = X[gv * sv + gp * sp];
+    tile[vv][pp] = x;
+  }
+  __syncthreads();
+  const int64_t nslabs = lpad / 16;
+  const int64_t npanels = vpad / PANEL;
+  // write: each thread produces 8 consecutive p (one 16-byte chunk) of one v
+  // per step; 64 v x 8 chunks = 512 chunks, 2 per thread.
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int c = tid + 256 * r;
+    const int vv = c & 63;
+    const int pc = c >> 6;  // chunk of 8 along p
+    const int64_t gv = v0 + vv;
+    const int64_t gp = p0 + pc * 8;
+    if (gv >= vpad || gp >= lpad) continue;
+    const int E = (gv < count) ? ex[gv] : 0;
+    uint16_t q[S][8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) {
+      const double x = tile[vv][pc * 8 + e];
+      if (E == kNonFinite) {
+        const uint32_t fb = __float_as_uint((float)x);
+        q[0][e] = (uint16_t)(fb >> 16);
+#pragma unroll
+        for (int t = 1; t < S; ++t) q[t][e] = 0;
+      } else {
+        double rr = ldexp(x, -E);
+#pragma unroll
+        for (int t = 1; t <= S; ++t) {
+          const double qt = rint(ldexp(rr, 8 * t));
+          rr -= ldexp(qt, -8 * t);
+          // qt is an integer with |qt| <= 256: exact in fp32 and bf16
+          q[t - 1][e] = (uint16_t)(__float_as_uint((float)qt) >> 16);
+        }
+      }
+    }
+    const int64_t pv = gv / PANEL, ks = gp / 16;
+    const int kh = (int)((gp % 16) / 8);
+    const int64_t within = (int64_t)kh * PANEL * 8 + (gv % PANEL) * 8;
+#pragma unroll
+    for (int t = 0; t < S; ++t) {
+      const int64_t base = (((int64_t)t * npanels + pv) * nslabs + ks) * (PANEL * 16) + within;
+      uint4 w;
+      w.x = (uint32_t)q[t][0] | ((uint32_t)q[t][1] << 16);
+      w.y = (uint32_t)q[t][2] | ((uint32_t)q[t][3] << 16);
+      w.z = (uint32_t)q[t][4] | ((uint32_t)q[t][5] << 16);
+      w.w = (uint32_t)q[t][6] | ((uint32_t)q[t][7] << 16);
+      *reinterpret_cast<uint4*>(Q + base) = w;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 3. fused multi-slice multiply.
+//
+// Workgroup: 8 wavefronts (4 x 2), tile 128 x 64 of C; each wavefront one
+// 32 x 32 block. Per 16-deep slab every wavefront loads S A-fragments and S
+// B-fragments (16 bytes per lane each) and issues the S(S+1)/2 products
+// t+u <= S+1 into S fp32 accumulators, one per level d = t+u. The MFMA takes
+// the B slice as its "A" operand, so lane l holds column j = 8(r/4)+4(l/32)+r%4
+// and row i = l%32 of the block: column-major stores are 256-byte coalesced.
+
+// One (32*WM) x (32*WN) tile (tm, tn) of one problem, from slices packed in
+// panels of BM = 32*WM rows of A and BN = 32*WN columns of B. Ends after a
+// workgroup barrier (LDS reusable at once); lanes past the edge return early.
+template <int S, int WM, int WN>
+__device__ __forceinline__ void split_tile(int64_t m, int64_t n, int64_t kpad, double alpha,
+                                           double beta, double* __restrict__ C, int64_t rsC,
+                                           int64_t csC, const uint16_t* __restrict__ Aq,
+                                           const uint16_t* __restrict__ Bq,
+                                           const int* __restrict__ EA,
+                                           const int* __restrict__ EB, int64_t tiles_m,
+                                           int64_t tiles_n, int slabs_per_chunk, int64_t tm,
+                                           int64_t tn) {
+  constexpr int NT = 64 * WM * WN;
+  constexpr int BM = 32 * WM, BN = 32 * WN;
+  constexpr int ABLK = BM * kSlab;  // bf16 per A slice-slab block
+  constexpr int BBLK = BN * kSlab;
+  constexpr int CH_A = S * ABLK / 8;  // 16-byte chunks per slab
+  constexpr int CH_B = S * BBLK / 8;
+  constexpr int CH = CH_A + CH_B;
+  constexpr int PER = (CH + NT - 1) / NT;
+  __shared__ __attribute__((aligned(16))) uint16_t As[2][S * ABLK];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][S * BBLK];
+
+  const int tid = threadIdx.x;
+  const int lane = tid & 63;
+  const int wave = tid >> 6;
+  const int wm = wave % WM;
+  const int wn = wave / WM;
+
+  const int64_t nslabs = kpad / kSlab;
+  // slab block of slice t: A panel tm, slab ks
+  const uint16_t* Abase = Aq + tm * nslabs * ABLK;
+  const uint16_t* Bbase = Bq + tn * nslabs * BBLK;
+  const int64_t a_slice_stride = tiles_m * nslabs * ABLK;
+  const int64_t b_slice_stride = tiles_n * nslabs * BBLK;
+
+  uint4 stage[PER];
+  auto load_slab = [&](int64_t ks) {
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+      const int c = tid + NT * r;
+      if (c < CH_A) {
+        const int sl = c / (ABLK / 8), w = c % (ABLK / 8);
+        stage[r] = *reinterpret_cast<const uint4*>(Abase + sl * a_slice_stride + ks * ABLK + w * 8);
+      } else if (c < CH) {
+        const int cb = c - CH_A;
+        const int sl = cb / (BBLK / 8), w = cb % (BBLK / 8);
+        stage[r] = *reinterpret_cast<const uint4*>(Bbase + sl * b_slice_stride + ks * BBLK + w * 8);
+      }
+    }
+  };
+  auto store_slab = [&](int buf) {
+#pragma unroll
+    for (int r = 0; r < PER; ++r) {
+      const int c = tid + NT * r;
+      if (c < CH_A) *reinterpret_cast<uint4*>(&As[buf][c * 8]) = stage[r];
+      else if (c < CH) *reinterpret_cast<uint4*>(&Bs[buf][(c - CH_A) * 8]) = stage[r];
+    }
+  };
+
+  v16f32 lev[S];
+#pragma unroll
+  for (int d = 0; d < S; ++d)
+#pragma unroll
+    for (int r = 0; r < 16; ++r) lev[d][r] = 0.0f;
+  double acc[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) acc[r] = 0.0;
+
+  // fragment offsets inside a slice-slab block: [kh][row][8]
+  const int kh = lane >> 5;
+  const int a_off = kh * BM * 8 + (wm * 32 + (lane & 31)) * 8;
+  const int b_off = kh * BN * 8 + (wn * 32 + (lane & 31)) * 8;
+
+  auto fold = [&]() {
+#pragma unroll
+    for (int d = S - 1; d >= 0; --d) {  // level d+2, smallest first
+      const double sc = ldexp(1.0, -8 * (d + 2));
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        acc[r] = __builtin_fma((double)lev[d][r], sc, acc[r]);
+        lev[d][r] = 0.0f;
+      }
+    }
+  };
+
+  if (nslabs > 0) {
+    load_slab(0);
+    store_slab(0);
+  }
+  __syncthreads();
+  int in_chunk = 0;
+  for (int64_t ks = 0; ks < nslabs; ++ks) {
+    const int cur = (int)(ks & 1);
+    if (ks + 1 < nslabs) load_slab(ks + 1);
+    v8bf16 af[S], bf[S];
+#pragma unroll
+    for (int s = 0; s < S; ++s) {
+      af[s] = *reinterpret_cast<const v8bf16*>(&As[cur][s * ABLK + a_off]);
+      bf[s] = *reinterpret_cast<const v8bf16*>(&Bs[cur][s * BBLK + b_off]);
+    }
+#pragma unroll
+    for (int d = 0; d < S; ++d)       // level d+2
+#pragma unroll
+      for (int ta = 0; ta <= d; ++ta)  // A slice ta+1, B slice d-ta+1
+        lev[d] = mfma_bf16_32x32x16(bf[d - ta], af[ta], lev[d]);
+    if (ks + 1 < nslabs) store_slab(cur ^ 1);
+    if (++in_chunk == slabs_per_chunk) {
+      fold();
+      in_chunk = 0;
+    }
+    __syncthreads();
+  }
+  if (in_chunk) fold();
+
+  // epilogue
+  const int64_t gi = tm * BM + wm * 32 + (lane & 31);
+  if (gi >= m) return;
+  const int ei = EA[gi];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int64_t gj = tn * BN + wn * 32 + 8 * (r >> 2) + 4 * kh + (r & 3);
+    if (gj < n) {
+      double v = alpha * acc[r];
+      if (ei != kNonFinite && EB[gj] != kNonFinite) v = ldexp(v, ei + EB[gj]);
+      double* cp = C + gi * rsC + gj * csC;
+      *cp = (beta == 0.0) ? v : __builtin_fma(beta, *cp, v);
+    }
+  }
+}
+
+}  // namespace momms

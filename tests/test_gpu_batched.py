@@ -1,0 +1,158 @@
+"""Batched uneven GEMMs in one persistent launch (and one hipGraph launch).
+
+Bars: plain FP64 batch == the loop of per-problem momms_hip_dgemm launches,
+bit for bit (the tile shape does not change any element's summation order),
+and within gamma_{k+1} of the double-double oracle; split-BF16 batch == the
+oracle's restatement bit for bit; graph replays == eager runs.
+"""
+import numpy as np
+import pytest
+import torch
+
+import pyoracle as O
+from conftest import SEED, gamma
+from momms_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+
+def arenas(rows, a_len, b_len, c_len, seed=SEED):
+    g = torch.Generator(device="cuda").manual_seed(seed & 0xFFFFFFFF)
+    A = torch.rand(a_len, generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+    B = torch.rand(b_len, generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+    C = torch.rand(c_len, generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+    return A, B, C
+
+
+def view(arena, off, h, w, ld):
+    return arena[off: off + ld * w].view(w, ld)[:, :h].T if h and w else np.zeros((h, w))
+
+
+def np_view(arena, off, h, w, ld):
+    if h == 0 or w == 0:
+        return np.zeros((h, w), order="F")
+    return np.asfortranarray(arena[off: off + ld * w].reshape(w, ld)[:, :h].T)
+
+
+def edge_rows():
+    """A table with ragged, degenerate and padded-leading-dimension problems."""
+    shapes = [(1, 1, 1), (17, 33, 5), (64, 64, 64), (65, 127, 200), (300, 7, 513), (0, 5, 5),
+              (5, 0, 5), (9, 9, 0), (129, 260, 31), (512, 512, 16), (16, 512, 512)]
+    rows, ao, bo, co = [], 0, 0, 0
+    for i, (m, n, k) in enumerate(shapes):
+        lda, ldb, ldc = m + (i % 3), k + 1, m + 2
+        rows.append((m, n, k, ao, bo, co, max(lda, 1), max(ldb, 1), max(ldc, 1)))
+        ao += max(lda, 1) * k
+        bo += max(ldb, 1) * n
+        co += max(ldc, 1) * n
+    return rows, ao, bo, co
+
+
+@pytest.mark.parametrize("table", ["edge", "uneven"])
+@pytest.mark.parametrize("autotune", [False, True])
+def test_plain_batch_equals_loop_and_oracle(table, autotune):
+    rows, al, bl, cl = edge_rows() if table == "edge" else capi.uneven_batch(300, 16, 300)
+    A, B, C0 = arenas(rows, al, bl, cl)
+    plan = capi.BatchPlan(rows, mode=0, autotune=autotune, A=A, B=B)
+    info = plan.describe()
+    assert info["mode"] == "fp64"
+    C1 = C0.clone()
+    plan.run(A, B, C1, 1.0, 1.0)
+    C2 = C0.clone()
+    capi.dgemm_batched_loop(rows, A, B, C2, 1.0, 1.0)
+    torch.cuda.synchronize()
+    assert torch.equal(C1, C2)
+    a, b, c0, c1 = (x.cpu().numpy() for x in (A, B, C0, C1))
+    for (m, n, k, ao, bo, co, lda, ldb, ldc) in rows[:60]:
+        if m == 0 or n == 0:
+            continue
+        Am, Bm, Cm0, Cm = (np_view(a, ao, m, k, lda), np_view(b, bo, k, n, ldb),
+                           np_view(c0, co, m, n, ldc), np_view(c1, co, m, n, ldc))
+        ref, mag = O.dd_gemm(Am, Bm, Cm0)
+        assert (np.abs(Cm - ref) <= gamma(k + 1) * mag).all(), (m, n, k)
+    # padding between problems (ldc > m) is untouched
+    untouched = np.ones(cl, dtype=bool)
+    for (m, n, k, ao, bo, co, lda, ldb, ldc) in rows:
+        for j in range(n):
+            untouched[co + j * ldc: co + j * ldc + m] = False
+    assert np.array_equal(c1[untouched], c0[untouched])
+    plan.close()
+
+
+@pytest.mark.parametrize("s", [3, 7])
+def test_split_batch_bitwise_against_restatement(s):
+    rows, al, bl, cl = edge_rows()
+    A, B, C0 = arenas(rows, al, bl, cl, seed=SEED + s)
+    plan = capi.BatchPlan(rows, mode=1, slices=s)
+    assert plan.describe()["slices"] == s
+    C = C0.clone()
+    plan.run(A, B, C, -0.5, 1.5)
+    torch.cuda.synchronize()
+    a, b, c0, c = (x.cpu().numpy() for x in (A, B, C0, C))
+    for (m, n, k, ao, bo, co, lda, ldb, ldc) in rows:
+        if m == 0 or n == 0:
+            continue
+        want = np_view(c0, co, m, n, ldc)
+        O.ozaki_gemm(np_view(a, ao, m, k, lda), np_view(b, bo, k, n, ldb), want, s, -0.5, 1.5)
+        got = np_view(c, co, m, n, ldc)
+        assert np.array_equal(got.view(np.int64), want.view(np.int64)), (m, n, k)
+    plan.close()
+
+
+@pytest.mark.parametrize("mode", [0, 1])
+def test_graph_replays_match_eager(mode):
+    rows, al, bl, cl = capi.uneven_batch(200, 16, 256, seed=SEED + mode)
+    A, B, C0 = arenas(rows, al, bl, cl)
+    plan = capi.BatchPlan(rows, mode=mode, slices=5)
+    Ce = C0.clone()
+    plan.run(A, B, Ce, 1.0, 1.0)
+    Cg = C0.clone()
+    plan.capture(A, B, Cg, 1.0, 1.0)
+    for rep in range(3):
+        Cg.copy_(C0)                      # eager work between replays
+        torch.cuda.synchronize()
+        plan.launch()
+        torch.cuda.synchronize()
+        assert torch.equal(Cg, Ce), f"replay {rep}"
+    plan.close()
+
+
+def test_split_slice_autotune_meets_accuracy():
+    rows, al, bl, cl = capi.uneven_batch(64, 16, 256, seed=SEED + 9)
+    A, B, C0 = arenas(rows, al, bl, cl)
+    plan = capi.BatchPlan(rows, mode=1, slices=0, autotune=True, A=A, B=B)
+    s = plan.describe()["slices"]
+    assert 4 <= s <= 8
+    C = C0.clone()
+    plan.run(A, B, C, 1.0, 0.0)
+    torch.cuda.synchronize()
+    a, b, c = (x.cpu().numpy() for x in (A, B, C))
+    worst = 0.0
+    for (m, n, k, ao, bo, co, lda, ldb, ldc) in rows[:20]:
+        ref, mag = O.dd_gemm(np_view(a, ao, m, k, lda), np_view(b, bo, k, n, ldb),
+                             np.zeros((m, n), order="F"), 1.0, 0.0)
+        worst = max(worst, float(np.max(np.abs(np_view(c, co, m, n, ldc) - ref) / mag)))
+    assert worst <= 4 * gamma(512)
+    plan.close()
+
+
+def test_config5_full_batch_equals_loop():
+    """4096 problems, m, n, k log-uniform in [16, 512]."""
+    rows, al, bl, cl = capi.uneven_batch(4096)
+    A, B, C0 = arenas(rows, al, bl, cl)
+    plan = capi.BatchPlan(rows, mode=0, autotune=True, A=A, B=B)
+    C1 = C0.clone()
+    plan.capture(A, B, C1, 1.0, 1.0)
+    plan.launch()
+    C2 = C0.clone()
+    capi.dgemm_batched_loop(rows, A, B, C2, 1.0, 1.0)
+    torch.cuda.synchronize()
+    assert torch.equal(C1, C2)
+    plan.close()
+
+
+def test_plan_rejects_bad_tables():
+    with pytest.raises(capi.MommsError):
+        capi.BatchPlan([(4, 4, 4, 0, 0, 0, 2, 4, 4)])   # lda < m
+    with pytest.raises(capi.MommsError):
+        capi.BatchPlan([(4, 4, 4, 0, 0, 0, 4, 4, 4)], mode=2)
