@@ -209,11 +209,12 @@ __device__ __forceinline__ void split_tile(int64_t m, int64_t n, int64_t kpad, d
       af[s] = *reinterpret_cast<const v8bf16*>(&As[cur][s * ABLK + a_off]);
       bf[s] = *reinterpret_cast<const v8bf16*>(&Bs[cur][s * BBLK + b_off]);
     }
+    // A slice ta+1 times B slice d-ta+1 adds to level d+2; the inner loop
+    // walks the levels so consecutive MFMAs use different accumulators.
 #pragma unroll
-    for (int d = 0; d < S; ++d)       // level d+2
+    for (int ta = 0; ta < S; ++ta)
 #pragma unroll
-      for (int ta = 0; ta <= d; ++ta)  // A slice ta+1, B slice d-ta+1
-        lev[d] = mfma_bf16_32x32x16(bf[d - ta], af[ta], lev[d]);
+      for (int d = ta; d < S; ++d) lev[d] = mfma_bf16_32x32x16(bf[d - ta], af[ta], lev[d]);
     if (ks + 1 < nslabs) store_slab(cur ^ 1);
     if (++in_chunk == slabs_per_chunk) {
       fold();
