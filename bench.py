@@ -1,16 +1,25 @@
 #!/usr/bin/env python3
 """Benchmark: FP64-accurate GEMM at m=n=k=8192 on one MI355X.
 
-One step = one C := C + A*B at m=n=k=8192 (column-major f64, entries
-U[-1,1) from a fixed seed, inputs resident in HBM) through the split-BF16
-path (slices chosen for FP64-level accuracy), with the plain FP64-MFMA GEMM
-timed beside it on the same inputs. Prints ONE JSON line.
+One step = one C := C + A*B at m=n=k=8192 (column-major f64, entries U[-1,1)
+from a fixed seed, inputs resident in HBM) through the split-BF16 path (slices
+for FP64-level accuracy, 7 by default), with the plain FP64-MFMA GEMM timed
+beside it on the same inputs. Prints ONE JSON line:
 
-  value       split-BF16 FP64-equivalent TFLOP/s (2mnk / time per GEMM)
-  roofline    the dominant kernel (the fused slice-product GEMM) against the
-              dense BF16 MFMA peak, algorithmic slice flops P*2mnk per launch
-  plain       the FP64-MFMA GEMM's TFLOP/s and fraction of the FP64 peak
-  cpu_baseline  the restated Goto + 6x8 CPU path on this host's cores
+  value        split-BF16 FP64-equivalent TFLOP/s (2mnk / time of one whole
+               split GEMM: slicing + multiply)
+  roofline     the dominant kernel (split_multiply, the fused slice-product
+               GEMM) timed alone: P*2mnk slice flops per launch against the
+               dense BF16 MFMA peak
+  plain        the FP64-MFMA GEMM: TFLOP/s and fraction of the FP64 peak
+  accuracy     max scaled / relative error of both against double-double
+               values at 65,536 sampled entries (momms_hip_gemm_check_sampled)
+  configs      the other configurations' lines (sweep, slices x exponent
+               spread, FP32/BF16, batched) unless --quick
+  cpu_baseline the restated Goto + 6x8 CPU path on this host's cores
+
+--only {plain,split,multiply,batched,copy} runs one kernel family in a loop
+(for profilers) and prints nothing else but its timing line.
 """
 from __future__ import annotations
 
@@ -26,7 +35,9 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "FP64-accurate TFLOP/s @8192^3: split-BF16 MFMA vs plain FP64 MFMA, % peak"
 PEAK_FP64 = 78.6      # TFLOP/s, MI355X dense FP64 matrix
+PEAK_FP32 = 157.3     # TFLOP/s, MI355X dense FP32 matrix
 PEAK_BF16 = 2516.6    # TFLOP/s, MI355X dense BF16 matrix
+PEAK_HBM = 8000.0     # GB/s
 SEED = 0x6D6F6D6D73
 
 
@@ -34,8 +45,10 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def time_loop(fn, steps: int, warmup: int):
-    """Average milliseconds per call over `steps` calls, after `warmup`."""
+def time_loop(fn, steps: int, warmup: int) -> float:
+    """Average milliseconds per call over `steps` calls after `warmup`, from
+    HIP events on torch's current stream (the stream every call launches on),
+    bracketed by device synchronizations."""
     import torch
 
     for _ in range(warmup):
@@ -43,14 +56,23 @@ def time_loop(fn, steps: int, warmup: int):
     torch.cuda.synchronize()
     ev0 = torch.cuda.Event(enable_timing=True)
     ev1 = torch.cuda.Event(enable_timing=True)
-    t0 = time.perf_counter()
     ev0.record()
     for _ in range(steps):
         fn()
     ev1.record()
     torch.cuda.synchronize()
-    wall = (time.perf_counter() - t0) * 1e3 / steps
-    return ev0.elapsed_time(ev1) / steps, wall
+    return ev0.elapsed_time(ev1) / steps
+
+
+def rand_colmajor(n_rows, n_cols, gen, scale_e=0):
+    """Column-major f64 U[-1,1) (times 2^E, E uniform in [-e, e]) generated on the GPU."""
+    import torch
+
+    x = torch.rand((n_cols, n_rows), generator=gen, device="cuda", dtype=torch.float64) * 2 - 1
+    if scale_e:
+        e = torch.randint(-scale_e, scale_e + 1, (n_cols, n_rows), generator=gen, device="cuda")
+        x = x * torch.pow(2.0, e.double())
+    return x.T
 
 
 def cpu_baseline(threads: int) -> dict:
@@ -77,8 +99,93 @@ def cpu_baseline(threads: int) -> dict:
         best = min(best, time.perf_counter() - t0)
     tf = 2.0 * m * n * k / best / 1e12
     return {"value": round(tf, 4), "unit": "TFLOP/s", "cores": threads, "kind": "port",
-            "sample": f"plain FP64 Goto+6x8 GEMM m={m} n={n} k={k} (1/8 of the 8192^3 columns), "
-                      f"best of 3, {best:.2f} s"}
+            "sample": f"plain FP64 Goto+6x8 GEMM (the reference path restated in C) "
+                      f"m={m} n={n} k={k} = 1/8 of the 8192^3 columns, best of 3, {best:.2f} s"}
+
+
+def sweep_plain(capi, torch, gen, steps):
+    out = []
+    shapes = [(s, s, s) for s in (512, 1024, 2048, 4096, 8192)] + [(8192, 8192, 256), (8192, 64, 8192)]
+    for m, n, k in shapes:
+        A, B, C = rand_colmajor(m, k, gen), rand_colmajor(k, n, gen), rand_colmajor(m, n, gen)
+        ms = time_loop(lambda: capi.dgemm(A, B, C, 1.0, 1.0), steps, 2)
+        tf = 2.0 * m * n * k / ms / 1e9
+        out.append({"m": m, "n": n, "k": k, "ms": round(ms, 4), "tflops": round(tf, 3),
+                    "frac_fp64_peak": round(tf / PEAK_FP64, 4)})
+        del A, B, C
+    return out
+
+
+def split_grid(capi, torch, gen, steps, sizes=(4096, 8192), spreads=(0, 8, 16), slices=range(3, 8)):
+    out = []
+    for n in sizes:
+        for e in spreads:
+            A, B, C0 = rand_colmajor(n, n, gen, e), rand_colmajor(n, n, gen, e), rand_colmajor(n, n, gen)
+            C = C0.clone()
+            Cp = C0.clone()
+            capi.dgemm(A, B, Cp, 1.0, 1.0)
+            errp = capi.sampled_error(A, B, C0, Cp, samples=8192, seed=SEED)
+            for s in slices:
+                ws = capi.SplitWorkspace(n, n, n, s)
+                ms = time_loop(lambda: capi.dgemm_split_bf16(A, B, C, s, 1.0, 1.0, workspace=ws), steps, 1)
+                C.copy_(C0)
+                capi.dgemm_split_bf16(A, B, C, s, 1.0, 1.0, workspace=ws)
+                err = capi.sampled_error(A, B, C0, C, samples=8192, seed=SEED)
+                tf = 2.0 * n ** 3 / ms / 1e9
+                out.append({"n": n, "e": e, "slices": s, "ms": round(ms, 4), "tflops_fp64_equiv": round(tf, 3),
+                            "frac_fp64_peak": round(tf / PEAK_FP64, 4),
+                            "max_scaled_err": err["max_scaled"], "max_rel_err": err["max_rel"],
+                            "plain_max_scaled_err": errp["max_scaled"]})
+                del ws
+            del A, B, C0, C, Cp
+    return out
+
+
+def reduced_precision(capi, torch, gen, steps, n=8192):
+    out = []
+    A64, B64 = rand_colmajor(n, n, gen), rand_colmajor(n, n, gen)
+    for name, dt, fn, peak in (("fp32", torch.float32, capi.sgemm, PEAK_FP32),
+                               ("bf16_fp32acc", torch.bfloat16, capi.gemm_bf16, PEAK_BF16)):
+        A = A64.to(dt)
+        B = B64.to(dt)
+        C = torch.zeros((n, n), device="cuda", dtype=torch.float32).T
+        ms = time_loop(lambda: fn(A, B, C, 1.0, 0.0), steps, 2)
+        fn(A, B, C, 1.0, 0.0)
+        # error against the FP64 oracle of the FP64 inputs (sampled, on the GPU)
+        err = capi.sampled_error(A64, B64, None, C.double(), 1.0, 0.0, samples=8192, seed=SEED)
+        tf = 2.0 * n ** 3 / ms / 1e9
+        out.append({"kind": name, "n": n, "ms": round(ms, 4), "tflops": round(tf, 3),
+                    "frac_peak": round(tf / peak, 4), "max_scaled_err": err["max_scaled"],
+                    "max_rel_err": err["max_rel"]})
+        del A, B, C
+    return out
+
+
+def batched(capi, torch, gen, steps):
+    rows, al, bl, cl = capi.uneven_batch(4096)
+    flops = sum(2.0 * r[0] * r[1] * r[2] for r in rows)
+    A = torch.rand(al, generator=gen, device="cuda", dtype=torch.float64) * 2 - 1
+    B = torch.rand(bl, generator=gen, device="cuda", dtype=torch.float64) * 2 - 1
+    C = torch.rand(cl, generator=gen, device="cuda", dtype=torch.float64) * 2 - 1
+    res = {"problems": len(rows), "gflop": round(flops / 1e9, 3)}
+    plan = capi.BatchPlan(rows, mode=0, autotune=True, A=A, B=B)
+    plan.capture(A, B, C, 1.0, 1.0)
+    ms = time_loop(plan.launch, steps, 2)
+    res["plain_graph"] = {"ms": round(ms, 4), "tflops": round(flops / ms / 1e9, 3),
+                          "plan": plan.describe()}
+    ms = time_loop(lambda: plan.run(A, B, C, 1.0, 1.0), steps, 2)
+    res["plain_eager"] = {"ms": round(ms, 4), "tflops": round(flops / ms / 1e9, 3)}
+    plan.close()
+    descs = capi.desc_array(rows)
+    ms = time_loop(lambda: capi.dgemm_batched_loop(descs, A, B, C, 1.0, 1.0), 2, 1)
+    res["loop_of_launches"] = {"ms": round(ms, 4), "tflops": round(flops / ms / 1e9, 3)}
+    splan = capi.BatchPlan(rows, mode=1, slices=7)
+    splan.capture(A, B, C, 1.0, 1.0)
+    ms = time_loop(splan.launch, steps, 2)
+    res["split7_graph"] = {"ms": round(ms, 4), "tflops_fp64_equiv": round(flops / ms / 1e9, 3),
+                           "plan": splan.describe()}
+    splan.close()
+    return res
 
 
 def main() -> int:
@@ -90,97 +197,121 @@ def main() -> int:
     ap.add_argument("--slices", type=int, default=7)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--quick", action="store_true", help="headline only, no other configs")
+    ap.add_argument("--only", choices=["plain", "split", "multiply", "batched", "copy"])
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU only: --gpus {args.gpus} is not supported; "
-              "this build runs on a single MI355X (multi-GPU sharding is design-only).",
-              file=sys.stderr)
+        print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported. This engine "
+              "runs on a single MI355X; multi-GPU sharding is design-only.", file=sys.stderr)
         return 2
 
-    import numpy as np
     import torch
 
     from momms_amd import capi
 
-    dev = torch.device("cuda:0")
     n = args.size
-    rng = np.random.default_rng(SEED)
-    log(f"bench: generating inputs {n}^3")
-    A = torch.from_numpy(np.asfortranarray(rng.uniform(-1, 1, (n, n)))).to(dev)
-    B = torch.from_numpy(np.asfortranarray(rng.uniform(-1, 1, (n, n)))).to(dev)
-    C0 = torch.from_numpy(np.asfortranarray(rng.uniform(-1, 1, (n, n)))).to(dev)
+    s = args.slices
+    gen = torch.Generator(device="cuda").manual_seed(SEED & 0xFFFFFFFF)
+    flops = 2.0 * n ** 3
+    log(f"bench: inputs {n}^3 on {torch.cuda.get_device_name(0)}")
+    A = rand_colmajor(n, n, gen)
+    B = rand_colmajor(n, n, gen)
+    C0 = rand_colmajor(n, n, gen)
     C = C0.clone()
-    flops = 2.0 * n * n * n
+
+    if args.only == "copy":
+        # a plain copy of known size, for scaling HBM counters
+        src = torch.empty(1 << 30, dtype=torch.uint8, device="cuda")
+        dst = torch.empty_like(src)
+        ms = time_loop(lambda: dst.copy_(src), args.steps, args.warmup)
+        print(json.dumps({"only": "copy", "bytes": 1 << 30, "ms": ms,
+                          "GBps": 2 * (1 << 30) / ms / 1e6}))
+        return 0
+    if args.only == "plain":
+        ms = time_loop(lambda: capi.dgemm(A, B, C, 1.0, 1.0), args.steps, args.warmup)
+        print(json.dumps({"only": "plain", "ms": ms, "tflops": flops / ms / 1e9}))
+        return 0
+    if args.only in ("split", "multiply"):
+        ws = capi.SplitWorkspace(n, n, n, s)
+        if args.only == "split":
+            ms = time_loop(lambda: capi.dgemm_split_bf16(A, B, C, s, 1.0, 1.0, workspace=ws),
+                           args.steps, args.warmup)
+        else:
+            ws.prepare(A, B)
+            ms = time_loop(lambda: ws.multiply(C, 1.0, 1.0), args.steps, args.warmup)
+        print(json.dumps({"only": args.only, "slices": s, "ms": ms, "tflops": flops / ms / 1e9}))
+        return 0
+    if args.only == "batched":
+        del A, B, C0, C
+        print(json.dumps({"only": "batched", **batched(capi, torch, gen, args.steps)}))
+        return 0
 
     log("bench: plain FP64 MFMA")
-    plain_ms, plain_wall = time_loop(lambda: capi.dgemm(A, B, C, 1.0, 1.0), args.steps, args.warmup)
+    plain_ms = time_loop(lambda: capi.dgemm(A, B, C, 1.0, 1.0), args.steps, args.warmup)
     plain_tf = flops / plain_ms / 1e9
+
+    log(f"bench: split-BF16, slices={s}")
+    ws = capi.SplitWorkspace(n, n, n, s)
+    Cs = C0.clone()
+    split_ms = time_loop(lambda: capi.dgemm_split_bf16(A, B, Cs, s, 1.0, 1.0, workspace=ws),
+                         args.steps, args.warmup)
+    ws.prepare(A, B)
+    mult_ms = time_loop(lambda: ws.multiply(Cs, 1.0, 1.0), args.steps, args.warmup)
+    split_tf = flops / split_ms / 1e9
+    P = s * (s + 1) // 2
+    ach = P * flops / mult_ms / 1e9
+
+    log("bench: accuracy (sampled double-double)")
+    Cs.copy_(C0)
+    capi.dgemm_split_bf16(A, B, Cs, s, 1.0, 1.0, workspace=ws)
+    Cp = C0.clone()
+    capi.dgemm(A, B, Cp, 1.0, 1.0)
+    err_s = capi.sampled_error(A, B, C0, Cs, 1.0, 1.0, samples=65536, seed=SEED)
+    err_p = capi.sampled_error(A, B, C0, Cp, 1.0, 1.0, samples=65536, seed=SEED)
+    del ws, Cs, Cp, C
 
     result = {
         "metric": METRIC,
-        "value": None,
+        "value": round(split_tf, 3),
         "unit": "TFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": None,
+        "ms_per_step": round(split_ms, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64 in/out; bf16 slices x bf16 slices -> fp32 (exact), f64 fold",
         "data": "synthetic: column-major f64 U[-1,1) from a fixed seed, resident in HBM",
-        "config": {"workload": f"FP64-accurate GEMM C += A*B, m=n=k={n}", "m": n, "n": n, "k": n},
+        "config": {"workload": f"FP64-accurate GEMM C += A*B, m=n=k={n}, split-BF16 with {s} slices",
+                   "m": n, "n": n, "k": n, "slices": s},
+        "roofline": {"bound": "mfma", "achieved": round(ach, 2), "peak": PEAK_BF16,
+                     "unit": "TFLOP/s", "frac": round(ach / PEAK_BF16, 4), "traffic": None,
+                     "kernel": f"split_multiply<{s}>", "ms_per_launch": round(mult_ms, 4),
+                     "algorithmic_flops_per_launch": P * flops},
+        "split": {"slices": s, "products": P, "tflops_fp64_equiv": round(split_tf, 3),
+                  "frac_fp64_peak": round(split_tf / PEAK_FP64, 4),
+                  "speedup_vs_plain": round(split_tf / plain_tf, 4),
+                  "max_scaled_err": err_s["max_scaled"], "max_rel_err": err_s["max_rel"]},
         "plain": {"tflops": round(plain_tf, 3), "ms": round(plain_ms, 4),
-                  "frac_fp64_peak": round(plain_tf / PEAK_FP64, 4)},
+                  "frac_fp64_peak": round(plain_tf / PEAK_FP64, 4),
+                  "max_scaled_err": err_p["max_scaled"], "max_rel_err": err_p["max_rel"]},
     }
+    del A, B, C0
+    torch.cuda.empty_cache()
 
-    has_split = hasattr(capi, "has") and capi.has("momms_hip_dgemm_split_bf16")
-    if has_split:
-        s = args.slices
-        log(f"bench: split-BF16 slices={s}")
-        ws = capi.SplitWorkspace(n, n, n, s)
-        Cs = C0.clone()
-        split_ms, _ = time_loop(lambda: capi.dgemm_split_bf16(A, B, Cs, s, 1.0, 1.0, workspace=ws),
-                                args.steps, args.warmup)
-        # the dominant kernel alone: slice-product GEMM on prepared slices
-        ws.prepare(A, B)
-        mult_ms, _ = time_loop(lambda: ws.multiply(Cs, 1.0, 1.0), args.steps, args.warmup)
-        split_tf = flops / split_ms / 1e9
-        P = s * (s + 1) // 2
-        ach = P * flops / mult_ms / 1e9
-        # accuracy on the same inputs (fresh C), sampled against the library's
-        # double-double checker
-        Cs.copy_(C0)
-        capi.dgemm_split_bf16(A, B, Cs, s, 1.0, 1.0, workspace=ws)
-        Cp = C0.clone()
-        capi.dgemm(A, B, Cp, 1.0, 1.0)
-        err_s = capi.sampled_error(A, B, C0, Cs, 1.0, 1.0, samples=65536, seed=SEED)
-        err_p = capi.sampled_error(A, B, C0, Cp, 1.0, 1.0, samples=65536, seed=SEED)
-        result.update({
-            "value": round(split_tf, 3),
-            "ms_per_step": round(split_ms, 4),
-            "dtype": "bf16 slices, fp32 accumulate, f64 reconstruct",
-            "roofline": {"bound": "mfma", "achieved": round(ach, 2), "peak": PEAK_BF16,
-                         "unit": "TFLOP/s", "frac": round(ach / PEAK_BF16, 4), "traffic": None,
-                         "kernel": "split_bf16_multiply", "ms": round(mult_ms, 4),
-                         "algorithmic_flops_per_launch": P * flops},
-            "split": {"slices": s, "products": P, "tflops_fp64_equiv": round(split_tf, 3),
-                      "frac_fp64_peak": round(split_tf / PEAK_FP64, 4),
-                      "speedup_vs_plain": round(split_tf / plain_tf, 4),
-                      "max_scaled_err": err_s["max_scaled"], "max_rel_err": err_s["max_rel"]},
-        })
-        result["plain"].update({"max_scaled_err": err_p["max_scaled"],
-                                "max_rel_err": err_p["max_rel"]})
-    else:
-        # The split path is not built yet: report the plain FP64 GEMM.
-        result.update({
-            "value": round(plain_tf, 3),
-            "ms_per_step": round(plain_ms, 4),
-            "roofline": {"bound": "mfma", "achieved": round(plain_tf, 3), "peak": PEAK_FP64,
-                         "unit": "TFLOP/s", "frac": round(plain_tf / PEAK_FP64, 4),
-                         "traffic": None, "kernel": "dgemm_mfma_kernel",
-                         "algorithmic_flops_per_launch": flops},
-        })
+    if not args.quick:
+        cfg = {}
+        log("bench: config 2 sweep (plain FP64)")
+        cfg["plain_sweep"] = sweep_plain(capi, torch, gen, 3)
+        log("bench: config 3 grid (split-BF16: size x spread x slices)")
+        cfg["split_grid"] = split_grid(capi, torch, gen, 2)
+        log("bench: config 4 (FP32, BF16->FP32)")
+        cfg["reduced_precision"] = reduced_precision(capi, torch, gen, 3)
+        log("bench: config 5 (4096 uneven GEMMs)")
+        cfg["batched"] = batched(capi, torch, gen, 5)
+        result["configs"] = cfg
+        torch.cuda.empty_cache()
 
     if not args.no_cpu:
         threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))

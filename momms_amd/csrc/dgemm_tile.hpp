@@ -56,13 +56,16 @@ __device__ __forceinline__ void dgemm_tile(int64_t m, int64_t n, int64_t k, doub
       const int i = (LA == kRow) ? e / BK : e % BM;
       const int kk = (LA == kRow) ? e % BK : e / BM;
       const int64_t gi = row0 + i, gk = k0 + kk;
-      double v = 0.0;
-      if ((m_full && k_full) || (gi < m && gk < k)) {
-        if (LA == kCol) v = A[gi + gk * csA];
-        else if (LA == kRow) v = A[gi * rsA + gk];
-        else v = A[gi * rsA + gk * csA];
+      const int64_t idx = (LA == kCol) ? gi + gk * csA : (LA == kRow) ? gi * rsA + gk
+                                                                     : gi * rsA + gk * csA;
+      if (m_full && k_full) {
+        ra[p] = A[idx];
+      } else {
+        // out-of-range elements load A[0] (always valid) and are replaced by 0
+        const bool ok = gi < m && gk < k;
+        const double v = A[ok ? idx : 0];
+        ra[p] = ok ? v : 0.0;
       }
-      ra[p] = v;
     }
 #pragma unroll
     for (int p = 0; p < B_PER; ++p) {
@@ -70,13 +73,15 @@ __device__ __forceinline__ void dgemm_tile(int64_t m, int64_t n, int64_t k, doub
       const int j = (LB == kRow) ? e % BN : e / BK;
       const int kk = (LB == kRow) ? e / BN : e % BK;
       const int64_t gj = col0 + j, gk = k0 + kk;
-      double v = 0.0;
-      if ((n_full && k_full) || (gj < n && gk < k)) {
-        if (LB == kCol) v = B[gk + gj * csB];
-        else if (LB == kRow) v = B[gk * rsB + gj];
-        else v = B[gk * rsB + gj * csB];
+      const int64_t idx = (LB == kCol) ? gk + gj * csB : (LB == kRow) ? gk * rsB + gj
+                                                                     : gk * rsB + gj * csB;
+      if (n_full && k_full) {
+        rb[p] = B[idx];
+      } else {
+        const bool ok = gj < n && gk < k;
+        const double v = B[ok ? idx : 0];
+        rb[p] = ok ? v : 0.0;
       }
-      rb[p] = v;
     }
   };
   auto store_slab = [&](int buf) {

@@ -62,3 +62,50 @@ def test_dgemm_tiled_rejects_unknown_tile():
     st = capi.lib().momms_hip_dgemm_tiled(1, 1, 1, ctypes.byref(al), 8, 1, 1, 8, 1, 1,
                                           ctypes.byref(al), 8, 1, 1, 99, None)
     assert st == -15
+
+
+def test_batch_plan_rejects_bad_tables_before_gpu_work():
+    h = ctypes.c_void_p()
+    bad = capi.desc_array([(4, 4, 4, 0, 0, 0, 2, 4, 4)])  # lda < m
+    st = capi.lib().momms_hip_batch_plan_create(ctypes.cast(bad, ctypes.c_void_p), 1, 0, 0, 0,
+                                                None, None, ctypes.byref(h), None)
+    assert st == -1 and "descs[0]" in capi.last_error()
+    good = capi.desc_array([(4, 4, 4, 0, 0, 0, 4, 4, 4)])
+    st = capi.lib().momms_hip_batch_plan_create(ctypes.cast(good, ctypes.c_void_p), 1, 2, 0, 0,
+                                                None, None, ctypes.byref(h), None)
+    assert st == -3
+    st = capi.lib().momms_hip_batch_plan_create(ctypes.cast(good, ctypes.c_void_p), 1, 1, 9, 0,
+                                                None, None, ctypes.byref(h), None)
+    assert st == -4
+    st = capi.lib().momms_hip_batch_plan_create(ctypes.cast(good, ctypes.c_void_p), 1, 0, 0, 1,
+                                                None, None, ctypes.byref(h), None)
+    assert st == -6
+
+
+def test_split_entry_points_validate_workspace():
+    al = D(1.0)
+    size = capi.lib().momms_hip_split_bf16_workspace_size(100, 50, 30, 7)
+    assert size > 7 * 128 * 32 * 2
+    assert capi.lib().momms_hip_split_bf16_workspace_size(100, 50, 30, 9) == 0
+    st = capi.lib().momms_hip_dgemm_split_bf16(100, 50, 30, ctypes.byref(al), 8, 1, 100, 8, 1, 30,
+                                               ctypes.byref(al), 8, 1, 100, 9, 8, size, None)
+    assert st == -15
+    st = capi.lib().momms_hip_dgemm_split_bf16(100, 50, 30, ctypes.byref(al), 8, 1, 100, 8, 1, 30,
+                                               ctypes.byref(al), 8, 1, 100, 7, 8, size - 1, None)
+    assert st == -17
+    st = capi.lib().momms_hip_split_bf16_multiply(100, 50, 30, ctypes.byref(al), ctypes.byref(al),
+                                                  8, 1, 100, 7, None, size, None)
+    assert st == -10
+
+
+def test_sample_index_is_splitmix64():
+    # SplitMix64 (Steele, Lea, Flood 2014) reference values for seed 0
+    def sm(x):
+        x = (x + 0x9E3779B97F4A7C15) & (2**64 - 1)
+        x = ((x ^ (x >> 30)) * 0xBF58476D1CE4E5B9) & (2**64 - 1)
+        x = ((x ^ (x >> 27)) * 0x94D049BB133111EB) & (2**64 - 1)
+        return x ^ (x >> 31)
+    assert sm(0) == 0xE220A8397B1DCDAF     # published first output of SplitMix64(0)
+    for e in range(5):
+        assert capi.lib().momms_sample_index(7, e, 0, 1000) == sm(7 + 2 * e) % 1000
+        assert capi.lib().momms_sample_index(7, e, 1, 999) == sm(7 + 2 * e + 1) % 999
