@@ -13,9 +13,10 @@
 // workgroup barrier between slabs.
 //
 // The MFMA is issued with the operands swapped (B slab as the "A" operand),
-// so it computes the 16x16 block of C^T. In that form lane l holds 4
-// consecutive columns of one row, and the 16 lanes of a row group cover 16
-// consecutive rows: stores to column-major C are 128-byte coalesced.
+// so it computes the 16x16 block of C^T. In that form lane l holds row l%16
+// of the block in 4 columns l/16 + 4r, and each group of 16 lanes covers 16
+// consecutive rows of one column: stores to column-major C are 128-byte
+// coalesced.
 #include "dgemm.hpp"
 #include "dgemm_tile.hpp"
 #include "mfma.hpp"

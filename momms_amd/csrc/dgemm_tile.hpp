@@ -137,16 +137,17 @@ __device__ __forceinline__ void dgemm_tile(int64_t m, int64_t n, int64_t k, doub
     __syncthreads();
   }
 
-  // Epilogue. Lane holds C[i][j0+r] for r<4 of each block (C^T block layout).
+  // Epilogue. In the C^T block layout lane l holds C[i][j] with i = l%16 and
+  // j = l/16 + 4r (r < 4) of each 16x16 block.
 #pragma unroll
   for (int a = 0; a < FM; ++a) {
     const int64_t gi = row0 + wm * WTM + a * 16 + lr;
 #pragma unroll
     for (int b = 0; b < FN; ++b) {
-      const int64_t gj0 = col0 + wn * WTN + b * 16 + 4 * lk;
+      const int64_t gj0 = col0 + wn * WTN + b * 16 + lk;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
-        const int64_t gj = gj0 + r;
+        const int64_t gj = gj0 + 4 * r;
         if (gi < m && gj < n) {
           double* cp = C + gi * rsC + gj * csC;
           const double s = alpha * acc[a][b][r];

@@ -2,7 +2,7 @@
 //
 // Operand and result lane layouts (64-wide wavefront, lane l, result register r):
 //   v_mfma_f64_16x16x4_f64   A[p][k]: p=l%16, k=l/16      B[k][q]: q=l%16, k=l/16
-//                            D[p][q]: p=4*(l/16)+r, q=l%16                 (r<4)
+//                            D[p][q]: p=l/16+4*r, q=l%16                   (r<4)
 //   v_mfma_f32_32x32x16_bf16 A[p][k]: p=l%32, k=8*(l/32)+e  B[k][q]: q=l%32, k=8*(l/32)+e (e<8)
 //                            D[p][q]: p=8*(r/4)+4*(l/32)+r%4, q=l%32       (r<16)
 //   v_mfma_f32_32x32x2_f32   A[p][k]: p=l%32, k=l/32      B[k][q]: q=l%32, k=l/32

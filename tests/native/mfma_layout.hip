@@ -35,7 +35,7 @@ int main(){
     double h[128]; for(int l=0;l<64;l++){h[l]=A[l%16][l/16]; h[64+l]=B[l/16][l%16];}
     double *din,*dout; CK(hipMalloc(&din,128*8)); CK(hipMalloc(&dout,256*8)); CK(hipMemcpy(din,h,128*8,hipMemcpyHostToDevice));
     kf64<<<1,64>>>(din,dout); CK(hipDeviceSynchronize()); double o[256]; CK(hipMemcpy(o,dout,256*8,hipMemcpyDeviceToHost));
-    int bad=0; for(int l=0;l<64;l++)for(int r=0;r<4;r++){int i=4*(l/16)+r,j=l%16; double s=0; for(int k=0;k<4;k++)s+=A[i][k]*B[k][j]; if(s!=o[l*4+r])bad++;}
+    int bad=0; for(int l=0;l<64;l++)for(int r=0;r<4;r++){int i=l/16+4*r,j=l%16; double s=0; for(int k=0;k<4;k++)s+=A[i][k]*B[k][j]; if(s!=o[l*4+r])bad++;}
     printf("f64_16x16x4: %s (%d bad)\n",bad?"FAIL":"PASS",bad); fails+=bad!=0;
   }
   { // bf16 32x32x16
