@@ -40,6 +40,37 @@ dgemm_mfma_kernel(int64_t m, int64_t n, int64_t k, double alpha,
                                                    beta, C, rsC, csC, tm * BM, tn * BN);
 }
 
+template <int STAGES>
+__global__ void __launch_bounds__(256)
+dgemm_dma_kernel(int64_t m, int64_t n, int64_t k, double alpha, const double* __restrict__ A,
+                 int64_t lda, uint32_t a_bytes, const double* __restrict__ B, int64_t ldb,
+                 uint32_t b_bytes, double beta, double* __restrict__ C, int64_t ldc,
+                 int64_t group_m) {
+  __shared__ DgemmDmaSmem<STAGES> sm;
+  const int64_t tiles_m = ceil_div(m, 128);
+  const int64_t tiles_n = ceil_div(n, 128);
+  const int64_t t = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
+  int64_t tm, tn;
+  grouped_tile(t, tiles_m, tiles_n, group_m, tm, tn);
+  dgemm_tile_dma<STAGES>(sm, m, n, k, alpha, make_rsrc(A, a_bytes), lda, make_rsrc(B, b_bytes),
+                         ldb, beta, C, ldc, tm * 128, tn * 128);
+}
+
+// The DMA path needs column-major A, B and C, 16-byte aligned columns (for
+// the 16-byte loads) and operand extents a 32-bit buffer offset can address.
+static bool dma_ok(const DgemmArgs& g, uint32_t& a_bytes, uint32_t& b_bytes) {
+  if (g.rsA != 1 || g.rsB != 1 || g.rsC != 1 || g.k == 0) return false;
+  if ((g.csA & 1) || (g.csB & 1)) return false;
+  if ((reinterpret_cast<uintptr_t>(g.A) & 15) || (reinterpret_cast<uintptr_t>(g.B) & 15)) return false;
+  const uint64_t ea = (uint64_t)((g.k - 1) * g.csA + g.m) * 8;
+  const uint64_t eb = (uint64_t)((g.n - 1) * g.csB + g.k) * 8;
+  const uint64_t lim = (1ull << 32) - (1ull << 20);
+  if (ea > lim || eb > lim) return false;
+  a_bytes = (uint32_t)ea;
+  b_bytes = (uint32_t)eb;
+  return true;
+}
+
 // ---------------------------------------------------------------------------
 // Host side: tile selection and dispatch over operand storage orders.
 
@@ -78,6 +109,14 @@ hipError_t dgemm_launch(const DgemmArgs& g, int tile_cfg) {
   const int la = layout_of(g.rsA, g.csA, g.m, g.k);
   const int lb = layout_of(g.rsB, g.csB, g.k, g.n);
   if (tile_cfg < 0) tile_cfg = dgemm_pick_tile(g.m, g.n, g.k);
+  uint32_t a_bytes = 0, b_bytes = 0;
+  if (tile_cfg == 0 && dma_ok(g, a_bytes, b_bytes)) {
+    const int64_t tiles = ceil_div(g.m, 128) * ceil_div(g.n, 128);
+    hipLaunchKernelGGL((dgemm_dma_kernel<4>), dim3((unsigned)tiles), dim3(256), 0, g.stream, g.m,
+                       g.n, g.k, g.alpha, g.A, g.csA, a_bytes, g.B, g.csB, b_bytes, g.beta, g.C,
+                       g.csC, (int64_t)8);
+    return hipGetLastError();
+  }
   switch (tile_cfg) {
     case 1: return launch_cfg<64, 64, 16, 2, 2>(g, la, lb, 8);
     case 2: return launch_cfg<128, 64, 16, 2, 2>(g, la, lb, 8);

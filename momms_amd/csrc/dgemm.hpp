@@ -19,8 +19,10 @@ struct DgemmArgs {
   hipStream_t stream;
 };
 
-// Tile configurations (BM x BN, 4 wavefronts): 0 = 128x128, 1 = 64x64, 2 = 128x64.
-constexpr int kDgemmTileConfigs = 3;
+// Tile configurations (BM x BN, 4 wavefronts): 0 = 128x128 (the pipelined
+// buffer->LDS form when the operands allow it), 1 = 64x64, 2 = 128x64,
+// 3 = 128x128 register-staged form only.
+constexpr int kDgemmTileConfigs = 4;
 
 int dgemm_pick_tile(int64_t m, int64_t n, int64_t k);
 hipError_t dgemm_launch(const DgemmArgs& g, int tile_cfg);

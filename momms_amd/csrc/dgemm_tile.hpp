@@ -159,3 +159,154 @@ __device__ __forceinline__ void dgemm_tile(int64_t m, int64_t n, int64_t k, doub
 }
 
 }  // namespace momms
+
+namespace momms {
+
+// ---------------------------------------------------------------------------
+// Column-major fast path: slabs move HBM -> LDS by buffer_load ... lds (no
+// register staging) through a ring of STAGES slabs, so STAGES-1 slabs are in
+// flight while one is multiplied. Each wavefront load moves 1 KiB: lane l
+// writes LDS bytes [16l, 16l+16) of the destination chosen by the wave.
+//   A slab [BK][LDA]: one load = one k-column of the tile (BM = 128 rows).
+//   B slab, n-major: 8 columns x 16 k per 1 KiB block; column c of a block
+//   keeps its k-pair kp at 16-byte slot c*8 + (kp ^ c), so the MFMA reads
+//   (16 columns at one k) fall in distinct banks.
+// Out-of-range elements: the buffer resource covers exactly the matrix
+// extent (reads past it return zero, never fault); rows >= m of A and
+// columns >= n of B may read neighbouring values, which only reach outputs
+// that are not stored; the k >= K entries of the last slab are zeroed in LDS
+// after they land.
+// Same per-element summation order as dgemm_tile (k4 steps in slab order).
+
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, bytes, 0x00020000);
+}
+
+__device__ __forceinline__ void dma16(__amdgpu_buffer_rsrc_t r, void* lds, uint32_t voff) {
+  __builtin_amdgcn_raw_ptr_buffer_load_lds(r, (__attribute__((address_space(3))) void*)lds, 16,
+                                           voff, 0, 0, 0);
+}
+
+// s_waitcnt with only the vector-memory counter constrained (gfx9 encoding:
+// vmcnt in bits 3:0 and 15:14, expcnt 6:4 and lgkmcnt 11:8 left at maximum).
+template <int N>
+__device__ __forceinline__ void wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+
+// Workgroup barrier without a memory fence: the caller has already waited for
+// what the other wavefronts must see (vmcnt for LDS DMA, lgkmcnt for LDS
+// stores). A fence here would make the compiler drain every outstanding
+// buffer->LDS load, which is exactly what the ring is meant to keep in flight.
+__device__ __forceinline__ void lds_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
+template <int STAGES>
+struct DgemmDmaSmem {
+  static constexpr int BM = 128, BN = 128, BK = 16, LDA = BM + 16;
+  double A[STAGES][BK][LDA];
+  double B[STAGES][BN * BK];
+};
+
+template <int STAGES>
+__device__ __forceinline__ void dgemm_tile_dma(DgemmDmaSmem<STAGES>& sm, int64_t m, int64_t n,
+                                               int64_t k, double alpha,
+                                               __amdgpu_buffer_rsrc_t ra, int64_t lda,
+                                               __amdgpu_buffer_rsrc_t rb, int64_t ldb,
+                                               double beta, double* __restrict__ C,
+                                               int64_t ldc, int64_t row0, int64_t col0) {
+  constexpr int BM = 128, BN = 128, BK = 16, WTM = 64, WTN = 64, FM = 4, FN = 4;
+  constexpr int LOADS = (BK + BN / 8) / 4;  // wavefront loads per slab per wave (8)
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave & 1, wn = wave >> 1;
+  const int64_t ktiles = ceil_div(k, BK);
+
+  // this wave's loads of slab kt into stage st: A rows kk = wave*4 .. +3,
+  // B blocks b = wave*4 .. +3
+  const int bc = lane >> 3, bkp = (lane & 7) ^ (lane >> 3);  // column in block, k-pair fetched
+  auto issue = [&](int64_t kt, int st) {
+    const int64_t k0 = kt * BK;
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int kk = wave * 4 + q;
+      const int64_t e = (row0 + 2 * lane) + (k0 + kk) * lda;  // element offset in A
+      dma16(ra, &sm.A[st][kk][0], (uint32_t)(e * 8));
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+      const int b = wave * 4 + q;
+      const int64_t e = (col0 + 8 * b + bc) * ldb + k0 + 2 * bkp;
+      dma16(rb, &sm.B[st][b * 128], (uint32_t)(e * 8));
+    }
+  };
+
+  v4f64 acc[FM][FN];
+#pragma unroll
+  for (int a = 0; a < FM; ++a)
+#pragma unroll
+    for (int b = 0; b < FN; ++b) acc[a][b] = v4f64{0.0, 0.0, 0.0, 0.0};
+
+  const int lr = lane & 15, lk = lane >> 4;
+  for (int s = 0; s < STAGES - 1; ++s)
+    if (s < ktiles) issue(s, s);
+
+  for (int64_t kt = 0; kt < ktiles; ++kt) {
+    const int64_t left = ktiles - 1 - kt;  // slabs issued after this one
+    if (left >= STAGES - 2) wait_vmcnt<(STAGES - 2) * LOADS>();
+    else if (left == 1) wait_vmcnt<LOADS>();
+    else wait_vmcnt<0>();
+    lds_barrier();  // slab kt visible; every wave is done with slab kt-1
+    const int cur = (int)(kt % STAGES);
+    if (kt * BK + BK > k) {
+      // last, partial slab: clear the k >= K entries of both slabs (their
+      // buffer offsets may have wrapped into the matrix), then sync again
+      const int kv = (int)(k - kt * BK);
+      for (int e = tid; e < BN * BK; e += 256) {
+        const int slot = e >> 1, c = (slot >> 3) & 7, kp = (slot & 7) ^ c;
+        if (2 * kp + (e & 1) >= kv) sm.B[cur][e] = 0.0;
+      }
+      for (int e = tid; e < (BK - kv) * BM; e += 256) sm.A[cur][kv + e / BM][e % BM] = 0.0;
+      lds_barrier();
+    }
+    if (kt + STAGES - 1 < ktiles) issue(kt + STAGES - 1, (int)((kt + STAGES - 1) % STAGES));
+#pragma unroll
+    for (int k4 = 0; k4 < BK / 4; ++k4) {
+      const int kk = k4 * 4 + lk;
+      double af[FM], bf[FN];
+#pragma unroll
+      for (int a = 0; a < FM; ++a) af[a] = sm.A[cur][kk][wm * WTM + a * 16 + lr];
+#pragma unroll
+      for (int b = 0; b < FN; ++b) {
+        const int j = wn * WTN + b * 16 + lr, c = j & 7, kp = kk >> 1;
+        bf[b] = sm.B[cur][(j >> 3) * 128 + (c * 8 + (kp ^ c)) * 2 + (kk & 1)];
+      }
+#pragma unroll
+      for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b) acc[a][b] = mfma_f64_16x16x4(bf[b], af[a], acc[a][b]);
+    }
+  }
+  lds_barrier();  // LDS free for the next tile of a persistent caller
+
+#pragma unroll
+  for (int a = 0; a < FM; ++a) {
+    const int64_t gi = row0 + wm * WTM + a * 16 + lr;
+    if (gi >= m) continue;
+#pragma unroll
+    for (int b = 0; b < FN; ++b) {
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gj = col0 + wn * WTN + b * 16 + lk + 4 * r;
+        if (gj < n) {
+          double* cp = C + gi + gj * ldc;
+          const double s = alpha * acc[a][b][r];
+          *cp = (beta == 0.0) ? s : __builtin_fma(beta, *cp, s);
+        }
+      }
+    }
+  }
+}
+
+}  // namespace momms

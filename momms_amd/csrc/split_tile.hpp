@@ -109,8 +109,35 @@ __device__ __forceinline__ void slice_block(const double* __restrict__ X, int64_
 // and row i = l%32 of the block: column-major stores are 256-byte coalesced.
 
 // One (32*WM) x (32*WN) tile (tm, tn) of one problem, from slices packed in
-// panels of BM = 32*WM rows of A and BN = 32*WN columns of B. Ends after a
-// workgroup barrier (LDS reusable at once); lanes past the edge return early.
+// panels of BM = 32*WM rows of A and BN = 32*WN columns of B. Slab blocks move
+// HBM -> LDS by global_load_lds_dwordx4 (1 KiB per wavefront load, no
+// register staging) through a ring of STAGES slabs, STAGES-1 in flight.
+// Ends after a workgroup barrier (LDS reusable at once); lanes past the edge
+// return early from the epilogue.
+template <int S, int WM, int WN>
+struct SplitRing {
+  static constexpr int BM = 32 * WM, BN = 32 * WN;
+  static constexpr int ABLK = BM * kSlab, BBLK = BN * kSlab;  // bf16 per slice-slab block
+  static constexpr int STAGES = 3;
+};
+
+__device__ __forceinline__ void lds_dma16(const void* src, void* lds) {
+  __builtin_amdgcn_global_load_lds((const void __attribute__((address_space(1)))*)src,
+                                   (void __attribute__((address_space(3)))*)lds, 16, 0, 0);
+}
+
+// s_waitcnt on the vector-memory counter only (gfx9 encoding).
+template <int N>
+__device__ __forceinline__ void split_wait_vmcnt() {
+  static_assert(N >= 0 && N < 64, "vmcnt range");
+  __builtin_amdgcn_s_waitcnt((N & 0xF) | ((N >> 4) << 14) | (0x7 << 4) | (0xF << 8));
+}
+
+// Workgroup barrier without a memory fence (a fence would drain the ring).
+__device__ __forceinline__ void split_barrier() {
+  asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
+}
+
 template <int S, int WM, int WN>
 __device__ __forceinline__ void split_tile(int64_t m, int64_t n, int64_t kpad, double alpha,
                                            double beta, double* __restrict__ C, int64_t rsC,
@@ -120,54 +147,41 @@ __device__ __forceinline__ void split_tile(int64_t m, int64_t n, int64_t kpad, d
                                            const int* __restrict__ EB, int64_t tiles_m,
                                            int64_t tiles_n, int slabs_per_chunk, int64_t tm,
                                            int64_t tn) {
-  constexpr int NT = 64 * WM * WN;
-  constexpr int BM = 32 * WM, BN = 32 * WN;
-  constexpr int ABLK = BM * kSlab;  // bf16 per A slice-slab block
-  constexpr int BBLK = BN * kSlab;
-  constexpr int CH_A = S * ABLK / 8;  // 16-byte chunks per slab
-  constexpr int CH_B = S * BBLK / 8;
-  constexpr int CH = CH_A + CH_B;
-  constexpr int PER = (CH + NT - 1) / NT;
-  __shared__ __attribute__((aligned(16))) uint16_t As[2][S * ABLK];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs[2][S * BBLK];
+  typedef SplitRing<S, WM, WN> R;
+  constexpr int NW = WM * WN;
+  constexpr int BM = R::BM, BN = R::BN, ABLK = R::ABLK, BBLK = R::BBLK, STAGES = R::STAGES;
+  constexpr int LA = ABLK / 512, LB = BBLK / 512;  // 1 KiB wavefront loads per slice block
+  constexpr int LOADS = S * (LA + LB);              // per slab, whole workgroup
+  constexpr int LO = LOADS / NW;                     // per wave: LO or LO+1
+  __shared__ __attribute__((aligned(16))) uint16_t As[STAGES][S * ABLK];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs[STAGES][S * BBLK];
 
   const int tid = threadIdx.x;
   const int lane = tid & 63;
   const int wave = tid >> 6;
   const int wm = wave % WM;
   const int wn = wave / WM;
+  const int my_loads = LO + (wave < LOADS % NW ? 1 : 0);
 
   const int64_t nslabs = kpad / kSlab;
-  // slab block of slice t: A panel tm, slab ks
   const uint16_t* Abase = Aq + tm * nslabs * ABLK;
   const uint16_t* Bbase = Bq + tn * nslabs * BBLK;
   const int64_t a_slice_stride = tiles_m * nslabs * ABLK;
   const int64_t b_slice_stride = tiles_n * nslabs * BBLK;
 
-  uint4 stage[PER];
-  auto load_slab = [&](int64_t ks) {
-#pragma unroll
-    for (int r = 0; r < PER; ++r) {
-      const int c = tid + NT * r;
-      if (c < CH_A) {
-        const int sl = c / (ABLK / 8), w = c % (ABLK / 8);
-        stage[r] = *reinterpret_cast<const uint4*>(Abase + sl * a_slice_stride + ks * ABLK + w * 8);
-      } else if (c < CH) {
-        const int cb = c - CH_A;
-        const int sl = cb / (BBLK / 8), w = cb % (BBLK / 8);
-        stage[r] = *reinterpret_cast<const uint4*>(Bbase + sl * b_slice_stride + ks * BBLK + w * 8);
+  auto issue = [&](int64_t ks, int st) {
+    for (int c = wave; c < LOADS; c += NW) {
+      if (c < S * LA) {
+        const int t = c / LA, q = c % LA;
+        lds_dma16(Abase + t * a_slice_stride + ks * ABLK + q * 512 + lane * 8,
+                  &As[st][t * ABLK + q * 512]);
+      } else {
+        const int cb = c - S * LA, t = cb / LB, q = cb % LB;
+        lds_dma16(Bbase + t * b_slice_stride + ks * BBLK + q * 512 + lane * 8,
+                  &Bs[st][t * BBLK + q * 512]);
       }
     }
   };
-  auto store_slab = [&](int buf) {
-#pragma unroll
-    for (int r = 0; r < PER; ++r) {
-      const int c = tid + NT * r;
-      if (c < CH_A) *reinterpret_cast<uint4*>(&As[buf][c * 8]) = stage[r];
-      else if (c < CH) *reinterpret_cast<uint4*>(&Bs[buf][(c - CH_A) * 8]) = stage[r];
-    }
-  };
-
   v16f32 lev[S];
 #pragma unroll
   for (int d = 0; d < S; ++d)
@@ -194,15 +208,20 @@ __device__ __forceinline__ void split_tile(int64_t m, int64_t n, int64_t kpad, d
     }
   };
 
-  if (nslabs > 0) {
-    load_slab(0);
-    store_slab(0);
-  }
-  __syncthreads();
+  for (int st = 0; st < STAGES - 1; ++st)
+    if (st < nslabs) issue(st, st);
   int in_chunk = 0;
   for (int64_t ks = 0; ks < nslabs; ++ks) {
-    const int cur = (int)(ks & 1);
-    if (ks + 1 < nslabs) load_slab(ks + 1);
+    // slab ks has landed once at most (slabs issued after it) x my_loads remain
+    if (ks + 1 < nslabs) {
+      if (my_loads == LO) split_wait_vmcnt<LO>();
+      else split_wait_vmcnt<LO + 1>();
+    } else {
+      split_wait_vmcnt<0>();
+    }
+    split_barrier();  // slab ks visible to all; everyone is done with slab ks-1
+    if (ks + STAGES - 1 < nslabs) issue(ks + STAGES - 1, (int)((ks + STAGES - 1) % STAGES));
+    const int cur = (int)(ks % STAGES);
     v8bf16 af[S], bf[S];
 #pragma unroll
     for (int s = 0; s < S; ++s) {
@@ -215,13 +234,12 @@ __device__ __forceinline__ void split_tile(int64_t m, int64_t n, int64_t kpad, d
     for (int ta = 0; ta < S; ++ta)
 #pragma unroll
       for (int d = ta; d < S; ++d) lev[d] = mfma_bf16_32x32x16(bf[d - ta], af[ta], lev[d]);
-    if (ks + 1 < nslabs) store_slab(cur ^ 1);
     if (++in_chunk == slabs_per_chunk) {
       fold();
       in_chunk = 0;
     }
-    __syncthreads();
   }
+  split_barrier();  // LDS free for the next tile of a persistent caller
   if (in_chunk) fold();
 
   // epilogue
