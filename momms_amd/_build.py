@@ -7,7 +7,7 @@ Produces, in-tree:
                               its own Makefile)
 
 Every HIP source is compiled for gfx950 only, with packed-FP32 instruction
-selection disabled (-packed-fp32-ops).
+selection disabled (-packed-fp32-ops) and no floating-point contraction.
 """
 from __future__ import annotations
 
@@ -34,6 +34,9 @@ HIP_FLAGS = [
     "-std=c++17",
     "-fPIC",
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    # arithmetic exactly as written: the error-free transformations (TwoSum,
+    # TwoProduct) and the bit-exact split reconstruction depend on it
+    "-ffp-contract=off",
     f"-I{ROOT / 'include'}",
     f"-I{CSRC}",
 ]
