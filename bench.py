@@ -2,8 +2,8 @@
 """Benchmark: FP64-accurate GEMM at m=n=k=8192 on one MI355X.
 
 One step = one C := C + A*B at m=n=k=8192 (column-major f64, entries U[-1,1)
-from a fixed seed, inputs resident in HBM) through the split-BF16 path (slices
-for FP64-level accuracy, 7 by default), with the plain FP64-MFMA GEMM timed
+from a fixed seed, inputs resident in HBM) through the split-BF16 path (the smallest
+slice count whose sampled errors stay within 2x plain FP64's), with the plain FP64-MFMA GEMM timed
 beside it on the same inputs. Prints ONE JSON line:
 
   value        split-BF16 FP64-equivalent TFLOP/s (2mnk / time of one whole
@@ -194,10 +194,12 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--size", type=int, default=8192)
-    ap.add_argument("--slices", type=int, default=7)
+    ap.add_argument("--slices", type=int, default=0,
+                    help="0 = smallest count within 2x plain FP64's error (FP64-accurate)")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only, no other configs")
+    ap.add_argument("--tile", type=int, default=None, help="FP64 tile configuration (--only plain)")
     ap.add_argument("--only", choices=["plain", "split", "multiply", "batched", "copy"])
     args = ap.parse_args()
     if args.gpus != 1:
@@ -210,7 +212,7 @@ def main() -> int:
     from momms_amd import capi
 
     n = args.size
-    s = args.slices
+    s = args.slices if args.slices else 7
     gen = torch.Generator(device="cuda").manual_seed(SEED & 0xFFFFFFFF)
     flops = 2.0 * n ** 3
     log(f"bench: inputs {n}^3 on {torch.cuda.get_device_name(0)}")
@@ -228,8 +230,9 @@ def main() -> int:
                           "GBps": 2 * (1 << 30) / ms / 1e6}))
         return 0
     if args.only == "plain":
-        ms = time_loop(lambda: capi.dgemm(A, B, C, 1.0, 1.0), args.steps, args.warmup)
-        print(json.dumps({"only": "plain", "ms": ms, "tflops": flops / ms / 1e9}))
+        ms = time_loop(lambda: capi.dgemm(A, B, C, 1.0, 1.0, tile_cfg=args.tile), args.steps,
+                       args.warmup)
+        print(json.dumps({"only": "plain", "tile": args.tile, "ms": ms, "tflops": flops / ms / 1e9}))
         return 0
     if args.only in ("split", "multiply"):
         ws = capi.SplitWorkspace(n, n, n, s)
@@ -250,6 +253,25 @@ def main() -> int:
     plain_ms = time_loop(lambda: capi.dgemm(A, B, C, 1.0, 1.0), args.steps, args.warmup)
     plain_tf = flops / plain_ms / 1e9
 
+    if args.slices == 0:
+        # FP64-accurate: the smallest slice count whose sampled errors (scaled
+        # and relative) are within 2x the plain FP64 GEMM's on these inputs
+        Cp = C0.clone()
+        capi.dgemm(A, B, Cp, 1.0, 1.0)
+        ref = capi.sampled_error(A, B, C0, Cp, 1.0, 1.0, samples=16384, seed=SEED)
+        s = 8
+        for cand in range(4, 9):
+            wsc = capi.SplitWorkspace(n, n, n, cand)
+            Cc = C0.clone()
+            capi.dgemm_split_bf16(A, B, Cc, cand, 1.0, 1.0, workspace=wsc)
+            e = capi.sampled_error(A, B, C0, Cc, 1.0, 1.0, samples=16384, seed=SEED)
+            del wsc, Cc
+            log(f"bench: slices={cand}: scaled {e['max_scaled']:.3e} rel {e['max_rel']:.3e} "
+                f"(plain {ref['max_scaled']:.3e} / {ref['max_rel']:.3e})")
+            if e["max_scaled"] <= 2 * ref["max_scaled"] and e["max_rel"] <= 2 * ref["max_rel"]:
+                s = cand
+                break
+        del Cp
     log(f"bench: split-BF16, slices={s}")
     ws = capi.SplitWorkspace(n, n, n, s)
     Cs = C0.clone()

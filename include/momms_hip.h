@@ -49,7 +49,8 @@ int momms_hip_dgemm(int64_t m, int64_t n, int64_t k, const double* alpha, const 
 /* momms_hip_dgemm with an explicit tile configuration instead of the
  * built-in choice, for tuning: 0 = 128x128 per workgroup (buffer->LDS
  * pipelined when A, B, C are column-major with even leading dimensions and
- * 16-byte aligned A, B), 1 = 64x64, 2 = 128x64, 3 = 128x128 register-staged.
+ * 16-byte aligned A, B), 1 = 64x64, 2 = 128x64, 3 = 128x128 register-staged,
+ * 4 = 128x128 pipelined with a 2-slab ring and two workgroups per CU.
  * Argument 15 is tile_cfg. Same semantics as momms_hip_dgemm. */
 int momms_hip_dgemm_tiled(int64_t m, int64_t n, int64_t k, const double* alpha,
                           const double* A, int64_t rsA, int64_t csA, const double* B,

@@ -40,8 +40,8 @@ dgemm_mfma_kernel(int64_t m, int64_t n, int64_t k, double alpha,
                                                    beta, C, rsC, csC, tm * BM, tn * BN);
 }
 
-template <int STAGES>
-__global__ void __launch_bounds__(256)
+template <int STAGES, int MINB>
+__global__ void __launch_bounds__(256, MINB)
 dgemm_dma_kernel(int64_t m, int64_t n, int64_t k, double alpha, const double* __restrict__ A,
                  int64_t lda, uint32_t a_bytes, const double* __restrict__ B, int64_t ldb,
                  uint32_t b_bytes, double beta, double* __restrict__ C, int64_t ldc,
@@ -112,15 +112,22 @@ hipError_t dgemm_launch(const DgemmArgs& g, int tile_cfg) {
   uint32_t a_bytes = 0, b_bytes = 0;
   if (tile_cfg == 0 && dma_ok(g, a_bytes, b_bytes)) {
     const int64_t tiles = ceil_div(g.m, 128) * ceil_div(g.n, 128);
-    hipLaunchKernelGGL((dgemm_dma_kernel<4>), dim3((unsigned)tiles), dim3(256), 0, g.stream, g.m,
-                       g.n, g.k, g.alpha, g.A, g.csA, a_bytes, g.B, g.csB, b_bytes, g.beta, g.C,
-                       g.csC, (int64_t)8);
+    hipLaunchKernelGGL((dgemm_dma_kernel<4, 1>), dim3((unsigned)tiles), dim3(256), 0, g.stream,
+                       g.m, g.n, g.k, g.alpha, g.A, g.csA, a_bytes, g.B, g.csB, b_bytes, g.beta,
+                       g.C, g.csC, (int64_t)8);
+    return hipGetLastError();
+  }
+  if (tile_cfg == 4 && dma_ok(g, a_bytes, b_bytes)) {
+    const int64_t tiles = ceil_div(g.m, 128) * ceil_div(g.n, 128);
+    hipLaunchKernelGGL((dgemm_dma_kernel<2, 2>), dim3((unsigned)tiles), dim3(256), 0, g.stream,
+                       g.m, g.n, g.k, g.alpha, g.A, g.csA, a_bytes, g.B, g.csB, b_bytes, g.beta,
+                       g.C, g.csC, (int64_t)8);
     return hipGetLastError();
   }
   switch (tile_cfg) {
     case 1: return launch_cfg<64, 64, 16, 2, 2>(g, la, lb, 8);
     case 2: return launch_cfg<128, 64, 16, 2, 2>(g, la, lb, 8);
-    default: return launch_cfg<128, 128, 16, 2, 2>(g, la, lb, 8);
+    default: return launch_cfg<128, 128, 16, 2, 2>(g, la, lb, 8);  // 0/3/4 without DMA
   }
 }
 

@@ -21,8 +21,9 @@ struct DgemmArgs {
 
 // Tile configurations (BM x BN, 4 wavefronts): 0 = 128x128 (the pipelined
 // buffer->LDS form when the operands allow it), 1 = 64x64, 2 = 128x64,
-// 3 = 128x128 register-staged form only.
-constexpr int kDgemmTileConfigs = 4;
+// 3 = 128x128 register-staged form only, 4 = 128x128 buffer->LDS with a
+// 2-slab ring and two workgroups per CU.
+constexpr int kDgemmTileConfigs = 5;
 
 int dgemm_pick_tile(int64_t m, int64_t n, int64_t k);
 hipError_t dgemm_launch(const DgemmArgs& g, int tile_cfg);

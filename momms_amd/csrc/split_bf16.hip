@@ -25,6 +25,8 @@
 //   lane conflict-free straight into the MFMA operand layout.
 #include <math.h>
 
+#include <cstdlib>
+
 #include "mfma.hpp"
 #include "split_bf16.hpp"
 #include "split_tile.hpp"
@@ -39,11 +41,32 @@ int split_chunk(int s) {
 
 static inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
+const SplitConfig kSplitConfig[kSplitConfigs] = {
+    {4, 2, 3, 1},  // 0: 128 x 64 tile, 8 wavefronts, 3-slab ring
+    {2, 2, 2, 2},  // 1: 64 x 64 tile, 4 wavefronts, 2-slab ring, two workgroups per CU
+    {2, 4, 3, 1},  // 2: 64 x 128 tile, 8 wavefronts, 3-slab ring
+};
+
+int split_config(int64_t m, int64_t n, int64_t /*k*/, int /*s*/) {
+  static int forced = [] {
+    const char* e = std::getenv("MOMMS_SPLIT_CFG");
+    const int v = e ? std::atoi(e) : -1;
+    return (v >= 0 && v < kSplitConfigs) ? v : -1;
+  }();
+  if (forced >= 0) return forced;
+  // small problems need more, smaller tiles to fill 256 CUs
+  if (ceil_div(m, 128) * ceil_div(n, 64) < 512) return 1;
+  return 0;
+}
+
 SplitLayout split_layout(int64_t m, int64_t n, int64_t k, int s) {
   SplitLayout L;
   L.s = s;
-  L.mpad = ceil_div(m, kPanelA) * kPanelA;
-  L.npad = ceil_div(n, kPanelB) * kPanelB;
+  L.cfg = split_config(m, n, k, s);
+  L.pa = 32 * kSplitConfig[L.cfg].wm;
+  L.pb = 32 * kSplitConfig[L.cfg].wn;
+  L.mpad = ceil_div(m, L.pa) * L.pa;
+  L.npad = ceil_div(n, L.pb) * L.pb;
   L.kpad = ceil_div(k > 0 ? k : 1, kSlab) * kSlab;
   size_t off = 0;
   L.off_aq = off;
@@ -137,20 +160,21 @@ slice_pack(const double* __restrict__ X, int64_t count, int64_t len, int64_t sv,
                         (int64_t)blockIdx.y * 64);
 }
 
-template <int S>
-__global__ void __launch_bounds__(512)
+template <int S, int CFG>
+__global__ void __launch_bounds__(64 * (CFG == 0 ? 8 : CFG == 1 ? 4 : 8), CFG == 1 ? 2 : 1)
 split_multiply(int64_t m, int64_t n, int64_t kpad, double alpha, double beta,
                double* __restrict__ C, int64_t rsC, int64_t csC,
                const uint16_t* __restrict__ Aq, const uint16_t* __restrict__ Bq,
                const int* __restrict__ EA, const int* __restrict__ EB, int64_t mpad, int64_t npad,
                int slabs_per_chunk) {
-  const int64_t tiles_m = mpad / kPanelA;
-  const int64_t tiles_n = npad / kPanelB;
+  constexpr int WM = CFG == 0 ? 4 : 2, WN = CFG == 2 ? 4 : 2, ST = CFG == 1 ? 2 : 3;
+  const int64_t tiles_m = mpad / (32 * WM);
+  const int64_t tiles_n = npad / (32 * WN);
   const int64_t t = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
   int64_t tm, tn;
   grouped_tile(t, tiles_m, tiles_n, 8, tm, tn);
-  split_tile<S, kPanelA / 32, kPanelB / 32>(m, n, kpad, alpha, beta, C, rsC, csC, Aq, Bq, EA, EB,
-                                            tiles_m, tiles_n, slabs_per_chunk, tm, tn);
+  split_tile<S, WM, WN, ST>(m, n, kpad, alpha, beta, C, rsC, csC, Aq, Bq, EA, EB, tiles_m,
+                            tiles_n, slabs_per_chunk, tm, tn);
 }
 
 // ---------------------------------------------------------------------------
@@ -219,9 +243,13 @@ hipError_t split_prepare(const SplitLayout& L, int64_t m, int64_t n, int64_t k, 
   hipLaunchKernelGGL(exponents, dim3((unsigned)ceil_div(L.npad, 256)), dim3(256), 0, st, mb, n,
                      L.npad, eb);
   if ((e = hipGetLastError()) != hipSuccess) return e;
-  if ((e = slice_launch<kPanelA>(L.s, A, m, k, rsA, csA, ea, L.mpad, L.kpad, aq, st)) != hipSuccess)
-    return e;
-  return slice_launch<kPanelB>(L.s, B, n, k, csB, rsB, eb, L.npad, L.kpad, bq, st);
+  auto slice = [&](int panel, const double* X, int64_t cnt, int64_t sv, int64_t sp, const int* ex,
+                   int64_t vpad, uint16_t* Q) {
+    return panel == 128 ? slice_launch<128>(L.s, X, cnt, k, sv, sp, ex, vpad, L.kpad, Q, st)
+                        : slice_launch<64>(L.s, X, cnt, k, sv, sp, ex, vpad, L.kpad, Q, st);
+  };
+  if ((e = slice(L.pa, A, m, rsA, csA, ea, L.mpad, aq)) != hipSuccess) return e;
+  return slice(L.pb, B, n, csB, rsB, eb, L.npad, bq);
 }
 
 hipError_t split_multiply_launch(const SplitLayout& L, int64_t m, int64_t n, int64_t k,
@@ -231,11 +259,18 @@ hipError_t split_multiply_launch(const SplitLayout& L, int64_t m, int64_t n, int
   const uint16_t* bq = (const uint16_t*)(ws + L.off_bq);
   const int* ea = (const int*)(ws + L.off_ea);
   const int* eb = (const int*)(ws + L.off_eb);
-  const int64_t tiles = (L.mpad / kPanelA) * (L.npad / kPanelB);
+  const int64_t tiles = (L.mpad / L.pa) * (L.npad / L.pb);
   const int spc = split_chunk(L.s) / kSlab;
   const int64_t kp = (k > 0) ? L.kpad : 0;
-  dim3 grid((unsigned)tiles), block(512);
-#define MOMMS_MU(S_) hipLaunchKernelGGL((split_multiply<S_>), grid, block, 0, st, m, n, kp, alpha, beta, C, rsC, csC, aq, bq, ea, eb, L.mpad, L.npad, spc)
+  const int threads = 64 * kSplitConfig[L.cfg].wm * kSplitConfig[L.cfg].wn;
+  dim3 grid((unsigned)tiles), block(threads);
+#define MOMMS_MU2(S_, C_) hipLaunchKernelGGL((split_multiply<S_, C_>), grid, block, 0, st, m, n, kp, alpha, beta, C, rsC, csC, aq, bq, ea, eb, L.mpad, L.npad, spc)
+#define MOMMS_MU(S_)                                  \
+  do {                                                \
+    if (L.cfg == 1) MOMMS_MU2(S_, 1);                 \
+    else if (L.cfg == 2) MOMMS_MU2(S_, 2);            \
+    else MOMMS_MU2(S_, 0);                            \
+  } while (0)
   switch (L.s) {
     case 1: MOMMS_MU(1); break;
     case 2: MOMMS_MU(2); break;
@@ -246,6 +281,7 @@ hipError_t split_multiply_launch(const SplitLayout& L, int64_t m, int64_t n, int
     case 7: MOMMS_MU(7); break;
     default: MOMMS_MU(8); break;
   }
+#undef MOMMS_MU2
 #undef MOMMS_MU
   return hipGetLastError();
 }

@@ -11,9 +11,21 @@ namespace momms {
 // per-row / per-column exponents and the max |x| scratch they come from.
 struct SplitLayout {
   int s;
+  int cfg;         // multiply configuration (split_config), fixes the panel sizes
+  int pa, pb;      // rows of A / columns of B per packed panel (= tile BM, BN)
   int64_t mpad, npad, kpad;
   size_t off_aq, off_bq, off_ea, off_eb, off_maxa, off_maxb, bytes;
 };
+
+// Multiply configurations: wavefront grid WM x WN (tile 32*WM x 32*WN),
+// ring depth, and workgroups per CU the kernel is compiled for.
+struct SplitConfig {
+  int wm, wn, stages, min_blocks;
+};
+constexpr int kSplitConfigs = 3;
+extern const SplitConfig kSplitConfig[kSplitConfigs];
+// The configuration used for a problem (MOMMS_SPLIT_CFG overrides the default).
+int split_config(int64_t m, int64_t n, int64_t k, int s);
 
 SplitLayout split_layout(int64_t m, int64_t n, int64_t k, int s);
 int split_chunk(int s);

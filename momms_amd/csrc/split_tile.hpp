@@ -9,8 +9,6 @@
 
 namespace momms {
 
-constexpr int kPanelA = 128;  // rows of A per packed panel (= multiply tile BM)
-constexpr int kPanelB = 64;   // columns of B per packed panel (= multiply tile BN)
 constexpr int kSlab = 16;     // k per slab (= one v_mfma_f32_32x32x16_bf16 step)
 constexpr int kMaxSlices = 8;
 
@@ -114,11 +112,12 @@ __device__ __forceinline__ void slice_block(const double* __restrict__ X, int64_
 // register staging) through a ring of STAGES slabs, STAGES-1 in flight.
 // Ends after a workgroup barrier (LDS reusable at once); lanes past the edge
 // return early from the epilogue.
-template <int S, int WM, int WN>
+template <int S, int WM, int WN, int STAGES_>
 struct SplitRing {
   static constexpr int BM = 32 * WM, BN = 32 * WN;
   static constexpr int ABLK = BM * kSlab, BBLK = BN * kSlab;  // bf16 per slice-slab block
-  static constexpr int STAGES = 3;
+  static constexpr int STAGES = STAGES_;
+  static_assert(STAGES >= 2 && STAGES <= 4, "ring depth");
 };
 
 __device__ __forceinline__ void lds_dma16(const void* src, void* lds) {
@@ -138,7 +137,7 @@ __device__ __forceinline__ void split_barrier() {
   asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory");
 }
 
-template <int S, int WM, int WN>
+template <int S, int WM, int WN, int STAGES_ = 3>
 __device__ __forceinline__ void split_tile(int64_t m, int64_t n, int64_t kpad, double alpha,
                                            double beta, double* __restrict__ C, int64_t rsC,
                                            int64_t csC, const uint16_t* __restrict__ Aq,
@@ -147,7 +146,7 @@ __device__ __forceinline__ void split_tile(int64_t m, int64_t n, int64_t kpad, d
                                            const int* __restrict__ EB, int64_t tiles_m,
                                            int64_t tiles_n, int slabs_per_chunk, int64_t tm,
                                            int64_t tn) {
-  typedef SplitRing<S, WM, WN> R;
+  typedef SplitRing<S, WM, WN, STAGES_> R;
   constexpr int NW = WM * WN;
   constexpr int BM = R::BM, BN = R::BN, ABLK = R::ABLK, BBLK = R::BBLK, STAGES = R::STAGES;
   constexpr int LA = ABLK / 512, LB = BBLK / 512;  // 1 KiB wavefront loads per slice block
@@ -213,7 +212,11 @@ __device__ __forceinline__ void split_tile(int64_t m, int64_t n, int64_t kpad, d
   int in_chunk = 0;
   for (int64_t ks = 0; ks < nslabs; ++ks) {
     // slab ks has landed once at most (slabs issued after it) x my_loads remain
-    if (ks + 1 < nslabs) {
+    const int64_t later = nslabs - 1 - ks < STAGES - 2 ? nslabs - 1 - ks : STAGES - 2;
+    if (later >= 2) {
+      if (my_loads == LO) split_wait_vmcnt<2 * LO>();
+      else split_wait_vmcnt<2 * LO + 2>();
+    } else if (later == 1) {
       if (my_loads == LO) split_wait_vmcnt<LO>();
       else split_wait_vmcnt<LO + 1>();
     } else {

@@ -78,7 +78,7 @@ def test_ragged_shapes_and_layouts(m, n, k, la, lb, lc):
     check_against_oracle(A, B, C0, C, 1.0, 1.0, k)
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
 def test_every_tile_config(tile):
     rng = np.random.default_rng(SEED + tile)
     m, n, k = 300, 250, 99
