@@ -47,10 +47,10 @@ int momms_hip_dgemm(int64_t m, int64_t n, int64_t k, const double* alpha, const 
                     const double* beta, double* C, int64_t rsC, int64_t csC, void* stream);
 
 /* momms_hip_dgemm with an explicit tile configuration instead of the
- * built-in choice, for tuning: 0 = 128x128 per workgroup (buffer->LDS
- * pipelined when A, B, C are column-major with even leading dimensions and
- * 16-byte aligned A, B), 1 = 64x64, 2 = 128x64, 3 = 128x128 register-staged,
- * 4 = 128x128 pipelined with a 2-slab ring and two workgroups per CU.
+ * built-in choice, for tuning: 0 = 128x128 per workgroup (buffer->LDS with a
+ * 2-slab ring, two workgroups per CU, when A, B, C are column-major with even
+ * leading dimensions and 16-byte aligned A, B), 1 = 64x64, 2 = 128x64,
+ * 3 = 128x128 register-staged, 4 = 128x128 buffer->LDS with a 4-slab ring.
  * Argument 15 is tile_cfg. Same semantics as momms_hip_dgemm. */
 int momms_hip_dgemm_tiled(int64_t m, int64_t n, int64_t k, const double* alpha,
                           const double* A, int64_t rsA, int64_t csA, const double* B,

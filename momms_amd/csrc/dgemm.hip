@@ -112,14 +112,14 @@ hipError_t dgemm_launch(const DgemmArgs& g, int tile_cfg) {
   uint32_t a_bytes = 0, b_bytes = 0;
   if (tile_cfg == 0 && dma_ok(g, a_bytes, b_bytes)) {
     const int64_t tiles = ceil_div(g.m, 128) * ceil_div(g.n, 128);
-    hipLaunchKernelGGL((dgemm_dma_kernel<4, 1>), dim3((unsigned)tiles), dim3(256), 0, g.stream,
+    hipLaunchKernelGGL((dgemm_dma_kernel<2, 2>), dim3((unsigned)tiles), dim3(256), 0, g.stream,
                        g.m, g.n, g.k, g.alpha, g.A, g.csA, a_bytes, g.B, g.csB, b_bytes, g.beta,
                        g.C, g.csC, (int64_t)8);
     return hipGetLastError();
   }
   if (tile_cfg == 4 && dma_ok(g, a_bytes, b_bytes)) {
     const int64_t tiles = ceil_div(g.m, 128) * ceil_div(g.n, 128);
-    hipLaunchKernelGGL((dgemm_dma_kernel<2, 2>), dim3((unsigned)tiles), dim3(256), 0, g.stream,
+    hipLaunchKernelGGL((dgemm_dma_kernel<4, 1>), dim3((unsigned)tiles), dim3(256), 0, g.stream,
                        g.m, g.n, g.k, g.alpha, g.A, g.csA, a_bytes, g.B, g.csB, b_bytes, g.beta,
                        g.C, g.csC, (int64_t)8);
     return hipGetLastError();

@@ -19,10 +19,10 @@ struct DgemmArgs {
   hipStream_t stream;
 };
 
-// Tile configurations (BM x BN, 4 wavefronts): 0 = 128x128 (the pipelined
-// buffer->LDS form when the operands allow it), 1 = 64x64, 2 = 128x64,
-// 3 = 128x128 register-staged form only, 4 = 128x128 buffer->LDS with a
-// 2-slab ring and two workgroups per CU.
+// Tile configurations (BM x BN, 4 wavefronts): 0 = 128x128 (buffer->LDS
+// with a 2-slab ring and two workgroups per CU when the operands allow it),
+// 1 = 64x64, 2 = 128x64, 3 = 128x128 register-staged form only,
+// 4 = 128x128 buffer->LDS with a 4-slab ring, one workgroup per CU.
 constexpr int kDgemmTileConfigs = 5;
 
 int dgemm_pick_tile(int64_t m, int64_t n, int64_t k);

@@ -20,7 +20,9 @@ constexpr int kNonFinite = 0x7fffffff;
 //
 // Vector v (a row of A or a column of B), element p lives at
 // X[v*sv + p*sp]. Output for slice t: panel pv = v / PANEL, slab ks = p / 16,
-//   Q[((t*npanels + pv)*nslabs + ks)*(PANEL*16) + kh*PANEL*8 + (v%PANEL)*8 + p%8]
+//   Q[((pv*nslabs + ks)*S + t)*(PANEL*16) + kh*PANEL*8 + (v%PANEL)*8 + p%8]
+// so the S slice blocks of one slab are one contiguous S*PANEL*32-byte chunk
+// (and nothing else in a tile's stream sits at a power-of-two stride).
 // with kh = (p % 16) / 8. Padding (v >= count or p >= len) is written as 0.
 // A 64 x 64 tile of (v, p) goes through LDS so that both the global read
 // (along whichever index is unit-stride) and the packed write are coalesced.
@@ -48,7 +50,6 @@ __device__ __forceinline__ void slice_block(const double* __restrict__ X, int64_
   }
   __syncthreads();
   const int64_t nslabs = lpad / 16;
-  const int64_t npanels = vpad / PANEL;
   // write: each thread produces 8 consecutive p (one 16-byte chunk) of one v
   // per step; 64 v x 8 chunks = 512 chunks, 2 per thread.
 #pragma unroll
@@ -85,7 +86,7 @@ __device__ __forceinline__ void slice_block(const double* __restrict__ X, int64_
     const int64_t within = (int64_t)kh * PANEL * 8 + (gv % PANEL) * 8;
 #pragma unroll
     for (int t = 0; t < S; ++t) {
-      const int64_t base = (((int64_t)t * npanels + pv) * nslabs + ks) * (PANEL * 16) + within;
+      const int64_t base = ((pv * nslabs + ks) * S + t) * (PANEL * 16) + within;
       uint4 w;
       w.x = (uint32_t)q[t][0] | ((uint32_t)q[t][1] << 16);
       w.y = (uint32_t)q[t][2] | ((uint32_t)q[t][3] << 16);
@@ -163,29 +164,26 @@ __device__ __forceinline__ void split_tile(int64_t m, int64_t n, int64_t kpad, d
   const int my_loads = LO + (wave < LOADS % NW ? 1 : 0);
 
   const int64_t nslabs = kpad / kSlab;
-  const uint16_t* Abase = Aq + tm * nslabs * ABLK;
-  const uint16_t* Bbase = Bq + tn * nslabs * BBLK;
-  const int64_t a_slice_stride = tiles_m * nslabs * ABLK;
-  const int64_t b_slice_stride = tiles_n * nslabs * BBLK;
+  // slab ks of this tile: S contiguous slice blocks at Abase + ks*S*ABLK
+  const uint16_t* Abase = Aq + tm * nslabs * S * ABLK;
+  const uint16_t* Bbase = Bq + tn * nslabs * S * BBLK;
+  (void)tiles_m;
+  (void)tiles_n;
 
   auto issue = [&](int64_t ks, int st) {
     for (int c = wave; c < LOADS; c += NW) {
       if (c < S * LA) {
         const int t = c / LA, q = c % LA;
-        lds_dma16(Abase + t * a_slice_stride + ks * ABLK + q * 512 + lane * 8,
+        lds_dma16(Abase + (ks * S + t) * ABLK + q * 512 + lane * 8,
                   &As[st][t * ABLK + q * 512]);
       } else {
         const int cb = c - S * LA, t = cb / LB, q = cb % LB;
-        lds_dma16(Bbase + t * b_slice_stride + ks * BBLK + q * 512 + lane * 8,
+        lds_dma16(Bbase + (ks * S + t) * BBLK + q * 512 + lane * 8,
                   &Bs[st][t * BBLK + q * 512]);
       }
     }
   };
-  v16f32 lev[S];
-#pragma unroll
-  for (int d = 0; d < S; ++d)
-#pragma unroll
-    for (int r = 0; r < 16; ++r) lev[d][r] = 0.0f;
+  v16f32 lev[S];  // level accumulators; the first slab of a chunk starts them from zero
   double acc[16];
 #pragma unroll
   for (int r = 0; r < 16; ++r) acc[r] = 0.0;
@@ -202,7 +200,6 @@ __device__ __forceinline__ void split_tile(int64_t m, int64_t n, int64_t kpad, d
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
         acc[r] = __builtin_fma((double)lev[d][r], sc, acc[r]);
-        lev[d][r] = 0.0f;
       }
     }
   };
@@ -232,9 +229,18 @@ __device__ __forceinline__ void split_tile(int64_t m, int64_t n, int64_t kpad, d
       bf[s] = *reinterpret_cast<const v8bf16*>(&Bs[cur][s * BBLK + b_off]);
     }
     // A slice ta+1 times B slice d-ta+1 adds to level d+2; the inner loop
-    // walks the levels so consecutive MFMAs use different accumulators.
+    // walks the levels so consecutive MFMAs use different accumulators. On
+    // the first slab of a chunk the A-slice-1 products (one per level) take a
+    // zero accumulator operand instead of clearing the levels after a fold.
+    if (in_chunk == 0) {
 #pragma unroll
-    for (int ta = 0; ta < S; ++ta)
+      for (int d = 0; d < S; ++d) lev[d] = mfma_bf16_32x32x16(bf[d], af[0], v16f32{});
+    } else {
+#pragma unroll
+      for (int d = 0; d < S; ++d) lev[d] = mfma_bf16_32x32x16(bf[d], af[0], lev[d]);
+    }
+#pragma unroll
+    for (int ta = 1; ta < S; ++ta)
 #pragma unroll
       for (int d = ta; d < S; ++d) lev[d] = mfma_bf16_32x32x16(bf[d - ta], af[ta], lev[d]);
     if (++in_chunk == slabs_per_chunk) {
