@@ -42,9 +42,9 @@ int split_chunk(int s) {
 static inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 const SplitConfig kSplitConfig[kSplitConfigs] = {
-    {4, 2, 3, 1},  // 0: 128 x 64 tile, 8 wavefronts, 3-slab ring
-    {2, 2, 2, 2},  // 1: 64 x 64 tile, 4 wavefronts, 2-slab ring, two workgroups per CU
-    {2, 4, 3, 1},  // 2: 64 x 128 tile, 8 wavefronts, 3-slab ring
+    {4, 2},  // 0: 128 x 64 tile, 8 wavefronts
+    {2, 2},  // 1: 64 x 64 tile, 4 wavefronts, 2-slab ring, two workgroups per CU
+    {2, 4},  // 2: 64 x 128 tile, 8 wavefronts
 };
 
 int split_config(int64_t m, int64_t n, int64_t /*k*/, int /*s*/) {
@@ -167,12 +167,17 @@ split_multiply(int64_t m, int64_t n, int64_t kpad, double alpha, double beta,
                const uint16_t* __restrict__ Aq, const uint16_t* __restrict__ Bq,
                const int* __restrict__ EA, const int* __restrict__ EB, int64_t mpad, int64_t npad,
                int slabs_per_chunk) {
-  constexpr int WM = CFG == 0 ? 4 : 2, WN = CFG == 2 ? 4 : 2, ST = CFG == 1 ? 2 : 3;
+  // Ring depth and raster measured at 8192^3: with five or more slices a
+  // 2-slab ring beats 3 and 4 slabs (about 15% at s = 7); with fewer, the
+  // smaller slabs want 4. Row-by-row raster (group 1) for the 128 x 64 tile.
+  constexpr int WM = CFG == 0 ? 4 : 2, WN = CFG == 2 ? 4 : 2;
+  constexpr int ST = (CFG == 1 || S >= 5) ? 2 : 4;
+  constexpr int GROUP = CFG == 0 ? 1 : 8;
   const int64_t tiles_m = mpad / (32 * WM);
   const int64_t tiles_n = npad / (32 * WN);
   const int64_t t = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
   int64_t tm, tn;
-  grouped_tile(t, tiles_m, tiles_n, 8, tm, tn);
+  grouped_tile(t, tiles_m, tiles_n, GROUP, tm, tn);
   split_tile<S, WM, WN, ST>(m, n, kpad, alpha, beta, C, rsC, csC, Aq, Bq, EA, EB, tiles_m,
                             tiles_n, slabs_per_chunk, tm, tn);
 }

@@ -20,7 +20,7 @@ struct SplitLayout {
 // Multiply configurations: wavefront grid WM x WN (tile 32*WM x 32*WN),
 // ring depth, and workgroups per CU the kernel is compiled for.
 struct SplitConfig {
-  int wm, wn, stages, min_blocks;
+  int wm, wn;  // wavefronts along m and n; the tile is 32*wm x 32*wn
 };
 constexpr int kSplitConfigs = 3;
 extern const SplitConfig kSplitConfig[kSplitConfigs];
