@@ -203,7 +203,8 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--quick", action="store_true", help="headline only, no other configs")
     ap.add_argument("--tile", type=int, default=None, help="FP64 tile configuration (--only plain)")
-    ap.add_argument("--only", choices=["plain", "split", "multiply", "batched", "copy"])
+    ap.add_argument("--moduli", type=int, default=14, help="moduli of the residue form (--only crt)")
+    ap.add_argument("--only", choices=["plain", "split", "multiply", "crt", "batched", "copy"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported. This engine "
@@ -246,6 +247,13 @@ def main() -> int:
             ws.prepare(A, B)
             ms = time_loop(lambda: ws.multiply(C, 1.0, 1.0), args.steps, args.warmup)
         print(json.dumps({"only": args.only, "slices": s, "ms": ms, "tflops": flops / ms / 1e9}))
+        return 0
+    if args.only == "crt":
+        ws = capi.CrtWorkspace(n, n, n, args.moduli)
+        ms = time_loop(lambda: capi.dgemm_crt_bf16(A, B, C, args.moduli, 1.0, 1.0, workspace=ws),
+                       args.steps, args.warmup)
+        print(json.dumps({"only": "crt", "moduli": args.moduli, "ms": ms,
+                          "tflops": flops / ms / 1e9}))
         return 0
     if args.only == "batched":
         del A, B, C0, C

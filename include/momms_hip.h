@@ -109,6 +109,32 @@ int momms_hip_split_bf16_multiply(int64_t m, int64_t n, int64_t k, const double*
                                   int slices, const void* workspace, size_t workspace_bytes,
                                   void* stream);
 
+/* ---- FP64-accurate GEMM from BF16 residues (Ozaki scheme II) ---------------
+ * No reference counterpart: the second form of the precision addition. Each
+ * row of A and column of B is scaled by a power of two and rounded to an
+ * integer of beta bits (momms_hip_crt_bf16_beta); the exact integer product
+ * is recovered from its residues modulo `moduli` (1..16) pairwise-coprime odd
+ * numbers <= 511 (momms_hip_crt_bf16_modulus), one BF16 matrix-core GEMM with
+ * exact FP32 accumulation per modulus, by the Chinese remainder theorem on
+ * the GPU; one correctly rounded conversion, the rescaling and
+ * C := beta*C + alpha*(...) follow in FP64. The result equals, bit for bit,
+ * the CPU restatement in oracle/crt.c. A row or column holding Inf/NaN is
+ * computed by an FP64 fma chain in the entries it touches.
+ * Returns -15 when `moduli` is outside 1..16 or too few for k (beta < 1). */
+int momms_hip_crt_bf16_beta(int64_t k, int moduli);     /* -1 if invalid */
+int momms_hip_crt_bf16_modulus(int index);              /* 0 outside 0..15 */
+size_t momms_hip_crt_bf16_workspace_size(int64_t m, int64_t n, int64_t k, int moduli);
+
+/* Arguments 1-14 as momms_hip_dgemm, then moduli (15), workspace (16),
+ * workspace_bytes (17), stream. The workspace is caller-owned device memory
+ * of momms_hip_crt_bf16_workspace_size bytes; calls sharing one must be
+ * ordered on one stream. */
+int momms_hip_dgemm_crt_bf16(int64_t m, int64_t n, int64_t k, const double* alpha,
+                             const double* A, int64_t rsA, int64_t csA, const double* B,
+                             int64_t rsB, int64_t csB, const double* beta, double* C,
+                             int64_t rsC, int64_t csC, int moduli, void* workspace,
+                             size_t workspace_bytes, void* stream);
+
 /* ---- batched uneven GEMMs -----------------------------------------------------
  * No reference counterpart: the batching addition. Problem i of a batch is
  * C_i := beta*C_i + alpha*A_i*B_i with column-major operands at element

@@ -68,9 +68,16 @@ _SIGS: dict[str, list] = {
     "momms_hip_dgemm_batched_loop": [_c_vp, _c_i64, _c_dp, _c_vp, _c_vp, _c_dp, _c_vp, _c_vp],
     "momms_hip_gemm_bf16": [_c_i64, _c_i64, _c_i64, _c_fp, _c_vp, _c_i64, _c_i64, _c_vp, _c_i64,
                             _c_i64, _c_fp, _c_vp, _c_i64, _c_i64, _c_vp],
+    "momms_hip_crt_bf16_beta": [_c_i64, _c_int],
+    "momms_hip_crt_bf16_modulus": [_c_int],
+    "momms_hip_crt_bf16_workspace_size": [_c_i64, _c_i64, _c_i64, _c_int],
+    "momms_hip_dgemm_crt_bf16": [_c_i64, _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_vp,
+                                 _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_int, _c_vp,
+                                 ctypes.c_size_t, _c_vp],
 }
 _RESTYPES = {"momms_last_error_string": ctypes.c_char_p,
              "momms_hip_split_bf16_workspace_size": ctypes.c_size_t,
+             "momms_hip_crt_bf16_workspace_size": ctypes.c_size_t,
              "momms_sample_index": ctypes.c_uint64,
              "momms_hip_batch_plan_describe": ctypes.c_char_p}
 
@@ -335,6 +342,52 @@ def dgemm_split_bf16(A, B, C, slices: int, alpha: float = 1.0, beta: float = 1.0
                                           ctypes.byref(be), c, rsc, csc, slices, ws.ptr,
                                           ws.nbytes, stream_handle(stream))
     check("momms_hip_dgemm_split_bf16", st)
+    return C
+
+
+class CrtWorkspace:
+    """Device workspace of the Chinese-remainder split GEMM for one (m, n, k, moduli)."""
+
+    def __init__(self, m: int, n: int, k: int, moduli: int, device="cuda"):
+        import torch
+
+        self.m, self.n, self.k, self.moduli = m, n, k, moduli
+        self.nbytes = int(lib().momms_hip_crt_bf16_workspace_size(m, n, k, moduli))
+        if self.nbytes == 0:
+            raise ValueError(f"invalid residue problem m={m} n={n} k={k} moduli={moduli}")
+        self.buf = torch.empty(self.nbytes, dtype=torch.uint8, device=device)
+
+    @property
+    def ptr(self) -> int:
+        return self.buf.data_ptr()
+
+
+def crt_beta(k: int, moduli: int) -> int:
+    """Bits of the exponent-aligned integers for k and `moduli` (-1: too few)."""
+    return int(lib().momms_hip_crt_bf16_beta(k, moduli))
+
+
+def crt_moduli(count: int) -> list[int]:
+    return [int(lib().momms_hip_crt_bf16_modulus(i)) for i in range(count)]
+
+
+def dgemm_crt_bf16(A, B, C, moduli: int, alpha: float = 1.0, beta: float = 1.0,
+                   workspace: CrtWorkspace | None = None, stream=None):
+    """FP64-accurate C := beta*C + alpha*A*B from BF16 residues (in place on C)."""
+    _f64((A, "A"), (B, "B"), (C, "C"))
+    m, k = A.shape
+    n = B.shape[1]
+    ws = workspace or CrtWorkspace(m, n, k, moduli, device=A.device)
+    if (ws.m, ws.n, ws.k, ws.moduli) != (m, n, k, moduli):
+        raise ValueError("workspace was made for another problem")
+    a, rsa, csa = _mat(A, m, k, "A")
+    b, rsb, csb = _mat(B, k, n, "B")
+    c, rsc, csc = _mat(C, m, n, "C")
+    al, be = ctypes.c_double(alpha), ctypes.c_double(beta)
+    st = lib().momms_hip_dgemm_crt_bf16(m, n, k, ctypes.byref(al), a, rsa, csa, b, rsb, csb,
+                                        ctypes.byref(be), c, rsc, csc, moduli, ws.ptr,
+                                        ws.nbytes, stream_handle(stream))
+    check("momms_hip_dgemm_crt_bf16", st)
     return C
 
 

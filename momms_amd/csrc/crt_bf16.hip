@@ -1,0 +1,534 @@
+// FP64-accurate GEMM from BF16 residues (Ozaki scheme II: Ozaki, Uchino,
+// Imamura, "Ozaki Scheme II: A GEMM-oriented emulation of floating-point
+// matrix multiplication using an integer modular technique", 2025) on BF16
+// matrix cores.
+//
+// No counterpart in the reference; it is the second form of the precision
+// addition beside the slice form of split_bf16.hip, and needs one BF16 GEMM
+// per modulus instead of s(s+1)/2 slice products. Exact arithmetic (restated
+// on the CPU for bit-exact comparison):
+//   moduli p_1..p_N: the N largest pairwise-coprime odd integers <= 511,
+//   P = p_1...p_N, kb = ceil(log2 k),
+//   beta = min(53, floor((bitlen(P) - 2 - kb) / 2), floor((125 - kb) / 2)).
+//   row i of A: E_i with max_p |a_ip| < 2^E_i; A'_ip = rint(a_ip 2^(beta - E_i));
+//   column j of B likewise (F_j, B'_pj). |A'|, |B'| <= 2^beta.
+//   C' = A'B' exactly: |C'| <= k 2^(2 beta) < P/2, so the residues of C'
+//   modulo p_1..p_N fix it.
+//   C_ij = beta_C C0_ij + alpha * 2^(E_i + F_j - 2 beta) * double(C'_ij)
+//   (double() rounds to nearest, ties to even; fma for the beta_C update,
+//   C not read when beta_C == 0). A row of A or column of B holding Inf or
+//   NaN is not converted: its entries take alpha * (fma chain over p).
+//
+// Device steps:
+//   1. exponents of the rows of A and columns of B (vector_exponents);
+//   2. crt_pack: the balanced residues of A' and B' modulo each p_l
+//      (|r| <= 255, integers bf16 holds exactly) in packed panels: for
+//      modulus l, panel pv and 16-deep k-slab ks, one contiguous block
+//      [kh][PANEL][8] (kh = which half of the slab);
+//   3. crt_multiply: one BF16 GEMM per modulus with FP32 accumulation. A
+//      chunk of 256 k-values adds at most 256 * 255^2 to an accumulator that
+//      starts below 0.51 p in magnitude, so every partial sum is an integer
+//      below 2^24 and exact in any order; after each chunk the accumulator is
+//      reduced modulo p_l (x - p rint(x / p), exact by fma). The tile ends as
+//      the balanced residue of C' modulo p_l (int16);
+//   4. crt_reconstruct: per entry, Garner's algorithm turns the N residues
+//      into balanced mixed-radix digits (each step exact in fp32), Horner's
+//      rule in 128-bit integers gives C', one correctly rounded conversion,
+//      the scaling and the alpha / beta update.
+#include <math.h>
+
+#include "crt_bf16.hpp"
+#include "mfma.hpp"
+#include "split_bf16.hpp"
+#include "split_tile.hpp"
+#include "status.hpp"
+
+namespace momms {
+
+// ---------------------------------------------------------------------------
+// moduli and the constants derived from them
+
+constexpr int kCrtModuli[kCrtMaxModuli] = {511, 509, 507, 505, 503, 499, 493, 491,
+                                           487, 479, 473, 467, 463, 461, 457, 449};
+
+constexpr int mod_inverse(int a, int m) {  // extended Euclid, gcd(a, m) = 1
+  int t = 0, nt = 1, r = m, nr = a % m;
+  while (nr != 0) {
+    const int q = r / nr;
+    const int t2 = t - q * nt;
+    t = nt;
+    nt = t2;
+    const int r2 = r - q * nr;
+    r = nr;
+    nr = r2;
+  }
+  return t < 0 ? t + m : t;
+}
+
+struct CrtTables {
+  float pf[kCrtMaxModuli], invpf[kCrtMaxModuli];
+  double pd[kCrtMaxModuli], invpd[kCrtMaxModuli];
+  float inv[kCrtMaxModuli][kCrtMaxModuli];  // inv[j][l] = p_j^-1 mod p_l, j != l
+};
+
+constexpr CrtTables make_crt_tables() {
+  CrtTables T{};
+  for (int l = 0; l < kCrtMaxModuli; ++l) {
+    T.pf[l] = (float)kCrtModuli[l];
+    T.invpf[l] = 1.0f / (float)kCrtModuli[l];
+    T.pd[l] = (double)kCrtModuli[l];
+    T.invpd[l] = 1.0 / (double)kCrtModuli[l];
+    for (int j = 0; j < kCrtMaxModuli; ++j)
+      T.inv[j][l] = j == l ? 0.0f : (float)mod_inverse(kCrtModuli[j] % kCrtModuli[l], kCrtModuli[l]);
+  }
+  return T;
+}
+
+__constant__ CrtTables kCrt = make_crt_tables();
+
+int crt_modulus(int l) { return (l >= 0 && l < kCrtMaxModuli) ? kCrtModuli[l] : 0; }
+
+static int product_bitlen(int N) {  // exact, in 32-bit limbs
+  uint32_t w[8] = {1, 0, 0, 0, 0, 0, 0, 0};
+  for (int l = 0; l < N; ++l) {
+    uint64_t carry = 0;
+    for (int i = 0; i < 8; ++i) {
+      const uint64_t t = (uint64_t)w[i] * (uint64_t)kCrtModuli[l] + carry;
+      w[i] = (uint32_t)t;
+      carry = t >> 32;
+    }
+  }
+  for (int i = 7; i >= 0; --i)
+    if (w[i]) return 32 * i + (32 - __builtin_clz(w[i]));
+  return 0;
+}
+
+int crt_beta(int64_t k, int N) {
+  if (N < 1 || N > kCrtMaxModuli || k < 0) return -1;
+  int kb = 0;
+  while (((int64_t)1 << kb) < k) ++kb;
+  const int room = product_bitlen(N) - 2 - kb;
+  int b = room >= 0 ? room / 2 : -1;
+  const int cap = (125 - kb) / 2;
+  if (b > cap) b = cap;
+  if (b > 53) b = 53;
+  return b;
+}
+
+static inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+CrtLayout crt_layout(int64_t m, int64_t n, int64_t k, int N) {
+  CrtLayout L;
+  L.nmod = N;
+  L.beta = crt_beta(k, N);
+  L.mpad = ceil_div(m > 0 ? m : 1, kCrtBM) * kCrtBM;
+  L.npad = ceil_div(n > 0 ? n : 1, kCrtBN) * kCrtBN;
+  L.kpad = ceil_div(k > 0 ? k : 1, kCrtKAlign) * kCrtKAlign;
+  size_t off = 0;
+  L.off_ra = off;
+  off = align256(off + (size_t)N * L.mpad * L.kpad * 2);
+  L.off_rb = off;
+  off = align256(off + (size_t)N * L.npad * L.kpad * 2);
+  L.off_rc = off;
+  off = align256(off + (size_t)N * L.mpad * L.npad * 2);
+  L.off_ea = off;
+  off = align256(off + (size_t)L.mpad * 4);
+  L.off_eb = off;
+  off = align256(off + (size_t)L.npad * 4);
+  L.off_maxa = off;
+  off = align256(off + (size_t)L.mpad * 8);
+  L.off_maxb = off;
+  off = align256(off + (size_t)L.npad * 8);
+  L.bytes = off;
+  return L;
+}
+
+// ---------------------------------------------------------------------------
+// 2. residues in packed panels.
+//
+// Vector v (a row of A or a column of B), element p at X[v*sv + p*sp]. For
+// modulus l the residue goes to
+//   R[l*vpad*lpad + (pv*nslabs + ks)*(PANEL*16) + kh*PANEL*8 + (v%PANEL)*8 + p%8]
+// with pv = v / PANEL, ks = p / 16, kh = (p % 16) / 8; padding is 0. A 64 x 64
+// block of (v, p) goes through LDS so the global read (along whichever index
+// is unit-stride) and the 16-byte packed writes are both coalesced.
+
+template <int PANEL>
+__global__ void __launch_bounds__(256)
+crt_pack(const double* __restrict__ X, int64_t count, int64_t len, int64_t sv, int64_t sp,
+         const int* __restrict__ ex, int64_t vpad, int64_t lpad, int nmod, int beta,
+         uint16_t* __restrict__ R) {
+  __shared__ double tile[64][65];
+  const int tid = threadIdx.x;
+  const int64_t v0 = (int64_t)blockIdx.x * 64, p0 = (int64_t)blockIdx.y * 64;
+  const bool v_unit = (sv == 1);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int e = tid + 256 * r;
+    const int a = e & 63, b = e >> 6;  // a runs fastest
+    const int vv = v_unit ? a : b;
+    const int pp = v_unit ? b : a;
+    const int64_t gv = v0 + vv, gp = p0 + pp;
+    double x = 0.0;
+    if (gv < count && gp < len) x = X[gv * sv + gp * sp];
+    tile[vv][pp] = x;
+  }
+  __syncthreads();
+  const int64_t nslabs = lpad / kSlab;
+  const int64_t mod_stride = vpad * lpad;
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int c = tid + 256 * r;
+    const int vv = c & 63;
+    const int pc = c >> 6;  // chunk of 8 along p
+    const int64_t gv = v0 + vv;
+    const int64_t gp = p0 + pc * 8;
+    if (gv >= vpad || gp >= lpad) continue;
+    const int E = (gv < count) ? ex[gv] : 0;
+    const bool finite = (E != kNonFinite);
+    double q[8];
+#pragma unroll
+    for (int e = 0; e < 8; ++e) q[e] = finite ? rint(ldexp(tile[vv][pc * 8 + e], beta - E)) : 0.0;
+    const int64_t pv = gv / PANEL, ks = gp / kSlab;
+    const int kh = (int)((gp % kSlab) / 8);
+    uint16_t* dst = R + (pv * nslabs + ks) * (PANEL * kSlab) + (int64_t)kh * PANEL * 8 +
+                    (gv % PANEL) * 8;
+    for (int l = 0; l < nmod; ++l) {
+      const double p = kCrt.pd[l], invp = kCrt.invpd[l], h = 0.5 * (p - 1.0);
+      uint32_t bits[8];
+#pragma unroll
+      for (int e = 0; e < 8; ++e) {
+        // q - p*rint(q/p) is an integer below 0.51 p in magnitude: exact by fma
+        double rr = __builtin_fma(-rint(q[e] * invp), p, q[e]);
+        rr = rr > h ? rr - p : rr;
+        rr = rr < -h ? rr + p : rr;
+        bits[e] = __float_as_uint((float)rr) >> 16;  // |rr| <= 255: exact in bf16
+      }
+      uint4 w;
+      w.x = bits[0] | (bits[1] << 16);
+      w.y = bits[2] | (bits[3] << 16);
+      w.z = bits[4] | (bits[5] << 16);
+      w.w = bits[6] | (bits[7] << 16);
+      *reinterpret_cast<uint4*>(dst + (int64_t)l * mod_stride) = w;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
+// 3. one residue GEMM per (modulus, tile).
+//
+// Workgroup: WM x WN wavefronts, each FM x FN blocks of 32 x 32 (tile BM x BN,
+// BM = 32*FM*WM, BN = 32*FN*WN). A stage is SPS consecutive 16-deep slabs;
+// stages move HBM -> LDS by global_load_lds_dwordx4 into two LDS buffers
+// that alternate, the next stage in flight while the current one is
+// multiplied. The two buffers are distinct __shared__ objects and the loop is
+// unrolled by two, so every LDS read names a buffer no DMA in flight writes
+// to: the compiler then needs no wait for the DMA before the reads (with one
+// ring array indexed at run time it must wait for every DMA in flight).
+// The MFMA takes the B block as its "A" operand, so lane l holds row
+// i = l%32 of a block and columns 8(r/4) + 4(l/32) + r%4: residue stores
+// along i are coalesced.
+
+#define MOMMS_CRT_STAGE(KS, Ac, Bc, An, Bn)                                                  \
+  {                                                                                          \
+    split_wait_vmcnt<0>(); /* this wavefront's part of stage KS has landed */                \
+    split_barrier();       /* everyone's part has; everyone is done with the other buffer */ \
+    if ((KS) + 1 < nst) issue((KS) + 1, An, Bn);                                             \
+    _Pragma("unroll") for (int ss = 0; ss < SPS; ++ss) {                                     \
+      v8bf16 af[FM], bf[FN];                                                                 \
+      _Pragma("unroll") for (int a = 0; a < FM; ++a) af[a] =                                 \
+          *reinterpret_cast<const v8bf16*>(&Ac[ss * ABLK + a_off + a * 32 * 8]);             \
+      _Pragma("unroll") for (int b = 0; b < FN; ++b) bf[b] =                                 \
+          *reinterpret_cast<const v8bf16*>(&Bc[ss * BBLK + b_off + b * 32 * 8]);             \
+      _Pragma("unroll") for (int a = 0; a < FM; ++a)                                         \
+          _Pragma("unroll") for (int b = 0; b < FN; ++b) acc[a][b] =                         \
+              mfma_bf16_32x32x16(bf[b], af[a], acc[a][b]);                                   \
+    }                                                                                        \
+    if (++in_chunk == CHUNK_STAGES) {                                                        \
+      reduce();                                                                              \
+      in_chunk = 0;                                                                          \
+    }                                                                                        \
+  }
+
+template <int FM, int FN, int WM, int WN, int SPS>
+__global__ void __launch_bounds__(64 * WM * WN, 1)
+crt_multiply(int64_t tiles_m, int64_t tiles_n, int64_t nslabs, int64_t mpad, int64_t npad,
+             const uint16_t* __restrict__ RA, const uint16_t* __restrict__ RB,
+             int16_t* __restrict__ RC) {
+  constexpr int NW = WM * WN, BM = 32 * FM * WM, BN = 32 * FN * WN;
+  constexpr int ABLK = BM * kSlab, BBLK = BN * kSlab;  // bf16 per slab block
+  constexpr int LA = ABLK / 512 * SPS, LB = BBLK / 512 * SPS;  // 1 KiB loads per stage
+  constexpr int LOADS = LA + LB;
+  static_assert(LOADS % NW == 0, "every wavefront issues the same number of loads");
+  static_assert((256 / kSlab) % SPS == 0, "a chunk is whole stages");
+  constexpr int LW = LOADS / NW;
+  constexpr int CHUNK_STAGES = 256 / kSlab / SPS;
+  __shared__ __attribute__((aligned(16))) uint16_t As0[ABLK * SPS];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs0[BBLK * SPS];
+  __shared__ __attribute__((aligned(16))) uint16_t As1[ABLK * SPS];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs1[BBLK * SPS];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  const int64_t tiles = tiles_m * tiles_n;
+  const int64_t t = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
+  const int l = (int)(t / tiles);
+  int64_t tm, tn;
+  grouped_tile(t - (int64_t)l * tiles, tiles_m, tiles_n, 8, tm, tn);
+  const int64_t kpad = nslabs * kSlab;
+  const int64_t nst = nslabs / SPS;  // nslabs is a multiple of SPS
+  const uint16_t* Abase = RA + (int64_t)l * mpad * kpad + tm * nslabs * ABLK;
+  const uint16_t* Bbase = RB + (int64_t)l * npad * kpad + tn * nslabs * BBLK;
+
+  auto issue = [&](int64_t kst, uint16_t* Ad, uint16_t* Bd) {
+#pragma unroll
+    for (int i = 0; i < LW; ++i) {
+      const int c = wave + NW * i;
+      if (c < LA)
+        lds_dma16(Abase + kst * SPS * ABLK + c * 512 + lane * 8, &Ad[c * 512]);
+      else
+        lds_dma16(Bbase + kst * SPS * BBLK + (c - LA) * 512 + lane * 8, &Bd[(c - LA) * 512]);
+    }
+  };
+
+  v16f32 acc[FM][FN];
+#pragma unroll
+  for (int a = 0; a < FM; ++a)
+#pragma unroll
+    for (int b = 0; b < FN; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
+  const float p = kCrt.pf[l], invp = kCrt.invpf[l];
+  auto reduce = [&]() {
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+      for (int b = 0; b < FN; ++b)
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const float x = acc[a][b][r];
+          acc[a][b][r] = __builtin_fmaf(-__builtin_rintf(x * invp), p, x);
+        }
+  };
+
+  const int kh = lane >> 5;
+  const int a_off = kh * BM * 8 + (wm * 32 * FM + (lane & 31)) * 8;
+  const int b_off = kh * BN * 8 + (wn * 32 * FN + (lane & 31)) * 8;
+
+  if (nst > 0) issue(0, As0, Bs0);
+  int in_chunk = 0;
+  for (int64_t ks = 0; ks < nst; ks += 2) {
+    MOMMS_CRT_STAGE(ks, As0, Bs0, As1, Bs1)
+    if (ks + 1 < nst) MOMMS_CRT_STAGE(ks + 1, As1, Bs1, As0, Bs0)
+  }
+  reduce();
+
+  // balanced residue in [-(p-1)/2, (p-1)/2]
+  const float h = 0.5f * (p - 1.0f);
+  int16_t* out = RC + (int64_t)l * mpad * npad;
+#pragma unroll
+  for (int a = 0; a < FM; ++a) {
+    const int64_t gi = tm * BM + wm * 32 * FM + a * 32 + (lane & 31);
+#pragma unroll
+    for (int b = 0; b < FN; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t gj = tn * BN + wn * 32 * FN + b * 32 + 8 * (r >> 2) + 4 * kh + (r & 3);
+        float x = acc[a][b][r];
+        x = x > h ? x - p : x;
+        x = x < -h ? x + p : x;
+        out[gj * mpad + gi] = (int16_t)(int)x;
+      }
+  }
+}
+#undef MOMMS_CRT_STAGE
+
+// ---------------------------------------------------------------------------
+// 4. reconstruction.
+
+// Correctly rounded (to nearest, ties to even) conversion of a signed 128-bit
+// integer: the top 64 bits with the shifted-out bits folded into a sticky
+// bit, converted by the exact-once-rounding u64 -> f64 conversion.
+__device__ __forceinline__ double i128_to_f64(__int128 y) {
+  const bool neg = y < 0;
+  const unsigned __int128 u = neg ? (unsigned __int128)0 - (unsigned __int128)y : (unsigned __int128)y;
+  const uint64_t hi = (uint64_t)(u >> 64), lo = (uint64_t)u;
+  double d;
+  if (hi == 0) {
+    d = (double)lo;
+  } else {
+    const int s = 64 - __builtin_clzll(hi);  // 1..64 bits above the low word
+    const uint64_t sticky_mask = s == 64 ? ~0ull : ((1ull << s) - 1);
+    uint64_t top = (uint64_t)(u >> s);
+    top |= (lo & sticky_mask) != 0 ? 1ull : 0ull;
+    d = ldexp((double)top, s);
+  }
+  return neg ? -d : d;
+}
+
+template <int N>
+__global__ void __launch_bounds__(256)
+crt_reconstruct(int64_t m, int64_t n, int64_t k, int beta_bits, double alpha, double beta,
+                double* __restrict__ C, int64_t rsC, int64_t csC, const int16_t* __restrict__ RC,
+                int64_t mpad, int64_t npad, const int* __restrict__ EA,
+                const int* __restrict__ EB, const double* __restrict__ A, int64_t rsA,
+                int64_t csA, const double* __restrict__ B, int64_t rsB, int64_t csB) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  const int ei = EA[i];
+  for (int64_t j = blockIdx.y; j < n; j += gridDim.y) {
+    const int ej = EB[j];
+    double v;
+    if (ei == kNonFinite || ej == kNonFinite) {
+      double s = 0.0;
+      for (int64_t p = 0; p < k; ++p) s = __builtin_fma(A[i * rsA + p * csA], B[p * rsB + j * csB], s);
+      v = alpha * s;
+    } else {
+      float r[N];
+#pragma unroll
+      for (int l = 0; l < N; ++l) r[l] = (float)RC[(int64_t)l * mpad * npad + j * mpad + i];
+      // Garner: balanced digits d_l with C' = d_0 + p_0 (d_1 + p_1 (d_2 + ...)).
+      // Every intermediate is an integer below 2^19: exact in fp32.
+      float d[N];
+#pragma unroll
+      for (int l = 0; l < N; ++l) {
+        const float pl = kCrt.pf[l], invl = kCrt.invpf[l], hl = 0.5f * (pl - 1.0f);
+        float x = r[l];
+#pragma unroll
+        for (int q = 0; q < l; ++q) {
+          x = (x - d[q]) * kCrt.inv[q][l];
+          x = __builtin_fmaf(-__builtin_rintf(x * invl), pl, x);
+        }
+        x = x > hl ? x - pl : x;
+        x = x < -hl ? x + pl : x;
+        d[l] = x;
+      }
+      __int128 y = (int)d[N - 1];
+#pragma unroll
+      for (int l = N - 2; l >= 0; --l) y = y * (__int128)(int)kCrt.pf[l] + (__int128)(int)d[l];
+      v = alpha * ldexp(i128_to_f64(y), ei + ej - 2 * beta_bits);
+    }
+    double* cp = C + i * rsC + j * csC;
+    *cp = (beta == 0.0) ? v : __builtin_fma(beta, *cp, v);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// host side
+
+constexpr int kCrtFM = 2, kCrtFN = 4, kCrtWM = 4, kCrtWN = 2, kCrtSPS = 2;
+static_assert(32 * kCrtFM * kCrtWM == kCrtBM && 32 * kCrtFN * kCrtWN == kCrtBN, "tile");
+
+hipError_t crt_prepare(const CrtLayout& L, int64_t m, int64_t n, int64_t k, const double* A,
+                       int64_t rsA, int64_t csA, const double* B, int64_t rsB, int64_t csB,
+                       char* ws, hipStream_t st) {
+  uint16_t* ra = (uint16_t*)(ws + L.off_ra);
+  uint16_t* rb = (uint16_t*)(ws + L.off_rb);
+  int* ea = (int*)(ws + L.off_ea);
+  int* eb = (int*)(ws + L.off_eb);
+  uint64_t* ma = (uint64_t*)(ws + L.off_maxa);
+  uint64_t* mb = (uint64_t*)(ws + L.off_maxb);
+  hipError_t e;
+  if ((e = vector_exponents(A, m, k, rsA, csA, L.mpad, ma, ea, st)) != hipSuccess) return e;
+  if ((e = vector_exponents(B, n, k, csB, rsB, L.npad, mb, eb, st)) != hipSuccess) return e;
+  dim3 ga((unsigned)ceil_div(L.mpad, 64), (unsigned)ceil_div(L.kpad, 64)), block(256);
+  hipLaunchKernelGGL((crt_pack<kCrtBM>), ga, block, 0, st, A, m, k, rsA, csA, ea, L.mpad, L.kpad,
+                     L.nmod, L.beta, ra);
+  dim3 gb((unsigned)ceil_div(L.npad, 64), (unsigned)ceil_div(L.kpad, 64));
+  hipLaunchKernelGGL((crt_pack<kCrtBN>), gb, block, 0, st, B, n, k, csB, rsB, eb, L.npad, L.kpad,
+                     L.nmod, L.beta, rb);
+  return hipGetLastError();
+}
+
+hipError_t crt_multiply_launch(const CrtLayout& L, const char* ws, hipStream_t st) {
+  const int64_t tm = L.mpad / kCrtBM, tn = L.npad / kCrtBN;
+  dim3 grid((unsigned)(tm * tn * L.nmod)), block(64 * kCrtWM * kCrtWN);
+  hipLaunchKernelGGL((crt_multiply<kCrtFM, kCrtFN, kCrtWM, kCrtWN, kCrtSPS>), grid, block, 0,
+                     st, tm, tn, L.kpad / kSlab, L.mpad, L.npad,
+                     (const uint16_t*)(ws + L.off_ra), (const uint16_t*)(ws + L.off_rb),
+                     (int16_t*)(ws + L.off_rc));
+  return hipGetLastError();
+}
+
+hipError_t crt_reconstruct_launch(const CrtLayout& L, int64_t m, int64_t n, int64_t k,
+                                  double alpha, double beta, double* C, int64_t rsC, int64_t csC,
+                                  const double* A, int64_t rsA, int64_t csA, const double* B,
+                                  int64_t rsB, int64_t csB, const char* ws, hipStream_t st) {
+  const int16_t* rc = (const int16_t*)(ws + L.off_rc);
+  const int* ea = (const int*)(ws + L.off_ea);
+  const int* eb = (const int*)(ws + L.off_eb);
+  dim3 grid((unsigned)ceil_div(m, 256), (unsigned)(n < 65535 ? n : 65535)), block(256);
+#define MOMMS_RC(N_) hipLaunchKernelGGL((crt_reconstruct<N_>), grid, block, 0, st, m, n, k, L.beta, alpha, beta, C, rsC, csC, rc, L.mpad, L.npad, ea, eb, A, rsA, csA, B, rsB, csB)
+  switch (L.nmod) {
+    case 1: MOMMS_RC(1); break;
+    case 2: MOMMS_RC(2); break;
+    case 3: MOMMS_RC(3); break;
+    case 4: MOMMS_RC(4); break;
+    case 5: MOMMS_RC(5); break;
+    case 6: MOMMS_RC(6); break;
+    case 7: MOMMS_RC(7); break;
+    case 8: MOMMS_RC(8); break;
+    case 9: MOMMS_RC(9); break;
+    case 10: MOMMS_RC(10); break;
+    case 11: MOMMS_RC(11); break;
+    case 12: MOMMS_RC(12); break;
+    case 13: MOMMS_RC(13); break;
+    case 14: MOMMS_RC(14); break;
+    case 15: MOMMS_RC(15); break;
+    default: MOMMS_RC(16); break;
+  }
+#undef MOMMS_RC
+  return hipGetLastError();
+}
+
+}  // namespace momms
+
+// ---------------------------------------------------------------------------
+// C ABI
+
+using namespace momms;
+
+extern "C" int momms_hip_crt_bf16_beta(int64_t k, int moduli) {
+  return (k < 0 || moduli < 1 || moduli > kCrtMaxModuli) ? -1 : crt_beta(k, moduli);
+}
+
+extern "C" int momms_hip_crt_bf16_modulus(int index) { return crt_modulus(index); }
+
+extern "C" size_t momms_hip_crt_bf16_workspace_size(int64_t m, int64_t n, int64_t k,
+                                                    int moduli) {
+  if (m < 0 || n < 0 || k < 0 || moduli < 1 || moduli > kCrtMaxModuli) return 0;
+  if (crt_beta(k, moduli) < 1) return 0;
+  return crt_layout(m, n, k, moduli).bytes;
+}
+
+extern "C" int momms_hip_dgemm_crt_bf16(int64_t m, int64_t n, int64_t k, const double* alpha,
+                                        const double* A, int64_t rsA, int64_t csA,
+                                        const double* B, int64_t rsB, int64_t csB,
+                                        const double* beta, double* C, int64_t rsC,
+                                        int64_t csC, int moduli, void* workspace,
+                                        size_t workspace_bytes, void* stream) {
+  static const char* fn = "momms_hip_dgemm_crt_bf16";
+  if (m < 0) return bad_arg(fn, 1, "m < 0");
+  if (n < 0) return bad_arg(fn, 2, "n < 0");
+  if (k < 0) return bad_arg(fn, 3, "k < 0");
+  if (alpha == nullptr) return bad_arg(fn, 4, "alpha is null");
+  int st;
+  if ((st = check_operand(fn, 5, A, m, k, rsA, csA))) return st;
+  if ((st = check_operand(fn, 8, B, k, n, rsB, csB))) return st;
+  if (beta == nullptr) return bad_arg(fn, 11, "beta is null");
+  if ((st = check_operand(fn, 12, C, m, n, rsC, csC))) return st;
+  if (moduli < 1 || moduli > kCrtMaxModuli) return bad_arg(fn, 15, "moduli must be 1..16");
+  if (crt_beta(k, moduli) < 1) return bad_arg(fn, 15, "too few moduli for this k");
+  if (m == 0 || n == 0) return 0;
+  const CrtLayout L = crt_layout(m, n, k, moduli);
+  if (workspace == nullptr) return bad_arg(fn, 16, "workspace is null");
+  if (workspace_bytes < L.bytes) return bad_arg(fn, 17, "workspace too small");
+  char* ws = (char*)workspace;
+  hipStream_t s = (hipStream_t)stream;
+  hipError_t e = crt_prepare(L, m, n, k, A, rsA, csA, B, rsB, csB, ws, s);
+  if (e == hipSuccess) e = crt_multiply_launch(L, ws, s);
+  if (e == hipSuccess)
+    e = crt_reconstruct_launch(L, m, n, k, *alpha, *beta, C, rsC, csC, A, rsA, csA, B, rsB, csB,
+                               ws, s);
+  return hip_status(fn, e);
+}

@@ -209,6 +209,15 @@ static hipError_t absmax(const double* X, int64_t count, int64_t len, int64_t sv
   return hipGetLastError();
 }
 
+hipError_t vector_exponents(const double* X, int64_t count, int64_t len, int64_t sv, int64_t se,
+                            int64_t padded, uint64_t* mx, int* e, hipStream_t st) {
+  hipError_t err = absmax(X, count, len, sv, se, mx, st);
+  if (err != hipSuccess) return err;
+  hipLaunchKernelGGL(exponents, dim3((unsigned)ceil_div(padded, 256)), dim3(256), 0, st, mx,
+                     count, padded, e);
+  return hipGetLastError();
+}
+
 template <int PANEL>
 static hipError_t slice_launch(int s, const double* X, int64_t count, int64_t len, int64_t sv,
                                int64_t sp, const int* ex, int64_t vpad, int64_t lpad,
@@ -240,14 +249,9 @@ hipError_t split_prepare(const SplitLayout& L, int64_t m, int64_t n, int64_t k, 
   uint64_t* mb = (uint64_t*)(ws + L.off_maxb);
   hipError_t e;
   // rows of A: vector stride rsA, element stride csA
-  if ((e = absmax(A, m, k, rsA, csA, ma, st)) != hipSuccess) return e;
+  if ((e = vector_exponents(A, m, k, rsA, csA, L.mpad, ma, ea, st)) != hipSuccess) return e;
   // columns of B: vector stride csB, element stride rsB
-  if ((e = absmax(B, n, k, csB, rsB, mb, st)) != hipSuccess) return e;
-  hipLaunchKernelGGL(exponents, dim3((unsigned)ceil_div(L.mpad, 256)), dim3(256), 0, st, ma, m,
-                     L.mpad, ea);
-  hipLaunchKernelGGL(exponents, dim3((unsigned)ceil_div(L.npad, 256)), dim3(256), 0, st, mb, n,
-                     L.npad, eb);
-  if ((e = hipGetLastError()) != hipSuccess) return e;
+  if ((e = vector_exponents(B, n, k, csB, rsB, L.npad, mb, eb, st)) != hipSuccess) return e;
   auto slice = [&](int panel, const double* X, int64_t cnt, int64_t sv, int64_t sp, const int* ex,
                    int64_t vpad, uint16_t* Q) {
     return panel == 128 ? slice_launch<128>(L.s, X, cnt, k, sv, sp, ex, vpad, L.kpad, Q, st)

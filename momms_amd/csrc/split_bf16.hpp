@@ -27,6 +27,13 @@ extern const SplitConfig kSplitConfig[kSplitConfigs];
 // The configuration used for a problem (MOMMS_SPLIT_CFG overrides the default).
 int split_config(int64_t m, int64_t n, int64_t k, int s);
 
+// Exponent E of each of count vectors of length len (vector v, element p at
+// X[v*sv + p*se]): max |x| < 2^E (frexp), 0 for a zero vector, kNonFinite for
+// a vector holding Inf or NaN; e[v] = 0 for count <= v < padded. mx is scratch
+// of count words.
+hipError_t vector_exponents(const double* X, int64_t count, int64_t len, int64_t sv, int64_t se,
+                            int64_t padded, uint64_t* mx, int* e, hipStream_t st);
+
 SplitLayout split_layout(int64_t m, int64_t n, int64_t k, int s);
 int split_chunk(int s);
 

@@ -9,7 +9,8 @@
  *            oracle_triple_loop    TripleLoop node
  *   dd.c     oracle_dd_entries / oracle_dd_gemm   double-double (Dot2) values
  *            oracle_residual       randomized residual check
- *   ozaki.c  oracle_ozaki_*        split-BF16 algorithm restatement
+ *   ozaki.c  oracle_ozaki_*        split-BF16 algorithm restatement (slices)
+ *   crt.c    oracle_crt_*          split-BF16 algorithm restatement (residues)
  */
 #ifndef MOMMS_ORACLE_H
 #define MOMMS_ORACLE_H
@@ -47,5 +48,17 @@ void oracle_ozaki_split(int64_t count, int64_t k, const double* X, int64_t strid
 void oracle_ozaki_gemm(int64_t m, int64_t n, int64_t k, const double* alpha, const double* A,
                        int64_t rsA, int64_t csA, const double* B, int64_t rsB, int64_t csB,
                        const double* beta, double* C, int64_t rsC, int64_t csC, int s);
+
+/* Exponent flag of a vector holding Inf or NaN. */
+#define ORACLE_NONFINITE 0x7fffffff
+#define ORACLE_CRT_MAX_MODULI 16
+int oracle_crt_modulus(int l);
+int oracle_crt_beta(int64_t k, int N);
+int oracle_crt_balanced(int64_t x, int p);
+void oracle_crt_integers(int64_t count, int64_t k, const double* X, int64_t sv, int64_t se,
+                         int beta, int64_t* out, int* E);
+int oracle_crt_gemm(int64_t m, int64_t n, int64_t k, const double* alpha, const double* A,
+                    int64_t rsA, int64_t csA, const double* B, int64_t rsB, int64_t csB,
+                    const double* betaC, double* C, int64_t rsC, int64_t csC, int N);
 
 #endif

@@ -173,3 +173,40 @@ def ozaki_gemm(A, B, C, s, alpha=1.0, beta=1.0):
                             _p(B), _i64(rb), _i64(cb), _d(beta), _p(C), _i64(rc), _i64(cc),
                             ctypes.c_int(s))
     return C
+
+
+CRT_MAX_MODULI = 16
+
+
+def crt_moduli(N: int) -> list[int]:
+    return [lib().oracle_crt_modulus(ctypes.c_int(l)) for l in range(N)]
+
+
+def crt_beta(k: int, N: int) -> int:
+    return lib().oracle_crt_beta(_i64(k), ctypes.c_int(N))
+
+
+def crt_integers_rows(A, beta):
+    """Integer images of the rows of A: (m x k int64, exponents int32)."""
+    m, k = A.shape
+    ra, ca = strides(A)
+    out = np.zeros((m, max(k, 1)), dtype=np.int64)
+    e = np.zeros(m, dtype=np.int32)
+    lib().oracle_crt_integers(_i64(m), _i64(k), _p(A), _i64(ra), _i64(ca), ctypes.c_int(beta),
+                              _p(out), _p(e))
+    return out[:, :k], e
+
+
+def crt_gemm(A, B, C, N, alpha=1.0, beta=1.0):
+    """Chinese-remainder split restatement with N moduli, in place on C."""
+    m, k = A.shape
+    n = B.shape[1]
+    ra, ca = strides(A)
+    rb, cb = strides(B)
+    rc, cc = strides(C)
+    st = lib().oracle_crt_gemm(_i64(m), _i64(n), _i64(k), _d(alpha), _p(A), _i64(ra), _i64(ca),
+                               _p(B), _i64(rb), _i64(cb), _d(beta), _p(C), _i64(rc), _i64(cc),
+                               ctypes.c_int(N))
+    if st != 0:
+        raise ValueError(f"oracle_crt_gemm: status {st} (too few moduli for k={k}?)")
+    return C

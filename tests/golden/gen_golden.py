@@ -14,7 +14,10 @@ from the arithmetic the reference's code fixes:
   - "triple": TripleLoop's x,z,y order with c = a*b + c rounded twice
     (src/composables/triple_loop.rs:8-17);
   - "exact": correctly rounded beta*C0 + alpha*A*B (for the double-double oracle);
-  - "ozaki": the library's split-BF16 algorithm (no reference counterpart).
+  - "ozaki": the library's split-BF16 algorithm (no reference counterpart);
+  - "crt": the library's Chinese-remainder split-BF16 algorithm: exact integer
+    images of the exponent-aligned rows/columns, their exact product (Python
+    integers), one correctly rounded conversion, scaling, alpha, fma with beta.
 Every double is stored with float.hex so the vectors are bit exact.
 
 Run: python tests/golden/gen_golden.py
@@ -176,9 +179,48 @@ def ozaki_case(rng, m, n, k, s, alpha, beta, spread):
             "out": hexs(out)}
 
 
+CRT_MODULI = [511, 509, 507, 505, 503, 499, 493, 491, 487, 479, 473, 467, 463, 461, 457, 449]
+
+
+def crt_beta(k, N):
+    P = math.prod(CRT_MODULI[:N])
+    kb = (k - 1).bit_length() if k > 1 else 0
+    room = P.bit_length() - 2 - kb
+    b = room // 2 if room >= 0 else -1
+    return min(b, (125 - kb) // 2, 53)
+
+
+def crt_integers(v, beta):
+    mu = max((abs(x) for x in v), default=0.0)
+    e = math.frexp(mu)[1] if mu > 0 else 0
+    return [round(F(x) * F(2) ** (beta - e)) for x in v], e
+
+
+def crt_case(rng, m, n, k, N, alpha, beta, spread, zero_row=False):
+    A, B, C = rand_mat(rng, m, k, spread), rand_mat(rng, k, n, spread), rand_mat(rng, m, n, spread)
+    if zero_row:
+        A[0, :] = 0.0
+    b = crt_beta(k, N)
+    rows = [crt_integers(list(A[i, :]), b) for i in range(m)]
+    cols = [crt_integers(list(B[:, j]), b) for j in range(n)]
+    out = np.zeros((m, n))
+    for i in range(m):
+        ia, ea = rows[i]
+        for j in range(n):
+            ib, eb = cols[j]
+            c = sum(x * y for x, y in zip(ia, ib))
+            assert 2 * abs(c) < math.prod(CRT_MODULI[:N])
+            v = math.ldexp(float(c), ea + eb - 2 * b)   # float(int) rounds to nearest even
+            v = rn(F(alpha) * F(v))
+            out[i, j] = v if beta == 0.0 else fma(beta, C[i, j], v)
+    return {"m": m, "n": n, "k": k, "N": N, "alpha": float(alpha).hex(),
+            "beta": float(beta).hex(), "A": hexs(A), "B": hexs(B), "C": hexs(C),
+            "out": hexs(out)}
+
+
 def main():
     rng = np.random.default_rng(0x6D6F6D6D73)
-    g = {"ukernel": [], "goto": [], "triple": [], "exact": [], "ozaki": []}
+    g = {"ukernel": [], "goto": [], "triple": [], "exact": [], "ozaki": [], "crt": []}
     for k, al, be in [(0, 1.0, 1.0), (1, 1.0, 1.0), (4, 1.0, 0.0), (5, -2.0, 0.5), (9, 0.75, 1.0),
                       (17, 1.0, -1.5)]:
         g["ukernel"].append(ukernel_case(rng, k, al, be))
@@ -192,6 +234,12 @@ def main():
     for m, n, k, s, al, be, sp in [(3, 2, 300, 3, 1.0, 1.0, 0), (2, 3, 130, 7, 1.0, 1.0, 8),
                                    (2, 2, 50, 4, -1.0, 0.0, 16)]:
         g["ozaki"].append(ozaki_case(rng, m, n, k, s, al, be, sp))
+    for m, n, k, N, al, be, sp, zr in [(3, 2, 300, 14, 1.0, 1.0, 0, False),
+                                       (2, 3, 130, 16, 1.0, 1.0, 16, True),
+                                       (3, 3, 64, 8, -0.75, 0.0, 8, False),
+                                       (2, 2, 4200, 13, 1.0, 2.0, 4, False),
+                                       (2, 2, 0, 14, 1.0, 0.5, 0, False)]:
+        g["crt"].append(crt_case(rng, m, n, k, N, al, be, sp, zr))
     OUT.write_text(json.dumps(g))
     print(f"wrote {OUT} ({OUT.stat().st_size} bytes)")
 

@@ -109,3 +109,31 @@ def test_sample_index_is_splitmix64():
     for e in range(5):
         assert capi.lib().momms_sample_index(7, e, 0, 1000) == sm(7 + 2 * e) % 1000
         assert capi.lib().momms_sample_index(7, e, 1, 999) == sm(7 + 2 * e + 1) % 999
+
+
+def test_crt_constants_match_the_oracle():
+    import pyoracle as O
+    assert capi.crt_moduli(16) == O.crt_moduli(16)
+    assert capi.lib().momms_hip_crt_bf16_modulus(16) == 0
+    for k in (0, 1, 2, 17, 300, 4096, 4097, 8192, 65536, 1 << 20, 1 << 26):
+        for N in range(1, 17):
+            assert capi.crt_beta(k, N) == O.crt_beta(k, N), (k, N)
+    assert capi.crt_beta(10, 0) == -1 and capi.crt_beta(10, 17) == -1
+
+
+def test_crt_entry_point_validates_arguments():
+    al = D(1.0)
+    size = capi.lib().momms_hip_crt_bf16_workspace_size(100, 50, 30, 14)
+    assert size >= 14 * 256 * 32 * 2 * 2
+    assert capi.lib().momms_hip_crt_bf16_workspace_size(100, 50, 30, 17) == 0
+    assert capi.lib().momms_hip_crt_bf16_workspace_size(100, 50, 8192, 1) == 0  # too few moduli
+
+    def call(moduli=14, ws=8, nbytes=size, k=30):
+        return capi.lib().momms_hip_dgemm_crt_bf16(100, 50, k, ctypes.byref(al), 8, 1, 100, 8, 1,
+                                                   k, ctypes.byref(al), 8, 1, 100, moduli, ws,
+                                                   nbytes, None)
+    assert call(moduli=0) == -15 and "moduli" in capi.last_error()
+    assert call(moduli=17) == -15
+    assert call(moduli=1, k=8192) == -15 and "too few" in capi.last_error()
+    assert call(ws=None) == -16
+    assert call(nbytes=size - 1) == -17

@@ -1,0 +1,139 @@
+"""Parity of the Chinese-remainder split-BF16 FP64-accurate GEMM.
+
+Bar: bit-exact against the oracle's restatement (oracle/crt.c computes the
+exponent-aligned integer product directly in 128-bit integers, so equality
+checks the device's residue GEMMs and reconstruction), which
+tests/test_oracle.py pins to exact-integer golden vectors and to the
+double-double oracle. At full sizes the error against double-double values at
+sampled entries is checked against the bound of the integer width beta.
+"""
+import numpy as np
+import pytest
+import torch
+
+import pyoracle as O
+from conftest import SEED
+from momms_amd import capi
+
+pytestmark = pytest.mark.gpu
+
+
+def make(rng, h, w, e=0, layout="col"):
+    x = rng.uniform(-1, 1, (h, w)) * np.exp2(rng.integers(-e, e + 1, (h, w)))
+    if layout == "col":
+        t = torch.empty((w, h), dtype=torch.float64, device="cuda").T
+    elif layout == "row":
+        t = torch.empty((h, w), dtype=torch.float64, device="cuda")
+    else:
+        big = torch.empty((2 * h + 1, 3 * w + 2), dtype=torch.float64, device="cuda")
+        t = big[1::2, 2::3][:h, :w]
+    t.copy_(torch.from_numpy(x))
+    return t
+
+
+def host(t):
+    return np.asfortranarray(t.cpu().numpy())
+
+
+def same_bits(a: np.ndarray, b: np.ndarray) -> bool:
+    return np.array_equal(np.ascontiguousarray(a).view(np.int64), np.ascontiguousarray(b).view(np.int64))
+
+
+def run_both(A, B, C, N, alpha, beta):
+    want = host(C)
+    O.crt_gemm(host(A), host(B), want, N, alpha, beta)
+    capi.dgemm_crt_bf16(A, B, C, N, alpha, beta)
+    torch.cuda.synchronize()
+    return host(C), want
+
+
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (5, 7, 3), (129, 65, 17), (200, 300, 129),
+                                   (257, 129, 1000), (64, 64, 0), (300, 40, 4100)])
+@pytest.mark.parametrize("N", [14, 9, 16])
+def test_bitwise_against_restatement(m, n, k, N):
+    rng = np.random.default_rng(SEED + m + n + k + N)
+    A, B, C = make(rng, m, k, 8), make(rng, k, n, 8), make(rng, m, n, 8)
+    got, want = run_both(A, B, C, N, 1.0, 1.0)
+    assert same_bits(got, want)
+
+
+@pytest.mark.parametrize("layouts", [("row", "row", "row"), ("gen", "col", "gen"),
+                                     ("col", "gen", "row")])
+@pytest.mark.parametrize("e", [0, 16])
+def test_bitwise_layouts_scalars_spread(layouts, e):
+    rng = np.random.default_rng(SEED + e + len(layouts[0]))
+    m, n, k = 150, 90, 300
+    A, B, C = make(rng, m, k, e, layouts[0]), make(rng, k, n, e, layouts[1]), make(rng, m, n, e, layouts[2])
+    got, want = run_both(A, B, C, 14, -0.75, 2.0)
+    assert same_bits(got, want)
+
+
+def test_zero_rows_and_columns_and_beta_zero_does_not_read_c():
+    rng = np.random.default_rng(SEED)
+    m, n, k = 130, 70, 40
+    A, B = make(rng, m, k), make(rng, k, n)
+    A[5, :] = 0.0
+    B[:, 9] = 0.0
+    C = torch.full((n, m), float("nan"), dtype=torch.float64, device="cuda").T
+    capi.dgemm_crt_bf16(A, B, C, 14, 1.0, 0.0)
+    torch.cuda.synchronize()
+    want = np.zeros((m, n), order="F")
+    O.crt_gemm(host(A), host(B), want, 14, 1.0, 0.0)
+    assert same_bits(host(C), want)
+    assert (host(C)[5, :] == 0).all() and (host(C)[:, 9] == 0).all()
+
+
+def test_nonfinite_rows_and_columns_take_the_fp64_path():
+    rng = np.random.default_rng(SEED)
+    m = n = k = 50
+    A, B, C = make(rng, m, k), make(rng, k, n), make(rng, m, n)
+    A[3, 5] = float("nan")
+    B[7, 9] = float("inf")
+    got, want = run_both(A, B, C, 14, 1.0, 1.0)
+    assert np.isnan(got[3, :]).all()
+    assert not np.isfinite(got[:, 9]).any()
+    mask = np.ones((m, n), dtype=bool)
+    mask[3, :] = False
+    mask[:, 9] = False
+    assert np.isfinite(got[mask]).all()
+    assert same_bits(got[mask], want[mask])
+    assert np.array_equal(np.isnan(got), np.isnan(want))
+
+
+def test_workspace_reuse_is_deterministic():
+    rng = np.random.default_rng(SEED)
+    m, n, k = 300, 200, 500
+    A, B, C0 = make(rng, m, k), make(rng, k, n), make(rng, m, n)
+    ws = capi.CrtWorkspace(m, n, k, 14)
+    C1, C2 = C0.clone(), C0.clone()
+    capi.dgemm_crt_bf16(A, B, C1, 14, workspace=ws)
+    capi.dgemm_crt_bf16(A, B, C2, 14, workspace=ws)
+    torch.cuda.synchronize()
+    assert torch.equal(C1, C2)
+
+
+@pytest.mark.parametrize("size", [1024, 4096])
+@pytest.mark.parametrize("e", [0, 8, 16])
+def test_config3_sampled_error_per_moduli(size, e):
+    """Config 3 inputs (U[-1,1)*2^E, E uniform in [-e, e]) at full sizes: the
+    error of each modulus count stays within the bound of its integer width
+    beta (truncation at 2^-beta of the row / column maxima, which exceed a
+    typical entry by up to 2^e each), and at 14 moduli reaches fp64 level."""
+    rng = np.random.default_rng(SEED + size + e)
+    m = n = k = size
+    a = np.asfortranarray(rng.uniform(-1, 1, (m, k)) * np.exp2(rng.integers(-e, e + 1, (m, k))))
+    b = np.asfortranarray(rng.uniform(-1, 1, (k, n)) * np.exp2(rng.integers(-e, e + 1, (k, n))))
+    c0 = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+    A, B, C0 = (torch.from_numpy(x).cuda() for x in (a, b, c0))
+    rows = rng.integers(0, m, 2048)
+    cols = rng.integers(0, n, 2048)
+    ref, mag = O.dd_entries(a, b, c0, rows, cols)
+    for N in (10, 12, 13, 14, 16):
+        beta = capi.crt_beta(k, N)
+        C = C0.clone()
+        capi.dgemm_crt_bf16(A, B, C, N, 1.0, 1.0)
+        torch.cuda.synchronize()
+        got = C.cpu().numpy()[rows, cols]
+        err = float(np.max(np.abs(got - ref) / mag))
+        bound = 4.0 * 2.0 ** (-beta) * 2.0 ** (2 * e) + 4 * 2.0 ** -53
+        assert err <= bound, (N, beta, e, err)

@@ -236,3 +236,125 @@ def test_planted_oracle_mistake_is_caught(golden):
     O.goto_scalar(A2, B, C, float.fromhex(case["alpha"]), float.fromhex(case["beta"]))
     out = unhex(case["out"], m, n)
     assert not np.array_equal(C.view(np.int64), out.view(np.int64))
+
+
+# --- Chinese-remainder split-BF16 restatement ---------------------------------
+
+def _balanced(x, p):
+    r = x % p
+    return r - p if r > (p - 1) // 2 else r
+
+
+def test_crt_golden(golden):
+    for case in golden["crt"]:
+        m, n, k, N = case["m"], case["n"], case["k"], case["N"]
+        A, B, C = unhex(case["A"], m, k), unhex(case["B"], k, n), unhex(case["C"], m, n)
+        al, be = float.fromhex(case["alpha"]), float.fromhex(case["beta"])
+        O.crt_gemm(A, B, C, N, al, be)
+        out = unhex(case["out"], m, n)
+        assert np.array_equal(C.view(np.int64), out.view(np.int64)), (m, n, k, N)
+
+
+def test_crt_moduli_are_the_largest_coprime_odd_set():
+    mods = O.crt_moduli(O.CRT_MAX_MODULI)
+    want, p = [], 511
+    while len(want) < O.CRT_MAX_MODULI:       # greedy from 511 down
+        if p % 2 and all(math.gcd(p, q) == 1 for q in want):
+            want.append(p)
+        p -= 1
+    assert mods == want
+    # balanced residues |r| <= 255 are integers bf16 holds exactly, and a
+    # 256-long chunk of their products (+ a reduced start <= 0.51 p) stays < 2^24
+    assert max(mods) <= 511 and 256 * 255 ** 2 + 0.51 * 511 < 2 ** 24
+
+
+@pytest.mark.parametrize("k", [0, 1, 2, 3, 300, 4096, 4097, 8192, 1 << 20, 1 << 24])
+def test_crt_beta_fits_the_product_of_moduli(k):
+    for N in range(1, O.CRT_MAX_MODULI + 1):
+        b = O.crt_beta(k, N)
+        P = math.prod(O.crt_moduli(N))
+        kb = (k - 1).bit_length() if k > 1 else 0
+        if b < 1:   # even beta = 1 would not fit: too few moduli for this k
+            assert 2 * 2 ** kb * 2 ** 2 > 2 ** (P.bit_length() - 2)
+            continue
+        # |C'| <= k 2^(2b) must lie strictly inside the balanced range and in 2^125
+        assert 2 * max(k, 1) * 2 ** (2 * b) < P
+        assert max(k, 1) * 2 ** (2 * b) <= 2 ** 125
+        # and b is the largest such (or capped at 53)
+        if b < 53:
+            assert 2 * 2 ** kb * 2 ** (2 * (b + 1)) >= 2 ** (P.bit_length() - 1) or \
+                2 ** kb * 2 ** (2 * (b + 1)) > 2 ** 125
+
+
+@pytest.mark.parametrize("e", [0, 16, 300])
+def test_crt_integers_are_exponent_aligned_roundings(e):
+    from fractions import Fraction as Fr
+    rng = np.random.default_rng(SEED + e)
+    A = np.asfortranarray(rng.uniform(-1, 1, (9, 40)) * np.exp2(rng.integers(-e, e + 1, (9, 40))))
+    A[3, :] = 0.0
+    b = 53
+    ia, E = O.crt_integers_rows(A, b)
+    assert E[3] == 0 and (ia[3] == 0).all()
+    for i in range(9):
+        if i == 3:
+            continue
+        mx = np.abs(A[i]).max()
+        assert 2.0 ** (E[i] - 1) <= mx < 2.0 ** E[i]
+        assert np.abs(ia[i]).max() <= 2 ** b and np.abs(ia[i]).max() >= 2 ** (b - 1)
+        for p in range(40):
+            assert ia[i, p] == round(Fr(float(A[i, p])) * Fr(2) ** (b - int(E[i])))
+
+
+def test_crt_residues_recover_the_integer():
+    """The device's reconstruction: balanced residues -> balanced mixed-radix
+    digits (Garner) -> value. Covers (-P/2, P/2) at N = 2 (every 7th integer)
+    and its ends and random members at N = 14..16."""
+    mods = O.crt_moduli(2)
+    P = mods[0] * mods[1]
+
+    def recover(x, mods):
+        r = [_balanced(x, p) for p in mods]
+        v = []
+        for l, p in enumerate(mods):
+            t = r[l]
+            for j in range(l):
+                t = _balanced((t - v[j]) * pow(mods[j], -1, p), p)
+            v.append(t)
+        y = 0
+        for l in range(len(mods) - 1, -1, -1):
+            y = y * mods[l] + v[l]
+        return y
+
+    for x in range(-(P // 2), P // 2 + 1, 7):
+        assert recover(x, mods) == x
+    import random
+    gen = random.Random(SEED)
+    for N in (14, 15, 16):
+        mods = O.crt_moduli(N)
+        half = math.prod(mods) // 2
+        for x in [half, -half] + [gen.randrange(-half, half + 1) for _ in range(300)]:
+            assert recover(x, mods) == x
+
+
+@pytest.mark.parametrize("N,bound", [(8, 2.0 ** -18), (11, 2.0 ** -33), (14, 2.0 ** -51)])
+def test_crt_error_against_double_double(N, bound):
+    rng = np.random.default_rng(SEED + N)
+    m, n, k = 24, 20, 300
+    A, B, C = (uniform_colmajor(rng, *sh) for sh in ((m, k), (k, n), (m, n)))
+    ref, mag = O.dd_gemm(A, B, C)
+    c = C.copy(order="F")
+    O.crt_gemm(A, B, c, N)
+    err = np.abs(c - ref) / mag
+    assert err.max() <= bound
+    if N < 14:
+        assert err.max() > bound / 2 ** 20   # the truncation at beta bits is visible
+
+
+def test_crt_planted_mistake_is_caught(golden):
+    """One modulus too few changes beta and must fail the golden comparison."""
+    case = golden["crt"][0]
+    m, n, k, N = case["m"], case["n"], case["k"], case["N"]
+    A, B, C = unhex(case["A"], m, k), unhex(case["B"], k, n), unhex(case["C"], m, n)
+    O.crt_gemm(A, B, C, N - 2, float.fromhex(case["alpha"]), float.fromhex(case["beta"]))
+    out = unhex(case["out"], m, n)
+    assert not np.array_equal(C.view(np.int64), out.view(np.int64))
