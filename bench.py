@@ -2,23 +2,25 @@
 """Benchmark: FP64-accurate GEMM at m=n=k=8192 on one MI355X.
 
 One step = one C := C + A*B at m=n=k=8192 (column-major f64, entries U[-1,1)
-from a fixed seed, inputs resident in HBM) through the split-BF16 path (the smallest
-slice count whose sampled errors stay within 2x plain FP64's), with the plain FP64-MFMA GEMM timed
-beside it on the same inputs. Prints ONE JSON line:
+from a fixed seed, inputs resident in HBM) through a split-BF16 path, with the
+plain FP64-MFMA GEMM timed beside it on the same inputs. Two split forms run,
+each at the smallest count whose sampled errors (scaled and relative, against
+double-double values) stay within 2x plain FP64's: BF16 slices (Ozaki scheme
+I, s slices, s(s+1)/2 slice products) and BF16 residues (Ozaki scheme II, N
+moduli, N residue GEMMs). Prints ONE JSON line:
 
-  value        split-BF16 FP64-equivalent TFLOP/s (2mnk / time of one whole
-               split GEMM: slicing + multiply)
-  roofline     the dominant kernel (split_multiply, the fused slice-product
-               GEMM) timed alone: P*2mnk slice flops per launch against the
+  value        FP64-equivalent TFLOP/s (2mnk / time of one whole split GEMM,
+               conversion + products + reconstruction) of the faster form
+  roofline     that form's dominant kernel (crt_multiply / split_multiply)
+               timed alone: its BF16 matrix-core flops per launch against the
                dense BF16 MFMA peak
-  plain        the FP64-MFMA GEMM: TFLOP/s and fraction of the FP64 peak
-  accuracy     max scaled / relative error of both against double-double
-               values at 65,536 sampled entries (momms_hip_gemm_check_sampled)
-  configs      the other configurations' lines (sweep, slices x exponent
-               spread, FP32/BF16, batched) unless --quick
+  crt, split   both forms: count, rate, fraction of the FP64 peak, errors
+  plain        the FP64-MFMA GEMM: TFLOP/s, fraction of the FP64 peak, errors
+  configs      the other configurations' lines (sweep, slices/moduli x
+               exponent spread, FP32/BF16, batched) unless --quick
   cpu_baseline the restated Goto + 6x8 CPU path on this host's cores
 
---only {plain,split,multiply,batched,copy} runs one kernel family in a loop
+--only {plain,split,multiply,crt,batched,copy} runs one kernel family in a loop
 (for profilers) and prints nothing else but its timing line.
 """
 from __future__ import annotations
@@ -144,6 +146,30 @@ def split_grid(capi, torch, gen, steps, sizes=(4096, 8192), spreads=(0, 8, 16), 
     return out
 
 
+def crt_grid(capi, torch, gen, steps, sizes=(4096, 8192), spreads=(0, 8, 16),
+             moduli=(10, 12, 13, 14, 16)):
+    """Config 3 inputs through the residue form: error and rate per moduli count."""
+    out = []
+    for n in sizes:
+        for e in spreads:
+            A, B, C0 = rand_colmajor(n, n, gen, e), rand_colmajor(n, n, gen, e), rand_colmajor(n, n, gen)
+            C = C0.clone()
+            for nm in moduli:
+                ws = capi.CrtWorkspace(n, n, n, nm)
+                ms = time_loop(lambda: capi.dgemm_crt_bf16(A, B, C, nm, 1.0, 1.0, workspace=ws), steps, 1)
+                C.copy_(C0)
+                capi.dgemm_crt_bf16(A, B, C, nm, 1.0, 1.0, workspace=ws)
+                err = capi.sampled_error(A, B, C0, C, samples=8192, seed=SEED)
+                tf = 2.0 * n ** 3 / ms / 1e9
+                out.append({"n": n, "e": e, "moduli": nm, "beta_bits": capi.crt_beta(n, nm),
+                            "ms": round(ms, 4), "tflops_fp64_equiv": round(tf, 3),
+                            "frac_fp64_peak": round(tf / PEAK_FP64, 4),
+                            "max_scaled_err": err["max_scaled"], "max_rel_err": err["max_rel"]})
+                del ws
+            del A, B, C0, C
+    return out
+
+
 def reduced_precision(capi, torch, gen, steps, n=8192):
     out = []
     A64, B64 = rand_colmajor(n, n, gen), rand_colmajor(n, n, gen)
@@ -264,12 +290,18 @@ def main() -> int:
     plain_ms = time_loop(lambda: capi.dgemm(A, B, C, 1.0, 1.0), args.steps, args.warmup)
     plain_tf = flops / plain_ms / 1e9
 
+    # FP64-accurate: the smallest slice count (moduli count) whose sampled
+    # errors, scaled and relative, are within 2x the plain FP64 GEMM's on these
+    # inputs, on the same sample set as the reported errors
+    Cp = C0.clone()
+    capi.dgemm(A, B, Cp, 1.0, 1.0)
+    err_p = capi.sampled_error(A, B, C0, Cp, 1.0, 1.0, samples=CHECK_SAMPLES, seed=SEED)
+    del Cp
+
+    def accurate(e):
+        return e["max_scaled"] <= 2 * err_p["max_scaled"] and e["max_rel"] <= 2 * err_p["max_rel"]
+
     if args.slices == 0:
-        # FP64-accurate: the smallest slice count whose sampled errors (scaled
-        # and relative) are within 2x the plain FP64 GEMM's on these inputs
-        Cp = C0.clone()
-        capi.dgemm(A, B, Cp, 1.0, 1.0)
-        ref = capi.sampled_error(A, B, C0, Cp, 1.0, 1.0, samples=CHECK_SAMPLES, seed=SEED)
         s = 8
         for cand in range(4, 9):
             wsc = capi.SplitWorkspace(n, n, n, cand)
@@ -278,57 +310,103 @@ def main() -> int:
             e = capi.sampled_error(A, B, C0, Cc, 1.0, 1.0, samples=CHECK_SAMPLES, seed=SEED)
             del wsc, Cc
             log(f"bench: slices={cand}: scaled {e['max_scaled']:.3e} rel {e['max_rel']:.3e} "
-                f"(plain {ref['max_scaled']:.3e} / {ref['max_rel']:.3e})")
-            if e["max_scaled"] <= 2 * ref["max_scaled"] and e["max_rel"] <= 2 * ref["max_rel"]:
+                f"(plain {err_p['max_scaled']:.3e} / {err_p['max_rel']:.3e})")
+            if accurate(e):
                 s = cand
                 break
-        del Cp
-    log(f"bench: split-BF16, slices={s}")
+    nm = 16
+    for cand in range(10, 17):
+        wsc = capi.CrtWorkspace(n, n, n, cand)
+        Cc = C0.clone()
+        capi.dgemm_crt_bf16(A, B, Cc, cand, 1.0, 1.0, workspace=wsc)
+        e = capi.sampled_error(A, B, C0, Cc, 1.0, 1.0, samples=CHECK_SAMPLES, seed=SEED)
+        del wsc, Cc
+        log(f"bench: moduli={cand}: scaled {e['max_scaled']:.3e} rel {e['max_rel']:.3e}")
+        if accurate(e):
+            nm = cand
+            break
+
+    log(f"bench: split-BF16 slices, s={s}")
     ws = capi.SplitWorkspace(n, n, n, s)
     Cs = C0.clone()
     split_ms = time_loop(lambda: capi.dgemm_split_bf16(A, B, Cs, s, 1.0, 1.0, workspace=ws),
                          args.steps, args.warmup)
     ws.prepare(A, B)
     mult_ms = time_loop(lambda: ws.multiply(Cs, 1.0, 1.0), args.steps, args.warmup)
-    split_tf = flops / split_ms / 1e9
-    P = s * (s + 1) // 2
-    ach = P * flops / mult_ms / 1e9
-
-    log("bench: accuracy (sampled double-double)")
     Cs.copy_(C0)
     capi.dgemm_split_bf16(A, B, Cs, s, 1.0, 1.0, workspace=ws)
-    Cp = C0.clone()
-    capi.dgemm(A, B, Cp, 1.0, 1.0)
     err_s = capi.sampled_error(A, B, C0, Cs, 1.0, 1.0, samples=CHECK_SAMPLES, seed=SEED)
-    err_p = capi.sampled_error(A, B, C0, Cp, 1.0, 1.0, samples=CHECK_SAMPLES, seed=SEED)
-    del ws, Cs, Cp, C
+    del ws, Cs
+    split_tf = flops / split_ms / 1e9
+    P = s * (s + 1) // 2
+
+    log(f"bench: split-BF16 residues, N={nm}")
+    wr = capi.CrtWorkspace(n, n, n, nm)
+    Cr = C0.clone()
+    crt_ms = time_loop(lambda: capi.dgemm_crt_bf16(A, B, Cr, nm, 1.0, 1.0, workspace=wr),
+                       args.steps, args.warmup)
+    wr.prepare(A, B)
+    prod_ms = time_loop(wr.products, args.steps, args.warmup)
+    Cr.copy_(C0)
+    capi.dgemm_crt_bf16(A, B, Cr, nm, 1.0, 1.0, workspace=wr)
+    err_r = capi.sampled_error(A, B, C0, Cr, 1.0, 1.0, samples=CHECK_SAMPLES, seed=SEED)
+    del wr, Cr, C
+    crt_tf = flops / crt_ms / 1e9
+
+    use_crt = crt_tf >= split_tf
+    if use_crt:
+        value, step_ms = crt_tf, crt_ms
+        workload = f"FP64-accurate GEMM C += A*B, m=n=k={n}, split-BF16 residues, {nm} moduli"
+        dtype = "f64 in/out; bf16 residues x bf16 residues -> fp32 (exact, mod p), 128-bit CRT"
+        roof = {"bound": "mfma", "achieved": round(nm * flops / prod_ms / 1e9, 2),
+                "peak": PEAK_BF16, "unit": "TFLOP/s",
+                "frac": round(nm * flops / prod_ms / 1e9 / PEAK_BF16, 4), "traffic": None,
+                "kernel": "crt_multiply (all moduli, one launch)", "ms_per_launch": round(prod_ms, 4),
+                "algorithmic_flops_per_launch": nm * flops}
+    else:
+        value, step_ms = split_tf, split_ms
+        workload = f"FP64-accurate GEMM C += A*B, m=n=k={n}, split-BF16 slices, {s} slices"
+        dtype = "f64 in/out; bf16 slices x bf16 slices -> fp32 (exact), f64 fold"
+        roof = {"bound": "mfma", "achieved": round(P * flops / mult_ms / 1e9, 2),
+                "peak": PEAK_BF16, "unit": "TFLOP/s",
+                "frac": round(P * flops / mult_ms / 1e9 / PEAK_BF16, 4), "traffic": None,
+                "kernel": f"split_multiply<{s}>", "ms_per_launch": round(mult_ms, 4),
+                "algorithmic_flops_per_launch": P * flops}
 
     result = {
         "metric": METRIC,
-        "value": round(split_tf, 3),
+        "value": round(value, 3),
         "unit": "TFLOP/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(split_ms, 4),
+        "ms_per_step": round(step_ms, 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64 in/out; bf16 slices x bf16 slices -> fp32 (exact), f64 fold",
+        "dtype": dtype,
         "data": "synthetic: column-major f64 U[-1,1) from a fixed seed, resident in HBM",
-        "config": {"workload": f"FP64-accurate GEMM C += A*B, m=n=k={n}, split-BF16 with {s} slices",
-                   "m": n, "n": n, "k": n, "slices": s},
-        "roofline": {"bound": "mfma", "achieved": round(ach, 2), "peak": PEAK_BF16,
-                     "unit": "TFLOP/s", "frac": round(ach / PEAK_BF16, 4), "traffic": None,
-                     "kernel": f"split_multiply<{s}>", "ms_per_launch": round(mult_ms, 4),
-                     "algorithmic_flops_per_launch": P * flops},
+        "config": {"workload": workload, "m": n, "n": n, "k": n,
+                   "form": "residues" if use_crt else "slices",
+                   "moduli" if use_crt else "slices": nm if use_crt else s},
+        "roofline": roof,
+        "speedup_vs_plain": round(value / plain_tf, 4),
+        "crt": {"moduli": nm, "beta_bits": capi.crt_beta(n, nm), "gemms": nm,
+                "tflops_fp64_equiv": round(crt_tf, 3), "ms": round(crt_ms, 4),
+                "products_ms": round(prod_ms, 4),
+                "frac_fp64_peak": round(crt_tf / PEAK_FP64, 4),
+                "speedup_vs_plain": round(crt_tf / plain_tf, 4),
+                "max_scaled_err": err_r["max_scaled"], "max_rel_err": err_r["max_rel"]},
         "split": {"slices": s, "products": P, "tflops_fp64_equiv": round(split_tf, 3),
+                  "ms": round(split_ms, 4), "multiply_ms": round(mult_ms, 4),
                   "frac_fp64_peak": round(split_tf / PEAK_FP64, 4),
                   "speedup_vs_plain": round(split_tf / plain_tf, 4),
                   "max_scaled_err": err_s["max_scaled"], "max_rel_err": err_s["max_rel"]},
         "plain": {"tflops": round(plain_tf, 3), "ms": round(plain_ms, 4),
                   "frac_fp64_peak": round(plain_tf / PEAK_FP64, 4),
                   "max_scaled_err": err_p["max_scaled"], "max_rel_err": err_p["max_rel"]},
+        "accuracy_rule": f"max scaled and max relative error over {CHECK_SAMPLES} sampled "
+                         "entries (double-double reference) within 2x plain FP64's",
     }
     del A, B, C0
     torch.cuda.empty_cache()
@@ -339,6 +417,8 @@ def main() -> int:
         cfg["plain_sweep"] = sweep_plain(capi, torch, gen, 3)
         log("bench: config 3 grid (split-BF16: size x spread x slices)")
         cfg["split_grid"] = split_grid(capi, torch, gen, 2)
+        log("bench: config 3 grid (split-BF16 residues: size x spread x moduli)")
+        cfg["crt_grid"] = crt_grid(capi, torch, gen, 2)
         log("bench: config 4 (FP32, BF16->FP32)")
         cfg["reduced_precision"] = reduced_precision(capi, torch, gen, 3)
         log("bench: config 5 (4096 uneven GEMMs)")

@@ -135,6 +135,29 @@ int momms_hip_dgemm_crt_bf16(int64_t m, int64_t n, int64_t k, const double* alph
                              int64_t rsC, int64_t csC, int moduli, void* workspace,
                              size_t workspace_bytes, void* stream);
 
+/* Staged form of momms_hip_dgemm_crt_bf16, for callers that time or overlap
+ * the steps; calls on one workspace with the same m, n, k, moduli must be
+ * ordered on one stream. Stage 1: exponents and packed BF16 residues of A
+ * (rows) and B (columns), the device form of PackA/PackB
+ * (src/composables/pack.rs:311-419). Arguments: m n k A rsA csA B rsB csB
+ * moduli(10) workspace(11) workspace_bytes(12) stream. */
+int momms_hip_crt_bf16_prepare(int64_t m, int64_t n, int64_t k, const double* A, int64_t rsA,
+                               int64_t csA, const double* B, int64_t rsB, int64_t csB,
+                               int moduli, void* workspace, size_t workspace_bytes,
+                               void* stream);
+/* Stage 2: the residue GEMMs, one per modulus, into the workspace.
+ * Arguments: m n k moduli(4) workspace(5) workspace_bytes(6) stream. */
+int momms_hip_crt_bf16_products(int64_t m, int64_t n, int64_t k, int moduli, void* workspace,
+                                size_t workspace_bytes, void* stream);
+/* Stage 3: Chinese remainder reconstruction and C := beta*C + alpha*(...).
+ * A and B are read only for rows / columns holding Inf or NaN. Arguments as
+ * momms_hip_dgemm_crt_bf16 (moduli 15, workspace 16, workspace_bytes 17). */
+int momms_hip_crt_bf16_reconstruct(int64_t m, int64_t n, int64_t k, const double* alpha,
+                                   const double* A, int64_t rsA, int64_t csA, const double* B,
+                                   int64_t rsB, int64_t csB, const double* beta, double* C,
+                                   int64_t rsC, int64_t csC, int moduli, const void* workspace,
+                                   size_t workspace_bytes, void* stream);
+
 /* ---- batched uneven GEMMs -----------------------------------------------------
  * No reference counterpart: the batching addition. Problem i of a batch is
  * C_i := beta*C_i + alpha*A_i*B_i with column-major operands at element

@@ -104,7 +104,8 @@ def build_host_programs() -> list[Path]:
 
 
 def build_test_programs() -> list[Path]:
-    """HIP programs the GPU tests run (tests/native/*.hip -> tests/native/bin/)."""
+    """Programs the tests run: tests/native/*.hip (GPU) and *.cpp (host builds of
+    device arithmetic) -> tests/native/bin/."""
     src_dir = ROOT / "tests" / "native"
     out_dir = src_dir / "bin"
     progs = []
@@ -114,6 +115,13 @@ def build_test_programs() -> list[Path]:
     for src in sorted(src_dir.glob("*.hip")):
         out = out_dir / src.stem
         _run([HIPCC, *HIP_FLAGS, str(src), "-o", str(out)])
+        progs.append(out)
+    # host builds of device-side arithmetic (CPU tests)
+    for src in sorted(src_dir.glob("*.cpp")):
+        out = out_dir / src.stem
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-Wno-unknown-pragmas", "-ffp-contract=off",
+              f"-I{CSRC}", str(src),
+              "-o", str(out)])
         progs.append(out)
     return progs
 

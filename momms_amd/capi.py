@@ -74,6 +74,13 @@ _SIGS: dict[str, list] = {
     "momms_hip_dgemm_crt_bf16": [_c_i64, _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_vp,
                                  _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_int, _c_vp,
                                  ctypes.c_size_t, _c_vp],
+    "momms_hip_crt_bf16_prepare": [_c_i64, _c_i64, _c_i64, _c_vp, _c_i64, _c_i64, _c_vp, _c_i64,
+                                   _c_i64, _c_int, _c_vp, ctypes.c_size_t, _c_vp],
+    "momms_hip_crt_bf16_products": [_c_i64, _c_i64, _c_i64, _c_int, _c_vp, ctypes.c_size_t,
+                                    _c_vp],
+    "momms_hip_crt_bf16_reconstruct": [_c_i64, _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64,
+                                       _c_vp, _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64,
+                                       _c_int, _c_vp, ctypes.c_size_t, _c_vp],
 }
 _RESTYPES = {"momms_last_error_string": ctypes.c_char_p,
              "momms_hip_split_bf16_workspace_size": ctypes.c_size_t,
@@ -360,6 +367,33 @@ class CrtWorkspace:
     @property
     def ptr(self) -> int:
         return self.buf.data_ptr()
+
+    def prepare(self, A, B, stream=None):
+        _f64((A, "A"), (B, "B"))
+        a, rsa, csa = _mat(A, self.m, self.k, "A")
+        b, rsb, csb = _mat(B, self.k, self.n, "B")
+        st = lib().momms_hip_crt_bf16_prepare(self.m, self.n, self.k, a, rsa, csa, b, rsb, csb,
+                                             self.moduli, self.ptr, self.nbytes,
+                                             stream_handle(stream))
+        check("momms_hip_crt_bf16_prepare", st)
+
+    def products(self, stream=None):
+        st = lib().momms_hip_crt_bf16_products(self.m, self.n, self.k, self.moduli, self.ptr,
+                                              self.nbytes, stream_handle(stream))
+        check("momms_hip_crt_bf16_products", st)
+
+    def reconstruct(self, A, B, C, alpha=1.0, beta=1.0, stream=None):
+        _f64((A, "A"), (B, "B"), (C, "C"))
+        a, rsa, csa = _mat(A, self.m, self.k, "A")
+        b, rsb, csb = _mat(B, self.k, self.n, "B")
+        c, rsc, csc = _mat(C, self.m, self.n, "C")
+        al, be = ctypes.c_double(alpha), ctypes.c_double(beta)
+        st = lib().momms_hip_crt_bf16_reconstruct(self.m, self.n, self.k, ctypes.byref(al), a,
+                                                 rsa, csa, b, rsb, csb, ctypes.byref(be), c, rsc,
+                                                 csc, self.moduli, self.ptr, self.nbytes,
+                                                 stream_handle(stream))
+        check("momms_hip_crt_bf16_reconstruct", st)
+        return C
 
 
 def crt_beta(k: int, moduli: int) -> int:

@@ -31,13 +31,14 @@
 //      below 2^24 and exact in any order; after each chunk the accumulator is
 //      reduced modulo p_l (x - p rint(x / p), exact by fma). The tile ends as
 //      the balanced residue of C' modulo p_l (int16);
-//   4. crt_reconstruct: per entry, Garner's algorithm turns the N residues
-//      into balanced mixed-radix digits (each step exact in fp32), Horner's
-//      rule in 128-bit integers gives C', one correctly rounded conversion,
-//      the scaling and the alpha / beta update.
+//   4. crt_reconstruct: per entry, the Chinese remainder sum of the N
+//      residues in 128-bit integers (the quotient by P from an fp64
+//      estimate) gives C', then one correctly rounded conversion, the scaling
+//      and the alpha / beta update.
 #include <math.h>
 
 #include "crt_bf16.hpp"
+#include "crt_recon.hpp"
 #include "mfma.hpp"
 #include "split_bf16.hpp"
 #include "split_tile.hpp"
@@ -48,27 +49,9 @@ namespace momms {
 // ---------------------------------------------------------------------------
 // moduli and the constants derived from them
 
-constexpr int kCrtModuli[kCrtMaxModuli] = {511, 509, 507, 505, 503, 499, 493, 491,
-                                           487, 479, 473, 467, 463, 461, 457, 449};
-
-constexpr int mod_inverse(int a, int m) {  // extended Euclid, gcd(a, m) = 1
-  int t = 0, nt = 1, r = m, nr = a % m;
-  while (nr != 0) {
-    const int q = r / nr;
-    const int t2 = t - q * nt;
-    t = nt;
-    nt = t2;
-    const int r2 = r - q * nr;
-    r = nr;
-    nr = r2;
-  }
-  return t < 0 ? t + m : t;
-}
-
 struct CrtTables {
   float pf[kCrtMaxModuli], invpf[kCrtMaxModuli];
   double pd[kCrtMaxModuli], invpd[kCrtMaxModuli];
-  float inv[kCrtMaxModuli][kCrtMaxModuli];  // inv[j][l] = p_j^-1 mod p_l, j != l
 };
 
 constexpr CrtTables make_crt_tables() {
@@ -78,8 +61,6 @@ constexpr CrtTables make_crt_tables() {
     T.invpf[l] = 1.0f / (float)kCrtModuli[l];
     T.pd[l] = (double)kCrtModuli[l];
     T.invpd[l] = 1.0 / (double)kCrtModuli[l];
-    for (int j = 0; j < kCrtMaxModuli; ++j)
-      T.inv[j][l] = j == l ? 0.0f : (float)mod_inverse(kCrtModuli[j] % kCrtModuli[l], kCrtModuli[l]);
   }
   return T;
 }
@@ -346,25 +327,10 @@ crt_multiply(int64_t tiles_m, int64_t tiles_n, int64_t nslabs, int64_t mpad, int
 // ---------------------------------------------------------------------------
 // 4. reconstruction.
 
-// Correctly rounded (to nearest, ties to even) conversion of a signed 128-bit
-// integer: the top 64 bits with the shifted-out bits folded into a sticky
-// bit, converted by the exact-once-rounding u64 -> f64 conversion.
-__device__ __forceinline__ double i128_to_f64(__int128 y) {
-  const bool neg = y < 0;
-  const unsigned __int128 u = neg ? (unsigned __int128)0 - (unsigned __int128)y : (unsigned __int128)y;
-  const uint64_t hi = (uint64_t)(u >> 64), lo = (uint64_t)u;
-  double d;
-  if (hi == 0) {
-    d = (double)lo;
-  } else {
-    const int s = 64 - __builtin_clzll(hi);  // 1..64 bits above the low word
-    const uint64_t sticky_mask = s == 64 ? ~0ull : ((1ull << s) - 1);
-    uint64_t top = (uint64_t)(u >> s);
-    top |= (lo & sticky_mask) != 0 ? 1ull : 0ull;
-    d = ldexp((double)top, s);
-  }
-  return neg ? -d : d;
-}
+// One thread: kCrtRows consecutive rows i0.. of one column j (the residue
+// matrices are padded to whole tiles, so the 16-byte residue loads stay in
+// bounds); the constants of each modulus are loaded once for all of them.
+constexpr int kCrtRows = 8;
 
 template <int N>
 __global__ void __launch_bounds__(256)
@@ -373,43 +339,40 @@ crt_reconstruct(int64_t m, int64_t n, int64_t k, int beta_bits, double alpha, do
                 int64_t mpad, int64_t npad, const int* __restrict__ EA,
                 const int* __restrict__ EB, const double* __restrict__ A, int64_t rsA,
                 int64_t csA, const double* __restrict__ B, int64_t rsB, int64_t csB) {
-  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= m) return;
-  const int ei = EA[i];
+  constexpr CrtRecon K = crt_recon_constants(N);
+  const int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * kCrtRows;
+  if (i0 >= m) return;
   for (int64_t j = blockIdx.y; j < n; j += gridDim.y) {
-    const int ej = EB[j];
-    double v;
-    if (ei == kNonFinite || ej == kNonFinite) {
-      double s = 0.0;
-      for (int64_t p = 0; p < k; ++p) s = __builtin_fma(A[i * rsA + p * csA], B[p * rsB + j * csB], s);
-      v = alpha * s;
-    } else {
-      float r[N];
+    CrtSums a[kCrtRows];
 #pragma unroll
-      for (int l = 0; l < N; ++l) r[l] = (float)RC[(int64_t)l * mpad * npad + j * mpad + i];
-      // Garner: balanced digits d_l with C' = d_0 + p_0 (d_1 + p_1 (d_2 + ...)).
-      // Every intermediate is an integer below 2^19: exact in fp32.
-      float d[N];
+    for (int e = 0; e < kCrtRows; ++e) crt_sums_init(a[e]);
 #pragma unroll
-      for (int l = 0; l < N; ++l) {
-        const float pl = kCrt.pf[l], invl = kCrt.invpf[l], hl = 0.5f * (pl - 1.0f);
-        float x = r[l];
+    for (int l = 0; l < N; ++l) {
+      const uint4 w = *reinterpret_cast<const uint4*>(RC + (int64_t)l * mpad * npad + j * mpad + i0);
+      const uint32_t h[4] = {w.x, w.y, w.z, w.w};
 #pragma unroll
-        for (int q = 0; q < l; ++q) {
-          x = (x - d[q]) * kCrt.inv[q][l];
-          x = __builtin_fmaf(-__builtin_rintf(x * invl), pl, x);
-        }
-        x = x > hl ? x - pl : x;
-        x = x < -hl ? x + pl : x;
-        d[l] = x;
+      for (int e = 0; e < kCrtRows; ++e) {
+        const int16_t r = (int16_t)(h[e / 2] >> (16 * (e % 2)));
+        crt_sums_add(a[e], (float)r, l, K);
       }
-      __int128 y = (int)d[N - 1];
-#pragma unroll
-      for (int l = N - 2; l >= 0; --l) y = y * (__int128)(int)kCrt.pf[l] + (__int128)(int)d[l];
-      v = alpha * ldexp(i128_to_f64(y), ei + ej - 2 * beta_bits);
     }
-    double* cp = C + i * rsC + j * csC;
-    *cp = (beta == 0.0) ? v : __builtin_fma(beta, *cp, v);
+    const int ej = EB[j];
+#pragma unroll
+    for (int e = 0; e < kCrtRows; ++e) {
+      const int64_t i = i0 + e;
+      if (i >= m) break;
+      const int ei = EA[i];
+      double v;
+      if (ei == kNonFinite || ej == kNonFinite) {
+        double s = 0.0;
+        for (int64_t p = 0; p < k; ++p) s = __builtin_fma(A[i * rsA + p * csA], B[p * rsB + j * csB], s);
+        v = alpha * s;
+      } else {
+        v = alpha * ldexp(crt_i128_to_f64(crt_sums_value(a[e], K)), ei + ej - 2 * beta_bits);
+      }
+      double* cp = C + i * rsC + j * csC;
+      *cp = (beta == 0.0) ? v : __builtin_fma(beta, *cp, v);
+    }
   }
 }
 
@@ -457,7 +420,7 @@ hipError_t crt_reconstruct_launch(const CrtLayout& L, int64_t m, int64_t n, int6
   const int16_t* rc = (const int16_t*)(ws + L.off_rc);
   const int* ea = (const int*)(ws + L.off_ea);
   const int* eb = (const int*)(ws + L.off_eb);
-  dim3 grid((unsigned)ceil_div(m, 256), (unsigned)(n < 65535 ? n : 65535)), block(256);
+  dim3 grid((unsigned)ceil_div(m, 256 * kCrtRows), (unsigned)(n < 65535 ? n : 65535)), block(256);
 #define MOMMS_RC(N_) hipLaunchKernelGGL((crt_reconstruct<N_>), grid, block, 0, st, m, n, k, L.beta, alpha, beta, C, rsC, csC, rc, L.mpad, L.npad, ea, eb, A, rsA, csA, B, rsB, csB)
   switch (L.nmod) {
     case 1: MOMMS_RC(1); break;
@@ -531,4 +494,66 @@ extern "C" int momms_hip_dgemm_crt_bf16(int64_t m, int64_t n, int64_t k, const d
     e = crt_reconstruct_launch(L, m, n, k, *alpha, *beta, C, rsC, csC, A, rsA, csA, B, rsB, csB,
                                ws, s);
   return hip_status(fn, e);
+}
+
+// Staged form: the same three steps as momms_hip_dgemm_crt_bf16, one call each.
+static int crt_check_common(const char* fn, int64_t m, int64_t n, int64_t k, int moduli,
+                            int moduli_arg, const void* workspace, size_t workspace_bytes,
+                            int ws_arg) {
+  if (m < 0) return bad_arg(fn, 1, "m < 0");
+  if (n < 0) return bad_arg(fn, 2, "n < 0");
+  if (k < 0) return bad_arg(fn, 3, "k < 0");
+  if (moduli < 1 || moduli > kCrtMaxModuli) return bad_arg(fn, moduli_arg, "moduli must be 1..16");
+  if (crt_beta(k, moduli) < 1) return bad_arg(fn, moduli_arg, "too few moduli for this k");
+  if (m == 0 || n == 0) return 0;
+  if (workspace == nullptr) return bad_arg(fn, ws_arg, "workspace is null");
+  if (workspace_bytes < crt_layout(m, n, k, moduli).bytes)
+    return bad_arg(fn, ws_arg + 1, "workspace too small");
+  return 0;
+}
+
+extern "C" int momms_hip_crt_bf16_prepare(int64_t m, int64_t n, int64_t k, const double* A,
+                                          int64_t rsA, int64_t csA, const double* B,
+                                          int64_t rsB, int64_t csB, int moduli, void* workspace,
+                                          size_t workspace_bytes, void* stream) {
+  static const char* fn = "momms_hip_crt_bf16_prepare";
+  int st;
+  if ((st = crt_check_common(fn, m, n, k, moduli, 10, workspace, workspace_bytes, 11))) return st;
+  if ((st = check_operand(fn, 4, A, m, k, rsA, csA))) return st;
+  if ((st = check_operand(fn, 7, B, k, n, rsB, csB))) return st;
+  if (m == 0 || n == 0) return 0;
+  return hip_status(fn, crt_prepare(crt_layout(m, n, k, moduli), m, n, k, A, rsA, csA, B, rsB,
+                                    csB, (char*)workspace, (hipStream_t)stream));
+}
+
+extern "C" int momms_hip_crt_bf16_products(int64_t m, int64_t n, int64_t k, int moduli,
+                                           void* workspace, size_t workspace_bytes,
+                                           void* stream) {
+  static const char* fn = "momms_hip_crt_bf16_products";
+  int st;
+  if ((st = crt_check_common(fn, m, n, k, moduli, 4, workspace, workspace_bytes, 5))) return st;
+  if (m == 0 || n == 0) return 0;
+  return hip_status(fn, crt_multiply_launch(crt_layout(m, n, k, moduli), (const char*)workspace,
+                                            (hipStream_t)stream));
+}
+
+extern "C" int momms_hip_crt_bf16_reconstruct(int64_t m, int64_t n, int64_t k,
+                                              const double* alpha, const double* A, int64_t rsA,
+                                              int64_t csA, const double* B, int64_t rsB,
+                                              int64_t csB, const double* beta, double* C,
+                                              int64_t rsC, int64_t csC, int moduli,
+                                              const void* workspace, size_t workspace_bytes,
+                                              void* stream) {
+  static const char* fn = "momms_hip_crt_bf16_reconstruct";
+  int st;
+  if ((st = crt_check_common(fn, m, n, k, moduli, 15, workspace, workspace_bytes, 16))) return st;
+  if (alpha == nullptr) return bad_arg(fn, 4, "alpha is null");
+  if ((st = check_operand(fn, 5, A, m, k, rsA, csA))) return st;
+  if ((st = check_operand(fn, 8, B, k, n, rsB, csB))) return st;
+  if (beta == nullptr) return bad_arg(fn, 11, "beta is null");
+  if ((st = check_operand(fn, 12, C, m, n, rsC, csC))) return st;
+  if (m == 0 || n == 0) return 0;
+  return hip_status(fn, crt_reconstruct_launch(crt_layout(m, n, k, moduli), m, n, k, *alpha,
+                                               *beta, C, rsC, csC, A, rsA, csA, B, rsB, csB,
+                                               (const char*)workspace, (hipStream_t)stream));
 }

@@ -5,9 +5,10 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include "crt_recon.hpp"
+
 namespace momms {
 
-constexpr int kCrtMaxModuli = 16;
 constexpr int kCrtBM = 256, kCrtBN = 256;  // residue GEMM tile = panel sizes
 constexpr int kCrtKAlign = 32;             // k padding: whole two-slab stages
 

@@ -137,3 +137,21 @@ def test_crt_entry_point_validates_arguments():
     assert call(moduli=1, k=8192) == -15 and "too few" in capi.last_error()
     assert call(ws=None) == -16
     assert call(nbytes=size - 1) == -17
+
+
+def test_crt_staged_entry_points_validate_arguments():
+    al = D(1.0)
+    size = capi.lib().momms_hip_crt_bf16_workspace_size(100, 50, 30, 14)
+    L = capi.lib()
+    assert L.momms_hip_crt_bf16_prepare(100, 50, 30, 8, 1, 100, 8, 1, 30, 0, 8, size, None) == -10
+    assert L.momms_hip_crt_bf16_prepare(100, 50, 30, 8, 1, 100, 8, 1, 30, 14, None, size,
+                                        None) == -11
+    assert L.momms_hip_crt_bf16_prepare(100, 50, 30, None, 1, 100, 8, 1, 30, 14, 8, size,
+                                        None) == -4
+    assert L.momms_hip_crt_bf16_products(100, 50, 30, 17, 8, size, None) == -4
+    assert L.momms_hip_crt_bf16_products(100, 50, 30, 14, 8, size - 1, None) == -6
+    assert L.momms_hip_crt_bf16_reconstruct(100, 50, 30, ctypes.byref(al), 8, 1, 100, 8, 1, 30,
+                                            None, 8, 1, 100, 14, 8, size, None) == -11
+    assert L.momms_hip_crt_bf16_reconstruct(100, 50, 30, ctypes.byref(al), 8, 1, 100, 8, 1, 30,
+                                            ctypes.byref(al), 8, 1, 100, 14, 8, 10, None) == -17
+    assert "momms_hip_crt_bf16_reconstruct" in capi.last_error()

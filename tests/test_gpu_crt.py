@@ -100,6 +100,20 @@ def test_nonfinite_rows_and_columns_take_the_fp64_path():
     assert np.array_equal(np.isnan(got), np.isnan(want))
 
 
+def test_staged_equals_one_shot():
+    rng = np.random.default_rng(SEED + 1)
+    m, n, k = 260, 130, 700
+    A, B, C0 = make(rng, m, k, 4), make(rng, k, n, 4), make(rng, m, n)
+    ws = capi.CrtWorkspace(m, n, k, 13)
+    C1, C2 = C0.clone(), C0.clone()
+    capi.dgemm_crt_bf16(A, B, C1, 13, 0.5, -1.0, workspace=ws)
+    ws.prepare(A, B)
+    ws.products()
+    ws.reconstruct(A, B, C2, 0.5, -1.0)
+    torch.cuda.synchronize()
+    assert torch.equal(C1, C2)
+
+
 def test_workspace_reuse_is_deterministic():
     rng = np.random.default_rng(SEED)
     m, n, k = 300, 200, 500
