@@ -17,6 +17,9 @@
 // of the block in 4 columns l/16 + 4r, and each group of 16 lanes covers 16
 // consecutive rows of one column: stores to column-major C are 128-byte
 // coalesced.
+#include <mutex>
+#include <vector>
+
 #include "dgemm.hpp"
 #include "dgemm_tile.hpp"
 #include "mfma.hpp"
@@ -54,6 +57,47 @@ dgemm_dma_kernel(int64_t m, int64_t n, int64_t k, double alpha, const double* __
   grouped_tile(t, tiles_m, tiles_n, group_m, tm, tn);
   dgemm_tile_dma<STAGES>(sm, m, n, k, alpha, make_rsrc(A, a_bytes), lda, make_rsrc(B, b_bytes),
                          ldb, beta, C, ldc, tm * 128, tn * 128);
+}
+
+// Split-k form for shapes with too few tiles to fill the device: workgroup
+// (part, tile) multiplies k-range [part*kper, (part+1)*kper) of the tile into
+// its own slab of the workspace W (m x n column-major per part, alpha = 1,
+// beta = 0); dgemm_splitk_reduce then forms C := beta*C + alpha*(W_0 + ... ),
+// the parts added in order, so the result does not depend on timing.
+template <int STAGES, int MINB>
+__global__ void __launch_bounds__(256, MINB)
+dgemm_dma_splitk_kernel(int64_t m, int64_t n, int64_t k, int64_t kper, const double* __restrict__ A,
+                        int64_t lda, uint64_t a_bytes, const double* __restrict__ B, int64_t ldb,
+                        uint64_t b_bytes, double* __restrict__ W, int64_t group_m) {
+  __shared__ DgemmDmaSmem<STAGES> sm;
+  const int64_t tiles_m = ceil_div(m, 128);
+  const int64_t tiles_n = ceil_div(n, 128);
+  const int64_t tiles = tiles_m * tiles_n;
+  const int64_t t = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
+  const int64_t part = t / tiles;
+  int64_t tm, tn;
+  grouped_tile(t - part * tiles, tiles_m, tiles_n, group_m, tm, tn);
+  const int64_t kb = part * kper;
+  const int64_t kl = (k - kb) < kper ? (k - kb) : kper;
+  // the part's operands start at column kb of A and row kb of B; the buffer
+  // extents shrink by what lies before them
+  dgemm_tile_dma<STAGES>(sm, m, n, kl, 1.0, make_rsrc(A + kb * lda, (uint32_t)(a_bytes - kb * lda * 8)),
+                         lda, make_rsrc(B + kb, (uint32_t)(b_bytes - kb * 8)), ldb, 0.0,
+                         W + part * m * n, m, tm * 128, tn * 128);
+}
+
+__global__ void __launch_bounds__(256)
+dgemm_splitk_reduce(int64_t m, int64_t n, int64_t parts, const double* __restrict__ W,
+                    double alpha, double beta, double* __restrict__ C, int64_t rsC, int64_t csC) {
+  const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= m) return;
+  for (int64_t j = blockIdx.y; j < n; j += gridDim.y) {
+    double s = W[i + j * m];
+    for (int64_t p = 1; p < parts; ++p) s += W[p * m * n + i + j * m];
+    double* cp = C + i * rsC + j * csC;
+    const double v = alpha * s;
+    *cp = (beta == 0.0) ? v : __builtin_fma(beta, *cp, v);
+  }
 }
 
 // The DMA path needs column-major A, B and C, 16-byte aligned columns (for
@@ -103,6 +147,51 @@ static int layout_of(int64_t rs, int64_t cs, int64_t h, int64_t w) {
   return kGen;
 }
 
+// Device workspace of the split-k form, one per (device, stream), grown on
+// demand and kept for the life of the process. Growing synchronizes the
+// stream first, so the old buffer is no longer in use when it is freed.
+static hipError_t splitk_workspace(hipStream_t st, size_t bytes, double** out) {
+  struct Entry {
+    int dev;
+    hipStream_t st;
+    double* p;
+    size_t bytes;
+  };
+  static std::mutex mu;
+  static std::vector<Entry> pool;
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  std::lock_guard<std::mutex> lock(mu);
+  for (Entry& x : pool) {
+    if (x.dev != dev || x.st != st) continue;
+    if (x.bytes < bytes) {
+      if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+      if ((e = hipFree(x.p)) != hipSuccess) return e;
+      x.p = nullptr;
+      x.bytes = 0;
+      if ((e = hipMalloc((void**)&x.p, bytes)) != hipSuccess) return e;
+      x.bytes = bytes;
+    }
+    *out = x.p;
+    return hipSuccess;
+  }
+  double* p = nullptr;
+  if ((e = hipMalloc((void**)&p, bytes)) != hipSuccess) return e;
+  pool.push_back({dev, st, p, bytes});
+  *out = p;
+  return hipSuccess;
+}
+
+// Number of k-parts for the 128 x 128 DMA tile: enough workgroups for two per
+// CU, each part at least 256 deep; 1 = no split.
+int dgemm_splitk_parts(int64_t m, int64_t n, int64_t k) {
+  const int64_t tiles = ceil_div(m, 128) * ceil_div(n, 128);
+  int64_t parts = 1;
+  while (tiles * parts < 512 && k / (parts * 2) >= 256 && parts < 16) parts *= 2;
+  return (int)parts;
+}
+
 hipError_t dgemm_launch(const DgemmArgs& g, int tile_cfg) {
   // A single column (or row) operand is unit-stride along its only index, so
   // it is treated as column (row) major whatever its other stride.
@@ -110,6 +199,24 @@ hipError_t dgemm_launch(const DgemmArgs& g, int tile_cfg) {
   const int lb = layout_of(g.rsB, g.csB, g.k, g.n);
   if (tile_cfg < 0) tile_cfg = dgemm_pick_tile(g.m, g.n, g.k);
   uint32_t a_bytes = 0, b_bytes = 0;
+  if (tile_cfg == 5 && dma_ok(g, a_bytes, b_bytes)) {
+    const int64_t parts = dgemm_splitk_parts(g.m, g.n, g.k);
+    const int64_t kper = ceil_div(ceil_div(g.k, parts), 16) * 16;
+    const int64_t used = ceil_div(g.k, kper);  // parts that hold any k
+    double* W = nullptr;
+    hipError_t e = splitk_workspace(g.stream, (size_t)used * g.m * g.n * 8, &W);
+    if (e != hipSuccess) return e;
+    const int64_t tiles = ceil_div(g.m, 128) * ceil_div(g.n, 128);
+    hipLaunchKernelGGL((dgemm_dma_splitk_kernel<2, 2>), dim3((unsigned)(tiles * used)), dim3(256),
+                       0, g.stream, g.m, g.n, g.k, kper, g.A, g.csA, (uint64_t)a_bytes, g.B,
+                       g.csB, (uint64_t)b_bytes, W, (int64_t)8);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    dim3 grid((unsigned)ceil_div(g.m, 256), (unsigned)(g.n < 65535 ? g.n : 65535));
+    hipLaunchKernelGGL(dgemm_splitk_reduce, grid, dim3(256), 0, g.stream, g.m, g.n, used, W,
+                       g.alpha, g.beta, g.C, g.rsC, g.csC);
+    return hipGetLastError();
+  }
+  if (tile_cfg == 5) tile_cfg = 0;  // operands the DMA path cannot take
   if (tile_cfg == 0 && dma_ok(g, a_bytes, b_bytes)) {
     const int64_t tiles = ceil_div(g.m, 128) * ceil_div(g.n, 128);
     hipLaunchKernelGGL((dgemm_dma_kernel<2, 2>), dim3((unsigned)tiles), dim3(256), 0, g.stream,
@@ -131,10 +238,12 @@ hipError_t dgemm_launch(const DgemmArgs& g, int tile_cfg) {
   }
 }
 
-int dgemm_pick_tile(int64_t m, int64_t n, int64_t /*k*/) {
-  // Enough workgroups to cover the 256 CUs at least once with the big tile;
-  // otherwise fall back to smaller tiles.
+int dgemm_pick_tile(int64_t m, int64_t n, int64_t k) {
+  // Two big-tile workgroups per CU (the DMA kernel runs two per CU); with
+  // fewer tiles, split k when it is deep enough, else use smaller tiles.
   const int64_t big = ceil_div(m, 128) * ceil_div(n, 128);
+  if (big >= 512) return 0;
+  if (dgemm_splitk_parts(m, n, k) > 1) return 5;
   if (big >= 256) return 0;
   const int64_t mid = ceil_div(m, 128) * ceil_div(n, 64);
   if (mid >= 256) return 2;

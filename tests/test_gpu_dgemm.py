@@ -78,7 +78,7 @@ def test_ragged_shapes_and_layouts(m, n, k, la, lb, lc):
     check_against_oracle(A, B, C0, C, 1.0, 1.0, k)
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4])
+@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5])
 def test_every_tile_config(tile):
     rng = np.random.default_rng(SEED + tile)
     m, n, k = 300, 250, 99
@@ -146,7 +146,8 @@ def test_submatrix_view_leaves_rest_untouched():
     assert torch.equal(big[mask], before[mask])
 
 
-@pytest.mark.parametrize("m,n,k", [(4096, 4096, 4096), (8192, 64, 8192), (2048, 2048, 256)])
+@pytest.mark.parametrize("m,n,k", [(4096, 4096, 4096), (8192, 64, 8192), (2048, 2048, 256),
+                                   (1024, 1024, 1024), (1000, 300, 5000)])
 def test_large_sampled_against_dd(m, n, k):
     """Full-size shapes: sampled entries against the double-double oracle, and
     the randomized residual of src/util.rs:55-85 (a size-independent check)."""
@@ -164,3 +165,22 @@ def test_large_sampled_against_dd(m, n, k):
     res = O.residual(a, b, c, c0, 1.0, 1.0, w)
     scale = np.linalg.norm(np.abs(a) @ (np.abs(b) @ np.abs(w)))
     assert np.sqrt(res) <= 4 * gamma(k + n + 2) * scale
+
+
+@pytest.mark.parametrize("m,n,k", [(1024, 1024, 1024), (300, 250, 4099), (8192, 64, 2048)])
+def test_split_k_is_deterministic_and_accurate(m, n, k):
+    """The split-k form (k parts reduced in fixed order): repeat calls give the
+    same bits, and the result is within gamma_{k+1} of double-double."""
+    rng = np.random.default_rng(SEED + m + k)
+    a, b, c0 = (uniform_colmajor(rng, *s) for s in ((m, k), (k, n), (m, n)))
+    A, B = to_dev(a), to_dev(b)
+    C1, C2 = to_dev(c0), to_dev(c0)
+    capi.dgemm(A, B, C1, 0.5, -1.0, tile_cfg=5)
+    capi.dgemm(A, B, C2, 0.5, -1.0, tile_cfg=5)
+    torch.cuda.synchronize()
+    assert torch.equal(C1, C2)
+    rows = rng.integers(0, m, 2048)
+    cols = rng.integers(0, n, 2048)
+    ref, mag = O.dd_entries(a, b, c0, rows, cols, 0.5, -1.0)
+    c = host(C1)
+    assert (np.abs(c[rows, cols] - ref) <= gamma(k + 1) * mag).all()
