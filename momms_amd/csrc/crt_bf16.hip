@@ -39,6 +39,7 @@
 
 #include "crt_bf16.hpp"
 #include "crt_recon.hpp"
+#include "panel_gemm.hpp"
 #include "mfma.hpp"
 #include "split_bf16.hpp"
 #include "split_tile.hpp"
@@ -196,81 +197,30 @@ crt_pack(const double* __restrict__ X, int64_t count, int64_t len, int64_t sv, i
 }
 
 // ---------------------------------------------------------------------------
-// 3. one residue GEMM per (modulus, tile).
-//
-// Workgroup: WM x WN wavefronts, each FM x FN blocks of 32 x 32 (tile BM x BN,
-// BM = 32*FM*WM, BN = 32*FN*WN). A stage is SPS consecutive 16-deep slabs;
-// stages move HBM -> LDS by global_load_lds_dwordx4 into two LDS buffers
-// that alternate, the next stage in flight while the current one is
-// multiplied. The two buffers are distinct __shared__ objects and the loop is
-// unrolled by two, so every LDS read names a buffer no DMA in flight writes
-// to: the compiler then needs no wait for the DMA before the reads (with one
-// ring array indexed at run time it must wait for every DMA in flight).
-// The MFMA takes the B block as its "A" operand, so lane l holds row
-// i = l%32 of a block and columns 8(r/4) + 4(l/32) + r%4: residue stores
-// along i are coalesced.
-
-#define MOMMS_CRT_STAGE(KS, Ac, Bc, An, Bn)                                                  \
-  {                                                                                          \
-    split_wait_vmcnt<0>(); /* this wavefront's part of stage KS has landed */                \
-    split_barrier();       /* everyone's part has; everyone is done with the other buffer */ \
-    if ((KS) + 1 < nst) issue((KS) + 1, An, Bn);                                             \
-    _Pragma("unroll") for (int ss = 0; ss < SPS; ++ss) {                                     \
-      v8bf16 af[FM], bf[FN];                                                                 \
-      _Pragma("unroll") for (int a = 0; a < FM; ++a) af[a] =                                 \
-          *reinterpret_cast<const v8bf16*>(&Ac[ss * ABLK + a_off + a * 32 * 8]);             \
-      _Pragma("unroll") for (int b = 0; b < FN; ++b) bf[b] =                                 \
-          *reinterpret_cast<const v8bf16*>(&Bc[ss * BBLK + b_off + b * 32 * 8]);             \
-      _Pragma("unroll") for (int a = 0; a < FM; ++a)                                         \
-          _Pragma("unroll") for (int b = 0; b < FN; ++b) acc[a][b] =                         \
-              mfma_bf16_32x32x16(bf[b], af[a], acc[a][b]);                                   \
-    }                                                                                        \
-    if (++in_chunk == CHUNK_STAGES) {                                                        \
-      reduce();                                                                              \
-      in_chunk = 0;                                                                          \
-    }                                                                                        \
-  }
+// 3. one residue GEMM per (modulus, tile): the panel GEMM (panel_gemm.hpp)
+// with the modular reduction every 256 k-values, ending as the balanced
+// residue of C' modulo p_l (int16, column-major, leading dimension mpad).
 
 template <int FM, int FN, int WM, int WN, int SPS>
 __global__ void __launch_bounds__(64 * WM * WN, 1)
 crt_multiply(int64_t tiles_m, int64_t tiles_n, int64_t nslabs, int64_t mpad, int64_t npad,
              const uint16_t* __restrict__ RA, const uint16_t* __restrict__ RB,
              int16_t* __restrict__ RC) {
-  constexpr int NW = WM * WN, BM = 32 * FM * WM, BN = 32 * FN * WN;
-  constexpr int ABLK = BM * kSlab, BBLK = BN * kSlab;  // bf16 per slab block
-  constexpr int LA = ABLK / 512 * SPS, LB = BBLK / 512 * SPS;  // 1 KiB loads per stage
-  constexpr int LOADS = LA + LB;
-  static_assert(LOADS % NW == 0, "every wavefront issues the same number of loads");
-  static_assert((256 / kSlab) % SPS == 0, "a chunk is whole stages");
-  constexpr int LW = LOADS / NW;
-  constexpr int CHUNK_STAGES = 256 / kSlab / SPS;
-  __shared__ __attribute__((aligned(16))) uint16_t As0[ABLK * SPS];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs0[BBLK * SPS];
-  __shared__ __attribute__((aligned(16))) uint16_t As1[ABLK * SPS];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs1[BBLK * SPS];
+  typedef PanelGemm<FM, FN, WM, WN, SPS> G;
+  __shared__ __attribute__((aligned(16))) uint16_t As0[G::ABLK * SPS];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs0[G::BBLK * SPS];
+  __shared__ __attribute__((aligned(16))) uint16_t As1[G::ABLK * SPS];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs1[G::BBLK * SPS];
 
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wm = wave % WM, wn = wave / WM;
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave % WM, wn = wave / WM, kh = lane >> 5;
   const int64_t tiles = tiles_m * tiles_n;
   const int64_t t = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
   const int l = (int)(t / tiles);
   int64_t tm, tn;
   grouped_tile(t - (int64_t)l * tiles, tiles_m, tiles_n, 8, tm, tn);
   const int64_t kpad = nslabs * kSlab;
-  const int64_t nst = nslabs / SPS;  // nslabs is a multiple of SPS
-  const uint16_t* Abase = RA + (int64_t)l * mpad * kpad + tm * nslabs * ABLK;
-  const uint16_t* Bbase = RB + (int64_t)l * npad * kpad + tn * nslabs * BBLK;
-
-  auto issue = [&](int64_t kst, uint16_t* Ad, uint16_t* Bd) {
-#pragma unroll
-    for (int i = 0; i < LW; ++i) {
-      const int c = wave + NW * i;
-      if (c < LA)
-        lds_dma16(Abase + kst * SPS * ABLK + c * 512 + lane * 8, &Ad[c * 512]);
-      else
-        lds_dma16(Bbase + kst * SPS * BBLK + (c - LA) * 512 + lane * 8, &Bd[(c - LA) * 512]);
-    }
-  };
+  const float p = kCrt.pf[l], invp = kCrt.invpf[l];
 
   v16f32 acc[FM][FN];
 #pragma unroll
@@ -279,42 +229,22 @@ crt_multiply(int64_t tiles_m, int64_t tiles_n, int64_t nslabs, int64_t mpad, int
     for (int b = 0; b < FN; ++b)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
-  const float p = kCrt.pf[l], invp = kCrt.invpf[l];
-  auto reduce = [&]() {
-#pragma unroll
-    for (int a = 0; a < FM; ++a)
-#pragma unroll
-      for (int b = 0; b < FN; ++b)
-#pragma unroll
-        for (int r = 0; r < 16; ++r) {
-          const float x = acc[a][b][r];
-          acc[a][b][r] = __builtin_fmaf(-__builtin_rintf(x * invp), p, x);
-        }
-  };
-
-  const int kh = lane >> 5;
-  const int a_off = kh * BM * 8 + (wm * 32 * FM + (lane & 31)) * 8;
-  const int b_off = kh * BN * 8 + (wn * 32 * FN + (lane & 31)) * 8;
-
-  if (nst > 0) issue(0, As0, Bs0);
-  int in_chunk = 0;
-  for (int64_t ks = 0; ks < nst; ks += 2) {
-    MOMMS_CRT_STAGE(ks, As0, Bs0, As1, Bs1)
-    if (ks + 1 < nst) MOMMS_CRT_STAGE(ks + 1, As1, Bs1, As0, Bs0)
-  }
-  reduce();
+  panel_mainloop<FM, FN, WM, WN, SPS, true>(
+      acc, RA + (int64_t)l * mpad * kpad + tm * nslabs * G::ABLK,
+      RB + (int64_t)l * npad * kpad + tn * nslabs * G::BBLK, nslabs, p, invp, As0, Bs0, As1, Bs1);
+  panel_reduce_mod<FM, FN>(acc, p, invp);
 
   // balanced residue in [-(p-1)/2, (p-1)/2]
   const float h = 0.5f * (p - 1.0f);
   int16_t* out = RC + (int64_t)l * mpad * npad;
 #pragma unroll
   for (int a = 0; a < FM; ++a) {
-    const int64_t gi = tm * BM + wm * 32 * FM + a * 32 + (lane & 31);
+    const int64_t gi = tm * G::BM + wm * 32 * FM + a * 32 + (lane & 31);
 #pragma unroll
     for (int b = 0; b < FN; ++b)
 #pragma unroll
       for (int r = 0; r < 16; ++r) {
-        const int64_t gj = tn * BN + wn * 32 * FN + b * 32 + 8 * (r >> 2) + 4 * kh + (r & 3);
+        const int64_t gj = tn * G::BN + wn * 32 * FN + b * 32 + 8 * (r >> 2) + 4 * kh + (r & 3);
         float x = acc[a][b][r];
         x = x > h ? x - p : x;
         x = x < -h ? x + p : x;
@@ -322,7 +252,6 @@ crt_multiply(int64_t tiles_m, int64_t tiles_n, int64_t nslabs, int64_t mpad, int
       }
   }
 }
-#undef MOMMS_CRT_STAGE
 
 // ---------------------------------------------------------------------------
 // 4. reconstruction.

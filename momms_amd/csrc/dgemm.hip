@@ -17,12 +17,10 @@
 // of the block in 4 columns l/16 + 4r, and each group of 16 lanes covers 16
 // consecutive rows of one column: stores to column-major C are 128-byte
 // coalesced.
-#include <mutex>
-#include <vector>
-
 #include "dgemm.hpp"
 #include "dgemm_tile.hpp"
 #include "mfma.hpp"
+#include "scratch.hpp"
 #include "status.hpp"
 
 namespace momms {
@@ -147,42 +145,6 @@ static int layout_of(int64_t rs, int64_t cs, int64_t h, int64_t w) {
   return kGen;
 }
 
-// Device workspace of the split-k form, one per (device, stream), grown on
-// demand and kept for the life of the process. Growing synchronizes the
-// stream first, so the old buffer is no longer in use when it is freed.
-static hipError_t splitk_workspace(hipStream_t st, size_t bytes, double** out) {
-  struct Entry {
-    int dev;
-    hipStream_t st;
-    double* p;
-    size_t bytes;
-  };
-  static std::mutex mu;
-  static std::vector<Entry> pool;
-  int dev = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e != hipSuccess) return e;
-  std::lock_guard<std::mutex> lock(mu);
-  for (Entry& x : pool) {
-    if (x.dev != dev || x.st != st) continue;
-    if (x.bytes < bytes) {
-      if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
-      if ((e = hipFree(x.p)) != hipSuccess) return e;
-      x.p = nullptr;
-      x.bytes = 0;
-      if ((e = hipMalloc((void**)&x.p, bytes)) != hipSuccess) return e;
-      x.bytes = bytes;
-    }
-    *out = x.p;
-    return hipSuccess;
-  }
-  double* p = nullptr;
-  if ((e = hipMalloc((void**)&p, bytes)) != hipSuccess) return e;
-  pool.push_back({dev, st, p, bytes});
-  *out = p;
-  return hipSuccess;
-}
-
 // Number of k-parts for the 128 x 128 DMA tile: enough workgroups for two per
 // CU, each part at least 256 deep; 1 = no split.
 int dgemm_splitk_parts(int64_t m, int64_t n, int64_t k) {
@@ -204,7 +166,7 @@ hipError_t dgemm_launch(const DgemmArgs& g, int tile_cfg) {
     const int64_t kper = ceil_div(ceil_div(g.k, parts), 16) * 16;
     const int64_t used = ceil_div(g.k, kper);  // parts that hold any k
     double* W = nullptr;
-    hipError_t e = splitk_workspace(g.stream, (size_t)used * g.m * g.n * 8, &W);
+    hipError_t e = stream_scratch(g.stream, (size_t)used * g.m * g.n * 8, (void**)&W);
     if (e != hipSuccess) return e;
     const int64_t tiles = ceil_div(g.m, 128) * ceil_div(g.n, 128);
     hipLaunchKernelGGL((dgemm_dma_splitk_kernel<2, 2>), dim3((unsigned)(tiles * used)), dim3(256),

@@ -5,7 +5,8 @@
 // reference's KNM sgemm leaf (KnmKernel -> sgemm_knm_asm_16x24,
 // src/kern/knm_kernel.rs:8-81, ukernels/sgemm_knm_asm_24x16.c:30-146).
 //
-// One template serves both: a slab of BK k-values is staged in LDS as
+// Large BF16 problems take the packed-panel form below (launch_bf16_panels).
+// Otherwise one template serves both: a slab of BK k-values is staged in LDS as
 // [ks][kh][row][V] where V is the elements per lane of one MFMA operand
 // (1 for f32, 8 for bf16) and kh = lane/32 picks the operand's k half, so
 // every lane reads its fragment with one contiguous load. Workgroup 256
@@ -13,6 +14,8 @@
 // 2 x 2 blocks of 32 x 32. Operands are swapped at the MFMA (B slab as the
 // "A" operand) so stores to column-major C are coalesced, as in dgemm.hip.
 #include "mfma.hpp"
+#include "panel_gemm.hpp"
+#include "scratch.hpp"
 #include "status.hpp"
 
 namespace momms {
@@ -162,6 +165,79 @@ gemm_f32acc_kernel(int64_t m, int64_t n, int64_t k, float alpha, const TIn* __re
   }
 }
 
+// BF16 in, FP32 out, large problems: both operands packed into 256-wide
+// panels (pack_bf16_panels, into library scratch), then the DMA-ring panel
+// GEMM of panel_gemm.hpp with a 256 x 256 tile and the alpha / beta epilogue.
+constexpr int kBfFM = 2, kBfFN = 4, kBfWM = 4, kBfWN = 2, kBfSPS = 2;
+typedef PanelGemm<kBfFM, kBfFN, kBfWM, kBfWN, kBfSPS> BfPanel;
+
+__global__ void __launch_bounds__(64 * kBfWM * kBfWN, 1)
+bf16_panel_gemm(int64_t m, int64_t n, int64_t tiles_m, int64_t tiles_n, int64_t nslabs,
+                const uint16_t* __restrict__ PA, const uint16_t* __restrict__ PB, float alpha,
+                float beta, float* __restrict__ C, int64_t rsC, int64_t csC) {
+  typedef BfPanel G;
+  constexpr int FM = kBfFM, FN = kBfFN, WM = kBfWM;
+  __shared__ __attribute__((aligned(16))) uint16_t As0[G::ABLK * kBfSPS];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs0[G::BBLK * kBfSPS];
+  __shared__ __attribute__((aligned(16))) uint16_t As1[G::ABLK * kBfSPS];
+  __shared__ __attribute__((aligned(16))) uint16_t Bs1[G::BBLK * kBfSPS];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave % WM, wn = wave / WM, kh = lane >> 5;
+  const int64_t t = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
+  int64_t tm, tn;
+  grouped_tile(t, tiles_m, tiles_n, 8, tm, tn);
+  v16f32 acc[FM][FN];
+#pragma unroll
+  for (int a = 0; a < FM; ++a)
+#pragma unroll
+    for (int b = 0; b < FN; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
+  panel_mainloop<FM, FN, kBfWM, kBfWN, kBfSPS, false>(acc, PA + tm * nslabs * G::ABLK,
+                                                      PB + tn * nslabs * G::BBLK, nslabs, 0.0f,
+                                                      0.0f, As0, Bs0, As1, Bs1);
+#pragma unroll
+  for (int a = 0; a < FM; ++a) {
+    const int64_t gi = tm * G::BM + wm * 32 * FM + a * 32 + (lane & 31);
+    if (gi >= m) continue;
+#pragma unroll
+    for (int b = 0; b < FN; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t gj = tn * G::BN + wn * 32 * FN + b * 32 + 8 * (r >> 2) + 4 * kh + (r & 3);
+        if (gj < n) {
+          float* cp = C + gi * rsC + gj * csC;
+          const float s = alpha * acc[a][b][r];
+          *cp = (beta == 0.0f) ? s : __builtin_fmaf(beta, *cp, s);
+        }
+      }
+  }
+}
+
+static hipError_t launch_bf16_panels(int64_t m, int64_t n, int64_t k, float alpha,
+                                     const uint16_t* A, int64_t rsA, int64_t csA,
+                                     const uint16_t* B, int64_t rsB, int64_t csB, float beta,
+                                     float* C, int64_t rsC, int64_t csC, hipStream_t st) {
+  const int64_t mpad = ceil_div(m, BfPanel::BM) * BfPanel::BM;
+  const int64_t npad = ceil_div(n, BfPanel::BN) * BfPanel::BN;
+  const int64_t kpad = ceil_div(k, BfPanel::KALIGN) * BfPanel::KALIGN;
+  uint16_t* pa = nullptr;
+  hipError_t e = stream_scratch(st, (size_t)(mpad + npad) * kpad * 2, (void**)&pa);
+  if (e != hipSuccess) return e;
+  uint16_t* pb = pa + mpad * kpad;
+  dim3 block(256);
+  hipLaunchKernelGGL((pack_bf16_panels<BfPanel::BM>),
+                     dim3((unsigned)ceil_div(mpad, 64), (unsigned)ceil_div(kpad, 64)), block, 0,
+                     st, A, m, k, rsA, csA, mpad, kpad, pa);
+  hipLaunchKernelGGL((pack_bf16_panels<BfPanel::BN>),
+                     dim3((unsigned)ceil_div(npad, 64), (unsigned)ceil_div(kpad, 64)), block, 0,
+                     st, B, n, k, csB, rsB, npad, kpad, pb);
+  const int64_t tm = mpad / BfPanel::BM, tn = npad / BfPanel::BN;
+  hipLaunchKernelGGL(bf16_panel_gemm, dim3((unsigned)(tm * tn)), dim3(64 * kBfWM * kBfWN), 0, st,
+                     m, n, tm, tn, kpad / kSlab, pa, pb, alpha, beta, C, rsC, csC);
+  return hipGetLastError();
+}
+
 static int layout_of(int64_t rs, int64_t cs, int64_t h, int64_t w) {
   if (rs == 1 || w == 1) return kCol;
   if (cs == 1 || h == 1) return kRow;
@@ -204,6 +280,12 @@ static int f32acc_entry(const char* fn, int64_t m, int64_t n, int64_t k, const f
   if ((st = check_operand(fn, 12, C, m, n, rsC, csC))) return st;
   if (m == 0 || n == 0) return 0;
   constexpr int BK = std::is_same<T, float>::value ? 16 : 32;
+  // the packed-panel form once the 256 x 256 tiles cover the device and k is
+  // deep enough to repay the packing pass
+  if (!std::is_same<T, float>::value && ceil_div(m, 256) * ceil_div(n, 256) >= 128 && k >= 256)
+    return hip_status(fn, launch_bf16_panels(m, n, k, *alpha, reinterpret_cast<const uint16_t*>(A),
+                                             rsA, csA, reinterpret_cast<const uint16_t*>(B), rsB,
+                                             csB, *beta, C, rsC, csC, (hipStream_t)stream));
   return hip_status(fn, launch_f32acc<T, BK>(m, n, k, *alpha, A, rsA, csA, B, rsB, csB, *beta, C,
                                              rsC, csC, (hipStream_t)stream));
 }

@@ -1,0 +1,161 @@
+// BF16 matrix-core GEMM over packed panels, shared by the residue split GEMM
+// (crt_bf16.hip, one GEMM per modulus, FP32 sums reduced modulo p) and the
+// BF16 -> FP32 GEMM (gemm_f32acc.hip, alpha/beta epilogue).
+//
+// Panel layout (written by the pack kernels): for a panel of PANEL rows of A
+// (columns of B) and 16-deep k-slab ks, one contiguous block [kh][PANEL][8]
+// of bf16 (kh = which 8-wide half of the slab); panels are slab-contiguous.
+//
+// Workgroup: WM x WN wavefronts, each FM x FN blocks of 32 x 32 (tile BM x BN,
+// BM = 32*FM*WM, BN = 32*FN*WN). A stage is SPS consecutive slabs; stages
+// move HBM -> LDS by global_load_lds_dwordx4 (1 KiB per wavefront load, no
+// register staging) into two LDS buffers that alternate, the next stage in
+// flight while the current one is multiplied. The two buffers are distinct
+// __shared__ objects and the loop is unrolled by two, so every LDS read names
+// a buffer no DMA in flight writes to: the compiler then needs no wait for
+// the DMA before the reads (with one ring array indexed at run time it must
+// wait for every DMA in flight, which serializes loads and multiplies).
+// The MFMA takes the B block as its "A" operand, so lane l holds row
+// i = l%32 of a block and columns 8(r/4) + 4(l/32) + r%4: column-major
+// stores along i are coalesced.
+#pragma once
+#include "mfma.hpp"
+#include "split_tile.hpp"
+
+namespace momms {
+
+template <int FM, int FN, int WM, int WN, int SPS>
+struct PanelGemm {
+  static constexpr int NW = WM * WN, BM = 32 * FM * WM, BN = 32 * FN * WN;
+  static constexpr int ABLK = BM * kSlab, BBLK = BN * kSlab;   // bf16 per slab block
+  static constexpr int LA = ABLK / 512 * SPS, LB = BBLK / 512 * SPS;  // 1 KiB loads per stage
+  static constexpr int LOADS = LA + LB;
+  static_assert(LOADS % NW == 0, "every wavefront issues the same number of loads");
+  static constexpr int LW = LOADS / NW;
+  static constexpr int KALIGN = kSlab * SPS;  // k padding of the panels
+};
+
+// The k loop: acc += A_panel * B_panel over nslabs slabs (a multiple of SPS).
+// With MOD, every 256 k-values (a chunk) each accumulator x becomes
+// x - p rint(x / p) (exact by fma for |x| < 2^24).
+#define MOMMS_PANEL_STAGE(KS, Ac, Bc, An, Bn)                                                \
+  {                                                                                          \
+    split_wait_vmcnt<0>(); /* this wavefront's part of stage KS has landed */                \
+    split_barrier();       /* everyone's part has; everyone is done with the other buffer */ \
+    if ((KS) + 1 < nst) issue((KS) + 1, An, Bn);                                             \
+    _Pragma("unroll") for (int ss = 0; ss < SPS; ++ss) {                                     \
+      v8bf16 af[FM], bf[FN];                                                                 \
+      _Pragma("unroll") for (int a = 0; a < FM; ++a) af[a] =                                 \
+          *reinterpret_cast<const v8bf16*>(&Ac[ss * G::ABLK + a_off + a * 32 * 8]);          \
+      _Pragma("unroll") for (int b = 0; b < FN; ++b) bf[b] =                                 \
+          *reinterpret_cast<const v8bf16*>(&Bc[ss * G::BBLK + b_off + b * 32 * 8]);          \
+      _Pragma("unroll") for (int a = 0; a < FM; ++a)                                         \
+          _Pragma("unroll") for (int b = 0; b < FN; ++b) acc[a][b] =                         \
+              mfma_bf16_32x32x16(bf[b], af[a], acc[a][b]);                                   \
+    }                                                                                        \
+    if (MOD && ++in_chunk == CHUNK_STAGES) {                                                 \
+      panel_reduce_mod<FM, FN>(acc, p, invp);                                                \
+      in_chunk = 0;                                                                          \
+    }                                                                                        \
+  }
+
+template <int FM, int FN>
+__device__ __forceinline__ void panel_reduce_mod(v16f32 (&acc)[FM][FN], float p, float invp) {
+#pragma unroll
+  for (int a = 0; a < FM; ++a)
+#pragma unroll
+    for (int b = 0; b < FN; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const float x = acc[a][b][r];
+        acc[a][b][r] = __builtin_fmaf(-__builtin_rintf(x * invp), p, x);
+      }
+}
+
+template <int FM, int FN, int WM, int WN, int SPS, bool MOD>
+__device__ __forceinline__ void panel_mainloop(v16f32 (&acc)[FM][FN],
+                                               const uint16_t* __restrict__ Abase,
+                                               const uint16_t* __restrict__ Bbase,
+                                               int64_t nslabs, float p, float invp,
+                                               uint16_t* As0, uint16_t* Bs0, uint16_t* As1,
+                                               uint16_t* Bs1) {
+  typedef PanelGemm<FM, FN, WM, WN, SPS> G;
+  static_assert(!MOD || (256 / kSlab) % SPS == 0, "a chunk is whole stages");
+  constexpr int CHUNK_STAGES = 256 / kSlab / SPS;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % WM, wn = wave / WM;
+  const int64_t nst = nslabs / SPS;
+
+  auto issue = [&](int64_t kst, uint16_t* Ad, uint16_t* Bd) {
+#pragma unroll
+    for (int i = 0; i < G::LW; ++i) {
+      const int c = wave + G::NW * i;
+      if (c < G::LA)
+        lds_dma16(Abase + kst * SPS * G::ABLK + c * 512 + lane * 8, &Ad[c * 512]);
+      else
+        lds_dma16(Bbase + kst * SPS * G::BBLK + (c - G::LA) * 512 + lane * 8,
+                  &Bd[(c - G::LA) * 512]);
+    }
+  };
+
+  const int kh = lane >> 5;
+  const int a_off = kh * G::BM * 8 + (wm * 32 * FM + (lane & 31)) * 8;
+  const int b_off = kh * G::BN * 8 + (wn * 32 * FN + (lane & 31)) * 8;
+
+  if (nst > 0) issue(0, As0, Bs0);
+  int in_chunk = 0;
+  (void)in_chunk;
+  (void)p;
+  (void)invp;
+  for (int64_t ks = 0; ks < nst; ks += 2) {
+    MOMMS_PANEL_STAGE(ks, As0, Bs0, As1, Bs1)
+    if (ks + 1 < nst) MOMMS_PANEL_STAGE(ks + 1, As1, Bs1, As0, Bs0)
+  }
+}
+#undef MOMMS_PANEL_STAGE
+
+// Packs count vectors of length len (vector v, element p at X[v*sv + p*sp],
+// bf16 bit patterns) into the panel layout, padding with zeros to vpad
+// vectors and lpad k-values: one 64 x 64 block of (v, p) per 256 threads,
+// through LDS so the global reads (along whichever index is unit-stride) and
+// the 16-byte panel writes are both coalesced.
+template <int PANEL>
+__global__ void __launch_bounds__(256)
+pack_bf16_panels(const uint16_t* __restrict__ X, int64_t count, int64_t len, int64_t sv,
+                 int64_t sp, int64_t vpad, int64_t lpad, uint16_t* __restrict__ Q) {
+  __shared__ uint16_t tile[64][72];
+  const int tid = threadIdx.x;
+  const int64_t v0 = (int64_t)blockIdx.x * 64, p0 = (int64_t)blockIdx.y * 64;
+  const bool v_unit = (sv == 1);
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int e = tid + 256 * r;
+    const int a = e & 63, b = e >> 6;  // a runs fastest
+    const int vv = v_unit ? a : b;
+    const int pp = v_unit ? b : a;
+    const int64_t gv = v0 + vv, gp = p0 + pp;
+    tile[vv][pp] = (gv < count && gp < len) ? X[gv * sv + gp * sp] : (uint16_t)0;
+  }
+  __syncthreads();
+  const int64_t nslabs = lpad / kSlab;
+#pragma unroll
+  for (int r = 0; r < 2; ++r) {
+    const int c = tid + 256 * r;
+    const int vv = c & 63;
+    const int pc = c >> 6;  // chunk of 8 along p
+    const int64_t gv = v0 + vv;
+    const int64_t gp = p0 + pc * 8;
+    if (gv >= vpad || gp >= lpad) continue;
+    uint4 w;
+    w.x = (uint32_t)tile[vv][pc * 8 + 0] | ((uint32_t)tile[vv][pc * 8 + 1] << 16);
+    w.y = (uint32_t)tile[vv][pc * 8 + 2] | ((uint32_t)tile[vv][pc * 8 + 3] << 16);
+    w.z = (uint32_t)tile[vv][pc * 8 + 4] | ((uint32_t)tile[vv][pc * 8 + 5] << 16);
+    w.w = (uint32_t)tile[vv][pc * 8 + 6] | ((uint32_t)tile[vv][pc * 8 + 7] << 16);
+    const int64_t pv = gv / PANEL, ks = gp / kSlab;
+    const int kh = (int)((gp % kSlab) / 8);
+    *reinterpret_cast<uint4*>(Q + (pv * nslabs + ks) * (PANEL * kSlab) + (int64_t)kh * PANEL * 8 +
+                              (gv % PANEL) * 8) = w;
+  }
+}
+
+}  // namespace momms

@@ -81,3 +81,17 @@ def test_config4_shape_sampled(fn, dtype):
     mag = (a.abs() * b.abs().T).sum(dim=1)
     err = (C.double()[rows, cols] - exact).abs()
     assert bool((err <= gamma32(n + 2) * mag).all())
+
+
+@pytest.mark.parametrize("m,n,k,layout", [(3000, 2900, 1000, "col"), (4100, 2060, 300, "row"),
+                                          (2048, 4096, 777, "col")])
+def test_bf16_packed_panel_form(m, n, k, layout):
+    """Shapes large enough for the packed-panel BF16 form (256 x 256 tiles
+    covering the device, k >= 256), ragged in every dimension."""
+    rng = np.random.default_rng(SEED + m + k)
+    A, B = mk(rng, m, k, torch.bfloat16, layout), mk(rng, k, n, torch.bfloat16, layout)
+    C0 = mk(rng, m, n, torch.float32, layout)
+    C = C0.clone()
+    capi.gemm_bf16(A, B, C, -0.75, 2.0)
+    torch.cuda.synchronize()
+    check(A, B, C0, C, -0.75, 2.0, k)
