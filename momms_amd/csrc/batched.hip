@@ -67,19 +67,34 @@ __device__ __forceinline__ int next_item(unsigned* counter) {
   return it;
 }
 
-template <int BM, int BN>
+// Plain mode, one persistent kernel over one queue: an item with prob >= 0
+// is a 64 x 64 workgroup tile (dgemm_tile_cc); an item with prob = -1 - q
+// is a quad: wavefront w of the workgroup computes 32 x 32 tile 4q + w of
+// the wave list (dgemm_wave_tile32), so small problems run four tiles per
+// workgroup with no LDS and no barrier inside a tile. Both lists are sorted
+// by k, largest first, so a quad's tiles take about equally long.
 __global__ void __launch_bounds__(256)
-batched_dgemm_kernel(const momms_gemm_desc* __restrict__ d, const BatchItem* __restrict__ items,
-                     int nitems, unsigned* counter, double alpha, const double* __restrict__ A,
-                     const double* __restrict__ B, double beta, double* __restrict__ C) {
+batched_plain_kernel(const momms_gemm_desc* __restrict__ d, const BatchItem* __restrict__ items,
+                     int nitems, const BatchItem* __restrict__ waves, int nwaves, unsigned* counter,
+                     double alpha, const double* __restrict__ A, const double* __restrict__ B,
+                     double beta, double* __restrict__ C) {
   for (;;) {
     const int it = next_item(counter);
     if (it >= nitems) return;
     const BatchItem w = items[it];
-    const momms_gemm_desc p = d[w.prob];
-    dgemm_tile<BM, BN, 16, 2, 2, kCol, kCol>(p.m, p.n, p.k, alpha, A + p.a_off, 1, p.lda,
-                                             B + p.b_off, 1, p.ldb, beta, C + p.c_off, 1, p.ldc,
-                                             (int64_t)w.tm * BM, (int64_t)w.tn * BN);
+    if (w.prob >= 0) {
+      const momms_gemm_desc p = d[w.prob];
+      dgemm_tile_cc<64, 64>(p.m, p.n, p.k, alpha, A + p.a_off, p.lda, B + p.b_off, p.ldb, beta,
+                            C + p.c_off, p.ldc, (int64_t)w.tm * 64, (int64_t)w.tn * 64);
+    } else {
+      const int wi = 4 * (-1 - w.prob) + (int)(threadIdx.x >> 6);
+      if (wi < nwaves) {
+        const BatchItem t = waves[wi];
+        const momms_gemm_desc p = d[t.prob];
+        dgemm_wave_tile32(p.m, p.n, p.k, alpha, A + p.a_off, p.lda, B + p.b_off, p.ldb, beta,
+                          C + p.c_off, p.ldc, (int64_t)t.tm * 32, (int64_t)t.tn * 32);
+      }
+    }
   }
 }
 
@@ -172,22 +187,15 @@ batched_split_kernel(const momms_gemm_desc* __restrict__ d, const SplitProb* __r
 // ---------------------------------------------------------------------------
 // host side
 
-// Tile shapes of the plain mode (BM = BN).
-constexpr int kTileCfgs = 3;
-static const int kTileEdge[kTileCfgs] = {32, 64, 128};
-constexpr int kBuckets = 5;
-static int bucket_of(const momms_gemm_desc& p) {
-  const int64_t e = std::max(p.m, p.n);
-  if (e <= 48) return 0;
-  if (e <= 96) return 1;
-  if (e <= 192) return 2;
-  if (e <= 384) return 3;
-  return 4;
-}
-static const int kDefaultTile[kBuckets] = {0, 1, 1, 2, 2};
+// Plain mode: a problem whose smaller side is at least the plan's threshold
+// T takes 64 x 64 workgroup tiles, the others 32 x 32 one-wavefront tiles,
+// all in one persistent kernel (batched_plain_kernel). The autotuner picks T
+// by timing the whole batch with each candidate.
+static const int kThresholds[] = {32, 48, 64, 96, 1 << 30};
+constexpr int kDefaultThreshold = 48;
 
-// Tuned tile per bucket, remembered for the process (keyed by bucket and a
-// coarse k class), so plans for similar batches reuse the measurement.
+// Tuned threshold, remembered for the process (keyed by a coarse class of the
+// batch: problem count and mean k), so plans for similar batches reuse it.
 static std::mutex g_tune_mu;
 static std::map<int, int> g_tuned;
 
@@ -199,7 +207,7 @@ static int occupancy(const void* kernel, int threads, int cus) {
 }
 
 struct Launchable {
-  int cfg;  // plain: tile config; split: unused
+  int cfg;  // unused
   int nitems;
   BatchItem* items;
   int grid;
@@ -214,7 +222,10 @@ struct momms_batch_plan {
   std::vector<momms_gemm_desc> descs;
   momms_gemm_desc* d_desc = nullptr;
   unsigned* d_counters = nullptr;
-  std::vector<momms::Launchable> classes;  // plain mode, one per used tile config
+  momms::Launchable plain{};                 // plain mode: the queue of tiles and quads
+  momms::BatchItem* d_waves = nullptr;       // plain mode: the 32 x 32 wave tiles
+  int nwaves = 0;
+  int threshold = 0;                         // plain mode: min(m, n) for 64 x 64 tiles
   // split mode
   std::vector<momms::SplitProb> sp;
   momms::SplitProb* d_sp = nullptr;
@@ -258,45 +269,41 @@ static std::vector<BatchItem> tile_items(const std::vector<momms_gemm_desc>& d,
   return out;
 }
 
-static hipError_t launch_plain_class(const momms_batch_plan* P, const Launchable& L, int slot,
-                                     double alpha, const double* A, const double* B, double beta,
-                                     double* C, hipStream_t st) {
-  dim3 block(256), grid((unsigned)L.grid);
-  unsigned* ctr = P->d_counters + slot;
-  switch (L.cfg) {
-    case 0:
-      hipLaunchKernelGGL((batched_dgemm_kernel<32, 32>), grid, block, 0, st, P->d_desc, L.items,
-                         L.nitems, ctr, alpha, A, B, beta, C);
-      break;
-    case 1:
-      hipLaunchKernelGGL((batched_dgemm_kernel<64, 64>), grid, block, 0, st, P->d_desc, L.items,
-                         L.nitems, ctr, alpha, A, B, beta, C);
-      break;
-    default:
-      hipLaunchKernelGGL((batched_dgemm_kernel<128, 128>), grid, block, 0, st, P->d_desc, L.items,
-                         L.nitems, ctr, alpha, A, B, beta, C);
-      break;
+// The plain-mode queue for threshold T (replacing an earlier one).
+static hipError_t build_plain(momms_batch_plan* P, int T) {
+  (void)hipFree(P->plain.items);
+  (void)hipFree(P->d_waves);
+  P->plain = Launchable{};
+  P->d_waves = nullptr;
+  P->threshold = T;
+  const auto& d = P->descs;
+  std::vector<int> big, small;
+  for (size_t i = 0; i < d.size(); ++i) {
+    if (!d[i].m || !d[i].n) continue;
+    (std::min(d[i].m, d[i].n) >= T ? big : small).push_back((int)i);
   }
-  return hipGetLastError();
-}
-
-static const void* plain_kernel_ptr(int cfg) {
-  switch (cfg) {
-    case 0: return reinterpret_cast<const void*>(&batched_dgemm_kernel<32, 32>);
-    case 1: return reinterpret_cast<const void*>(&batched_dgemm_kernel<64, 64>);
-    default: return reinterpret_cast<const void*>(&batched_dgemm_kernel<128, 128>);
-  }
-}
-
-static Launchable make_class(const std::vector<momms_gemm_desc>& d, const std::vector<int>& probs,
-                             int cfg, int cus, hipError_t& err) {
-  Launchable L{cfg, 0, nullptr, 0};
-  auto items = tile_items(d, probs, kTileEdge[cfg]);
-  L.nitems = (int)items.size();
-  err = upload((void**)&L.items, items.data(), items.size() * sizeof(BatchItem));
-  const int slots = occupancy(plain_kernel_ptr(cfg), 256, cus);
-  L.grid = std::max(1, std::min(L.nitems, slots));
-  return L;
+  const auto tiles = tile_items(d, big, 64);
+  const auto waves = tile_items(d, small, 32);
+  // one queue by k, largest first: tiles and quads of consecutive wave tiles
+  std::vector<std::pair<int64_t, BatchItem>> q;
+  for (const auto& t : tiles) q.push_back({d[t.prob].k, t});
+  for (size_t w = 0; w < waves.size(); w += 4)
+    q.push_back({d[waves[w].prob].k, BatchItem{-1 - (int32_t)(w / 4), 0, 0, 0}});
+  std::stable_sort(q.begin(), q.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+  std::vector<BatchItem> items;
+  items.reserve(q.size());
+  for (auto& x : q) items.push_back(x.second);
+  hipError_t e;
+  if ((e = upload((void**)&P->plain.items, items.data(), items.size() * sizeof(BatchItem))) != hipSuccess)
+    return e;
+  if ((e = upload((void**)&P->d_waves, waves.data(), waves.size() * sizeof(BatchItem))) != hipSuccess)
+    return e;
+  P->plain.nitems = (int)items.size();
+  P->nwaves = (int)waves.size();
+  const int slots = occupancy(reinterpret_cast<const void*>(&batched_plain_kernel), 256, P->cus);
+  P->plain.grid = std::max(1, std::min(P->plain.nitems, slots));
+  P->total_tiles = (int64_t)tiles.size() + (int64_t)waves.size();
+  return hipSuccess;
 }
 
 #define MOMMS_SPLIT_SWITCH(S_, CALL) \
@@ -313,17 +320,16 @@ static Launchable make_class(const std::vector<momms_gemm_desc>& d, const std::v
 
 static hipError_t enqueue(const momms_batch_plan* P, double alpha, const double* A,
                           const double* B, double beta, double* C, hipStream_t st) {
-  const int nctr = P->mode == 0 ? (int)P->classes.size() : 1;
+  const int nctr = 1;
   hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, st, P->d_counters, nctr);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (P->mode == 0) {
-    for (size_t c = 0; c < P->classes.size(); ++c) {
-      if (!P->classes[c].nitems) continue;
-      if ((e = launch_plain_class(P, P->classes[c], (int)c, alpha, A, B, beta, C, st)) != hipSuccess)
-        return e;
-    }
-    return hipSuccess;
+    if (!P->plain.nitems) return hipSuccess;
+    hipLaunchKernelGGL(batched_plain_kernel, dim3((unsigned)P->plain.grid), dim3(256), 0, st,
+                       P->d_desc, P->plain.items, P->plain.nitems, P->d_waves, P->nwaves,
+                       P->d_counters, alpha, A, B, beta, C);
+    return hipGetLastError();
   }
   if (P->n_exp_tasks) {
     hipLaunchKernelGGL(batched_exponents, dim3((unsigned)P->n_exp_tasks), dim3(256), 0, st,
@@ -352,7 +358,8 @@ static const void* split_kernel_ptr(int s) {
 
 static void free_plan(momms_batch_plan* P) {
   if (!P) return;
-  for (auto& c : P->classes) (void)hipFree(c.items);
+  (void)hipFree(P->plain.items);
+  (void)hipFree(P->d_waves);
   (void)hipFree(P->mul.items);
   (void)hipFree(P->d_desc);
   (void)hipFree(P->d_counters);
@@ -507,85 +514,67 @@ extern "C" int momms_hip_batch_plan_create(const momms_gemm_desc* descs, int64_t
 
   char line[256];
   if (mode == 0) {
-    std::vector<int> bucket_probs[kBuckets];
-    for (int64_t i = 0; i < count; ++i)
-      if (P->descs[i].m && P->descs[i].n) bucket_probs[bucket_of(P->descs[i])].push_back((int)i);
-    int choice[kBuckets];
-    for (int b = 0; b < kBuckets; ++b) {
-      choice[b] = kDefaultTile[b];
-      if (bucket_probs[b].empty()) continue;
-      int64_t ksum = 0;
-      for (int p : bucket_probs[b]) ksum += P->descs[p].k;
-      const int kcls = (int)std::min<int64_t>(8, (int64_t)std::log2(1.0 + (double)ksum / bucket_probs[b].size()));
-      const int key = b * 16 + kcls;
-      {
-        std::lock_guard<std::mutex> g(g_tune_mu);
-        auto it = g_tuned.find(key);
-        if (it != g_tuned.end()) {
-          choice[b] = it->second;
-          continue;
-        }
+    int64_t ksum = 0, live = 0;
+    for (auto& p : P->descs)
+      if (p.m && p.n) {
+        ksum += p.k;
+        ++live;
       }
-      if (!autotune) continue;
+    const int kcls = live ? (int)std::min<int64_t>(8, (int64_t)std::log2(1.0 + (double)ksum / live)) : 0;
+    const int ccls = (int)std::min<int64_t>(15, (int64_t)std::log2(1.0 + (double)live));
+    const int key = ccls * 16 + kcls;
+    int T = kDefaultThreshold;
+    bool known = false;
+    {
+      std::lock_guard<std::mutex> g(g_tune_mu);
+      auto it = g_tuned.find(key);
+      if (it != g_tuned.end()) {
+        T = it->second;
+        known = true;
+      }
+    }
+    if (!known && autotune && live) {
       float best = 1e30f;
-      for (int cfg = 0; cfg < kTileCfgs; ++cfg) {
-        momms_batch_plan T;
-        T.mode = 0;
-        T.cus = P->cus;
-        T.d_desc = P->d_desc;
-        T.d_counters = P->d_counters;
-        T.classes.push_back(make_class(P->descs, bucket_probs[b], cfg, P->cus, e));
-        if (e != hipSuccess) {
-          (void)hipFree(T.classes[0].items);
-          T.d_desc = nullptr;
-          T.d_counters = nullptr;
+      for (int cand : kThresholds) {
+        if ((e = build_plain(P, cand)) != hipSuccess) {
+          (void)hipFree(scratchC);
           MOMMS_FAIL(e);
         }
-        (void)time_enqueue(&T, A, B, scratchC, st, 1);  // warm up
-        const float ms = time_enqueue(&T, A, B, scratchC, st, 3);
+        (void)time_enqueue(P, A, B, scratchC, st, 1);  // warm up
+        const float ms = time_enqueue(P, A, B, scratchC, st, 3);
         if (ms < best) {
           best = ms;
-          choice[b] = cfg;
+          T = cand;
         }
-        (void)hipFree(T.classes[0].items);
-        T.classes.clear();
-        T.d_desc = nullptr;  // owned by P
-        T.d_counters = nullptr;
       }
       std::lock_guard<std::mutex> g(g_tune_mu);
-      g_tuned[key] = choice[b];
+      g_tuned[key] = T;
     }
-    // one launch class per tile config in use, largest tile first
-    for (int cfg = kTileCfgs - 1; cfg >= 0; --cfg) {
-      std::vector<int> probs;
-      for (int b = 0; b < kBuckets; ++b)
-        if (choice[b] == cfg) probs.insert(probs.end(), bucket_probs[b].begin(), bucket_probs[b].end());
-      if (probs.empty()) continue;
-      P->classes.push_back(make_class(P->descs, probs, cfg, P->cus, e));
-      if (e != hipSuccess) MOMMS_FAIL(e);
-      P->total_tiles += P->classes.back().nitems;
+    if ((e = build_plain(P, T)) != hipSuccess) {
+      (void)hipFree(scratchC);
+      MOMMS_FAIL(e);
     }
-    P->info = "{\"mode\":\"fp64\",\"buckets\":[";
-    for (int b = 0; b < kBuckets; ++b) {
-      std::snprintf(line, sizeof(line), "%s{\"bucket\":%d,\"problems\":%zu,\"tile\":%d}", b ? "," : "",
-                    b, bucket_probs[b].size(), kTileEdge[choice[b]]);
-      P->info += line;
-    }
-    P->info += "]}";
+    std::snprintf(line, sizeof(line),
+                  "{\"mode\":\"fp64\",\"threshold\":%d,\"tiles_64x64\":%lld,"
+                  "\"tiles_32x32_wavefront\":%d,\"queue\":%d,\"workgroups\":%d}",
+                  T, (long long)(P->total_tiles - P->nwaves), P->nwaves, P->plain.nitems,
+                  P->plain.grid);
+    P->info = line;
   } else {
     int s = slices;
     if (s == 0 && autotune) {
       // smallest slice count whose sampled error is within 2x plain FP64's
       if ((e = build_split(P, 8)) != hipSuccess) MOMMS_FAIL(e);
-      momms_batch_plan T;
+      momms_batch_plan T;  // the plain FP64 batch, for the reference error
       T.mode = 0;
       T.cus = P->cus;
+      T.descs = P->descs;
       T.d_desc = P->d_desc;
       T.d_counters = P->d_counters;
       std::vector<int> all;
       for (int64_t i = 0; i < count; ++i)
         if (P->descs[i].m && P->descs[i].n) all.push_back((int)i);
-      T.classes.push_back(make_class(P->descs, all, 1, P->cus, e));
+      e = build_plain(&T, kDefaultThreshold);
       double* plainC = nullptr;
       if (e == hipSuccess) e = hipMalloc((void**)&plainC, std::max<size_t>(cext, 1) * sizeof(double));
       if (e == hipSuccess) e = enqueue(&T, 1.0, A, B, 0.0, plainC, st);
@@ -616,9 +605,11 @@ extern "C" int momms_hip_batch_plan_create(const momms_gemm_desc* descs, int64_t
           break;
         }
       }
-      (void)hipFree(T.classes[0].items);
-      T.classes.clear();
-      T.d_desc = nullptr;
+      (void)hipFree(T.plain.items);
+      (void)hipFree(T.d_waves);
+      T.plain.items = nullptr;
+      T.d_waves = nullptr;
+      T.d_desc = nullptr;  // owned by P
       T.d_counters = nullptr;
       (void)hipFree(plainC);
       if (e != hipSuccess) {

@@ -158,6 +158,219 @@ __device__ __forceinline__ void dgemm_tile(int64_t m, int64_t n, int64_t k, doub
   }
 }
 
+// dgemm_tile for column-major A, B and C with lean addressing (the batched
+// kernel's case): each thread keeps base pointers to the row of A and the
+// columns of B it stages, advanced once per slab, and every element is
+// loaded through a clamped index and masked (no separate full-tile path),
+// so the tile keeps to fewer registers. BK-deep slabs (a deeper slab halves
+// the global round trips of a short tile). Same slab layout as dgemm_tile;
+// each element sums its k-values in order, four per MFMA step.
+template <int BM, int BN, int BK = 16>
+__device__ __forceinline__ void dgemm_tile_cc(int64_t m, int64_t n, int64_t k, double alpha,
+                                              const double* __restrict__ A, int64_t lda,
+                                              const double* __restrict__ B, int64_t ldb,
+                                              double beta, double* __restrict__ C, int64_t ldc,
+                                              int64_t row0, int64_t col0) {
+  constexpr int NT = 256, WTM = BM / 2, WTN = BN / 2, FM = WTM / 16, FN = WTN / 16;
+  constexpr int PAD = 16, LDA = BM + PAD, LDB = BN + PAD;
+  constexpr int A_PER = BM * BK / NT, B_PER = BK * BN / NT;
+  constexpr int AK = NT / BM;  // k rows a pass of the A slab covers
+  constexpr int BJ = NT / BK;  // columns a pass of the B slab covers
+  static_assert(NT % BM == 0 && A_PER * AK == BK && B_PER * BJ == BN, "slab split");
+  __shared__ double As[2][BK][LDA];
+  __shared__ double Bs[2][BK][LDB];
+
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wm = wave % 2, wn = wave / 2;
+  const int ai = tid % BM, ak = tid / BM;  // this thread's A row and first k
+  const int bk = tid % BK, bj = tid / BK;  // this thread's B k and first column
+  const bool arow_ok = row0 + ai < m;
+  const double* pa = A + (arow_ok ? row0 + ai + (int64_t)ak * lda : 0);
+  const double* pb[B_PER];
+  bool bcol_ok[B_PER];
+#pragma unroll
+  for (int q = 0; q < B_PER; ++q) {
+    const int64_t j = col0 + bj + BJ * q;
+    bcol_ok[q] = j < n;
+    pb[q] = B + (bcol_ok[q] ? bk + j * ldb : 0);
+  }
+  const int64_t ktiles = ceil_div(k, BK);
+  double ra[A_PER], rb[B_PER];
+  auto load_slab = [&](int64_t kt) {
+    const int64_t k0 = kt * BK;
+#pragma unroll
+    for (int q = 0; q < A_PER; ++q) {
+      const bool ok = arow_ok && k0 + ak + AK * q < k;
+      const double v = pa[ok ? (int64_t)AK * q * lda : 0];
+      ra[q] = ok ? v : 0.0;
+    }
+    const bool kok = k0 + bk < k;
+#pragma unroll
+    for (int q = 0; q < B_PER; ++q) {
+      const bool ok = bcol_ok[q] && kok;
+      const double v = pb[q][ok ? k0 : 0];
+      rb[q] = ok ? v : 0.0;
+    }
+    if (arow_ok) pa += (int64_t)BK * lda;
+  };
+  auto store_slab = [&](int buf) {
+#pragma unroll
+    for (int q = 0; q < A_PER; ++q) As[buf][ak + AK * q][ai] = ra[q];
+#pragma unroll
+    for (int q = 0; q < B_PER; ++q) Bs[buf][bk][bj + BJ * q] = rb[q];
+  };
+
+  v4f64 acc[FM][FN];
+#pragma unroll
+  for (int a = 0; a < FM; ++a)
+#pragma unroll
+    for (int b = 0; b < FN; ++b) acc[a][b] = v4f64{0.0, 0.0, 0.0, 0.0};
+  const int lr = lane & 15, lk = lane >> 4;
+  const int arow = wm * WTM + lr, bcol = wn * WTN + lr;
+
+  if (ktiles > 0) {
+    load_slab(0);
+    store_slab(0);
+  }
+  __syncthreads();
+  for (int64_t kt = 0; kt < ktiles; ++kt) {
+    const int cur = (int)(kt & 1);
+    if (kt + 1 < ktiles) load_slab(kt + 1);
+#pragma unroll
+    for (int k4 = 0; k4 < BK / 4; ++k4) {
+      double af[FM], bf[FN];
+#pragma unroll
+      for (int a = 0; a < FM; ++a) af[a] = As[cur][k4 * 4 + lk][arow + a * 16];
+#pragma unroll
+      for (int b = 0; b < FN; ++b) bf[b] = Bs[cur][k4 * 4 + lk][bcol + b * 16];
+#pragma unroll
+      for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b) acc[a][b] = mfma_f64_16x16x4(bf[b], af[a], acc[a][b]);
+    }
+    if (kt + 1 < ktiles) store_slab(cur ^ 1);
+    __syncthreads();
+  }
+#pragma unroll
+  for (int a = 0; a < FM; ++a) {
+    const int64_t gi = row0 + wm * WTM + a * 16 + lr;
+    if (gi >= m) continue;
+#pragma unroll
+    for (int b = 0; b < FN; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t gj = col0 + wn * WTN + b * 16 + lk + 4 * r;
+        if (gj < n) {
+          double* cp = C + gi + gj * ldc;
+          const double v = alpha * acc[a][b][r];
+          *cp = (beta == 0.0) ? v : __builtin_fma(beta, *cp, v);
+        }
+      }
+  }
+}
+
+// One 32 x 32 tile of C := beta*C + alpha*A*B (column-major A, B, C) computed
+// by a single wavefront, its operands loaded straight from global memory into
+// the MFMA operand registers (no LDS, no workgroup barrier), so small
+// problems run many tiles per CU at once. k goes in chunks of 16; inside a
+// chunk, lane group q = l/16 takes k0 + 4q .. k0 + 4q + 3 over the four MFMA
+// steps (the MFMA sums over its four lane groups, so any fixed assignment of
+// k to (group, step) is the same product). A lane's four B values of a chunk
+// are then four consecutive doubles of one column, and its A values one per
+// step along a row; every element's order of summation is fixed.
+// Out-of-range rows, columns and k-values load element 0 of the operand and
+// use 0 in its place.
+__device__ __forceinline__ void dgemm_wave_tile32(int64_t m, int64_t n, int64_t k, double alpha,
+                                                  const double* __restrict__ A, int64_t lda,
+                                                  const double* __restrict__ B, int64_t ldb,
+                                                  double beta, double* __restrict__ C,
+                                                  int64_t ldc, int64_t row0, int64_t col0) {
+  const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
+  // per-lane element pointers at k = 4*lq of rows row0+lr, +16 and columns
+  // col0+lr, +16, advanced by one chunk (16 k) per step; out-of-range rows /
+  // columns point at element 0 and are masked
+  const bool i0 = row0 + lr < m, i1 = row0 + 16 + lr < m;
+  const bool j0 = col0 + lr < n, j1 = col0 + 16 + lr < n;
+  const double* pa0 = A + (i0 ? row0 + lr + 4 * lq * lda : 0);
+  const double* pa1 = A + (i1 ? row0 + 16 + lr + 4 * lq * lda : 0);
+  const double* pb0 = B + (j0 ? 4 * lq + (col0 + lr) * ldb : 0);
+  const double* pb1 = B + (j1 ? 4 * lq + (col0 + 16 + lr) * ldb : 0);
+  const int la = (int)lda;  // < 2^24 (checked by the plan)
+  const int64_t astep = i0 || i1 ? 16 * lda : 0;
+  v4f64 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b) acc[a][b] = v4f64{0.0, 0.0, 0.0, 0.0};
+
+  // raw loads of one chunk; the out-of-range mask is applied when the values
+  // are used, so the loads of the next chunk stay in flight across the MFMAs
+  // of this one
+  auto load = [&](int kleft, double (&xa)[2][4], double (&xb)[2][4]) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const bool kok = s < kleft;
+      xa[0][s] = pa0[(i0 && kok) ? s * la : 0];
+      xa[1][s] = pa1[(i1 && kok) ? s * la : 0];
+      xb[0][s] = pb0[(j0 && kok) ? s : 0];
+      xb[1][s] = pb1[(j1 && kok) ? s : 0];
+    }
+    if (i0) pa0 += astep;
+    if (i1) pa1 += astep;
+    if (j0) pb0 += 16;
+    if (j1) pb1 += 16;
+  };
+
+  auto mma = [&](int kv, const double (&xa)[2][4], const double (&xb)[2][4]) {
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const bool kok = s < kv;
+      const double a0 = (i0 && kok) ? xa[0][s] : 0.0, a1 = (i1 && kok) ? xa[1][s] : 0.0;
+      const double b0 = (j0 && kok) ? xb[0][s] : 0.0, b1 = (j1 && kok) ? xb[1][s] : 0.0;
+      acc[0][0] = mfma_f64_16x16x4(b0, a0, acc[0][0]);
+      acc[0][1] = mfma_f64_16x16x4(b1, a0, acc[0][1]);
+      acc[1][0] = mfma_f64_16x16x4(b0, a1, acc[1][0]);
+      acc[1][1] = mfma_f64_16x16x4(b1, a1, acc[1][1]);
+    }
+  };
+  // two register sets in turn: the loads of one chunk are in flight while
+  // the other chunk is multiplied
+  const int64_t nchunks = ceil_div(k, 16);
+  const int64_t k0 = k - 4 * lq;  // k-values from this lane group's first
+  auto kv = [&](int64_t c) {
+    const int64_t r = k0 - 16 * c;
+    return (int)(r < 0 ? 0 : r < 4 ? r : 4);
+  };
+  double xa0[2][4], xb0[2][4], xa1[2][4], xb1[2][4];
+  if (nchunks > 0) load(kv(0), xa0, xb0);
+  // the loads are issued unconditionally (past the end they are masked reads
+  // of element 0), so the wait before each multiply covers only its own set
+#pragma unroll 1
+  for (int64_t c = 0; c < nchunks; c += 2) {
+    load(kv(c + 1), xa1, xb1);
+    mma(kv(c), xa0, xb0);
+    load(kv(c + 2), xa0, xb0);
+    if (c + 1 < nchunks) mma(kv(c + 1), xa1, xb1);
+  }
+  // lane l holds C[i][j], i = l%16, j = l/16 + 4r of each 16x16 block
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    if (!(a ? i1 : i0)) continue;
+    double* crow = C + row0 + 16 * a + lr;
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 4; ++r) {
+        const int64_t j = col0 + 16 * b + lq + 4 * r;
+        if (j < n) {
+          double* cp = crow + j * ldc;
+          const double v = alpha * acc[a][b][r];
+          *cp = (beta == 0.0) ? v : __builtin_fma(beta, *cp, v);
+        }
+      }
+  }
+}
+
 }  // namespace momms
 
 namespace momms {

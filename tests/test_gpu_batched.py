@@ -1,9 +1,11 @@
 """Batched uneven GEMMs in one persistent launch (and one hipGraph launch).
 
-Bars: plain FP64 batch == the loop of per-problem momms_hip_dgemm launches,
-bit for bit (the tile shape does not change any element's summation order),
-and within gamma_{k+1} of the double-double oracle; split-BF16 batch == the
-oracle's restatement bit for bit; graph replays == eager runs.
+Bars: plain FP64 batch within gamma_{k+1} of the double-double oracle, and so
+within 2 gamma_{k+1} of the loop of per-problem momms_hip_dgemm launches
+(the two take different summation orders: one-wavefront 32 x 32 tiles,
+128 x 128 tiles, split k); the same batch twice gives the same bits;
+split-BF16 batch == the oracle's restatement bit for bit; graph replays ==
+eager runs.
 """
 import numpy as np
 import pytest
@@ -50,7 +52,7 @@ def edge_rows():
 
 @pytest.mark.parametrize("table", ["edge", "uneven"])
 @pytest.mark.parametrize("autotune", [False, True])
-def test_plain_batch_equals_loop_and_oracle(table, autotune):
+def test_plain_batch_within_gamma_of_oracle(table, autotune):
     rows, al, bl, cl = edge_rows() if table == "edge" else capi.uneven_batch(300, 16, 300)
     A, B, C0 = arenas(rows, al, bl, cl)
     plan = capi.BatchPlan(rows, mode=0, autotune=autotune, A=A, B=B)
@@ -58,18 +60,20 @@ def test_plain_batch_equals_loop_and_oracle(table, autotune):
     assert info["mode"] == "fp64"
     C1 = C0.clone()
     plan.run(A, B, C1, 1.0, 1.0)
+    C3 = C0.clone()
+    plan.run(A, B, C3, 1.0, 1.0)
     C2 = C0.clone()
     capi.dgemm_batched_loop(rows, A, B, C2, 1.0, 1.0)
     torch.cuda.synchronize()
-    assert torch.equal(C1, C2)
-    a, b, c0, c1 = (x.cpu().numpy() for x in (A, B, C0, C1))
-    for (m, n, k, ao, bo, co, lda, ldb, ldc) in rows[:60]:
+    assert torch.equal(C1, C3)                     # deterministic
+    a, b, c0, c1, c2 = (x.cpu().numpy() for x in (A, B, C0, C1, C2))
+    for (m, n, k, ao, bo, co, lda, ldb, ldc) in rows[:80]:
         if m == 0 or n == 0:
             continue
-        Am, Bm, Cm0, Cm = (np_view(a, ao, m, k, lda), np_view(b, bo, k, n, ldb),
-                           np_view(c0, co, m, n, ldc), np_view(c1, co, m, n, ldc))
+        Am, Bm, Cm0 = np_view(a, ao, m, k, lda), np_view(b, bo, k, n, ldb), np_view(c0, co, m, n, ldc)
         ref, mag = O.dd_gemm(Am, Bm, Cm0)
-        assert (np.abs(Cm - ref) <= gamma(k + 1) * mag).all(), (m, n, k)
+        for cc in (c1, c2):
+            assert (np.abs(np_view(cc, co, m, n, ldc) - ref) <= gamma(k + 1) * mag).all(), (m, n, k)
     # padding between problems (ldc > m) is untouched
     untouched = np.ones(cl, dtype=bool)
     for (m, n, k, ao, bo, co, lda, ldb, ldc) in rows:
@@ -136,8 +140,10 @@ def test_split_slice_autotune_meets_accuracy():
     plan.close()
 
 
-def test_config5_full_batch_equals_loop():
-    """4096 problems, m, n, k log-uniform in [16, 512]."""
+def test_config5_full_batch_against_loop():
+    """4096 problems, m, n, k log-uniform in [16, 512]: the captured batch and
+    the loop of per-problem launches agree within 2 gamma_{k+1} (|A||B| + |C0|)
+    in every entry (each is within gamma_{k+1} of the exact value)."""
     rows, al, bl, cl = capi.uneven_batch(4096)
     A, B, C0 = arenas(rows, al, bl, cl)
     plan = capi.BatchPlan(rows, mode=0, autotune=True, A=A, B=B)
@@ -147,7 +153,16 @@ def test_config5_full_batch_equals_loop():
     C2 = C0.clone()
     capi.dgemm_batched_loop(rows, A, B, C2, 1.0, 1.0)
     torch.cuda.synchronize()
-    assert torch.equal(C1, C2)
+    worst = 0.0
+    for (m, n, k, ao, bo, co, lda, ldb, ldc) in rows:
+        Am = A[ao: ao + lda * k].view(k, lda)[:, :m].T
+        Bm = B[bo: bo + ldb * n].view(n, ldb)[:, :k].T
+        Cm0 = C0[co: co + ldc * n].view(n, ldc)[:, :m].T
+        mag = Am.abs().contiguous() @ Bm.abs().contiguous() + Cm0.abs()
+        d = (C1[co: co + ldc * n].view(n, ldc)[:, :m].T - C2[co: co + ldc * n].view(n, ldc)[:, :m].T).abs()
+        r = float((d / (2 * gamma(k + 1) * mag)).max())
+        worst = max(worst, r)
+    assert worst <= 1.0, worst
     plan.close()
 
 
