@@ -353,6 +353,14 @@ def main() -> int:
     del wr, Cr, C
     crt_tf = flops / crt_ms / 1e9
 
+    # HBM bytes per launch of the dominant kernel, measured with rocprofv3
+    # FETCH_SIZE / WRITE_SIZE (calibrated on a plain copy) for this exact
+    # workload; null for a workload that was not profiled
+    traffic = {}
+    tpath = ROOT / "profiles" / "roofline_traffic.json"
+    if tpath.exists():
+        traffic = json.loads(tpath.read_text())
+
     use_crt = crt_tf >= split_tf
     if use_crt:
         value, step_ms = crt_tf, crt_ms
@@ -360,7 +368,8 @@ def main() -> int:
         dtype = "f64 in/out; bf16 residues x bf16 residues -> fp32 (exact, mod p), 128-bit CRT"
         roof = {"bound": "mfma", "achieved": round(nm * flops / prod_ms / 1e9, 2),
                 "peak": PEAK_BF16, "unit": "TFLOP/s",
-                "frac": round(nm * flops / prod_ms / 1e9 / PEAK_BF16, 4), "traffic": None,
+                "frac": round(nm * flops / prod_ms / 1e9 / PEAK_BF16, 4),
+                "traffic": traffic.get(f"crt_multiply moduli={nm} m=n=k={n}", {}).get("traffic_bytes"),
                 "kernel": "crt_multiply (all moduli, one launch)", "ms_per_launch": round(prod_ms, 4),
                 "algorithmic_flops_per_launch": nm * flops}
     else:
