@@ -68,7 +68,7 @@ __device__ __forceinline__ int next_item(unsigned* counter) {
 }
 
 // Plain mode, one persistent kernel over one queue: an item with prob >= 0
-// is a 64 x 64 workgroup tile (dgemm_tile_cc); an item with prob = -1 - q
+// is a 64 x 64 workgroup tile (dgemm_tile_deep); an item with prob = -1 - q
 // is a quad: wavefront w of the workgroup computes 32 x 32 tile 4q + w of
 // the wave list (dgemm_wave_tile32), so small problems run four tiles per
 // workgroup with no LDS and no barrier inside a tile. Both lists are sorted
@@ -84,8 +84,8 @@ batched_plain_kernel(const momms_gemm_desc* __restrict__ d, const BatchItem* __r
     const BatchItem w = items[it];
     if (w.prob >= 0) {
       const momms_gemm_desc p = d[w.prob];
-      dgemm_tile_cc<64, 64>(p.m, p.n, p.k, alpha, A + p.a_off, p.lda, B + p.b_off, p.ldb, beta,
-                            C + p.c_off, p.ldc, (int64_t)w.tm * 64, (int64_t)w.tn * 64);
+      dgemm_tile_deep<64, 64, 3>(p.m, p.n, p.k, alpha, A + p.a_off, p.lda, B + p.b_off, p.ldb,
+                                 beta, C + p.c_off, p.ldc, (int64_t)w.tm * 64, (int64_t)w.tn * 64);
     } else {
       const int wi = 4 * (-1 - w.prob) + (int)(threadIdx.x >> 6);
       if (wi < nwaves) {

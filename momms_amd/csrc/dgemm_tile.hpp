@@ -158,32 +158,36 @@ __device__ __forceinline__ void dgemm_tile(int64_t m, int64_t n, int64_t k, doub
   }
 }
 
-// dgemm_tile for column-major A, B and C with lean addressing (the batched
-// kernel's case): each thread keeps base pointers to the row of A and the
-// columns of B it stages, advanced once per slab, and every element is
-// loaded through a clamped index and masked (no separate full-tile path),
-// so the tile keeps to fewer registers. BK-deep slabs (a deeper slab halves
-// the global round trips of a short tile). Same slab layout as dgemm_tile;
-// each element sums its k-values in order, four per MFMA step.
-template <int BM, int BN, int BK = 16>
-__device__ __forceinline__ void dgemm_tile_cc(int64_t m, int64_t n, int64_t k, double alpha,
-                                              const double* __restrict__ A, int64_t lda,
-                                              const double* __restrict__ B, int64_t ldb,
-                                              double beta, double* __restrict__ C, int64_t ldc,
-                                              int64_t row0, int64_t col0) {
-  constexpr int NT = 256, WTM = BM / 2, WTN = BN / 2, FM = WTM / 16, FN = WTN / 16;
+// 64 x 64-style tile of C := beta*C + alpha*A*B for column-major A, B and C
+// (the batched kernel's case), with D slabs of global loads in flight: D
+// register sets of the staged slab, the k loop unrolled by D, each set
+// reloaded D slabs ahead right after it went to LDS. Each thread keeps base
+// pointers to the row of A and the columns of B it stages. Loads are issued
+// unconditionally: an element outside the problem (row >= m, column >= n or
+// k-index >= k) is read from the problem's element (0, 0) instead, which
+// exists whenever k > 0, and is masked when stored to LDS. So the compiler's
+// wait before a set is stored covers that set only. For short tiles of
+// small problems the round trip per 16 k, not the MFMA, is the cost; D = 3
+// hides most of it. Each element sums its k-values in order, four per MFMA
+// step.
+template <int BM, int BN, int D>
+__device__ __forceinline__ void dgemm_tile_deep(int64_t m, int64_t n, int64_t k, double alpha,
+                                                const double* __restrict__ A, int64_t lda,
+                                                const double* __restrict__ B, int64_t ldb,
+                                                double beta, double* __restrict__ C, int64_t ldc,
+                                                int64_t row0, int64_t col0) {
+  constexpr int BK = 16, NT = 256, WTM = BM / 2, WTN = BN / 2, FM = WTM / 16, FN = WTN / 16;
   constexpr int PAD = 16, LDA = BM + PAD, LDB = BN + PAD;
   constexpr int A_PER = BM * BK / NT, B_PER = BK * BN / NT;
-  constexpr int AK = NT / BM;  // k rows a pass of the A slab covers
-  constexpr int BJ = NT / BK;  // columns a pass of the B slab covers
+  constexpr int AK = NT / BM, BJ = NT / BK;
   static_assert(NT % BM == 0 && A_PER * AK == BK && B_PER * BJ == BN, "slab split");
   __shared__ double As[2][BK][LDA];
   __shared__ double Bs[2][BK][LDB];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % 2, wn = wave / 2;
-  const int ai = tid % BM, ak = tid / BM;  // this thread's A row and first k
-  const int bk = tid % BK, bj = tid / BK;  // this thread's B k and first column
+  const int ai = tid % BM, ak = tid / BM;
+  const int bk = tid % BK, bj = tid / BK;
   const bool arow_ok = row0 + ai < m;
   const double* pa = A + (arow_ok ? row0 + ai + (int64_t)ak * lda : 0);
   const double* pb[B_PER];
@@ -195,29 +199,28 @@ __device__ __forceinline__ void dgemm_tile_cc(int64_t m, int64_t n, int64_t k, d
     pb[q] = B + (bcol_ok[q] ? bk + j * ldb : 0);
   }
   const int64_t ktiles = ceil_div(k, BK);
-  double ra[A_PER], rb[B_PER];
-  auto load_slab = [&](int64_t kt) {
+  double ra[D][A_PER], rb[D][B_PER];
+  // loads of slab kt (any kt >= 0; pa and pb[q] are dereferenced only at
+  // elements inside the problem, the others read A[0] / B[0])
+  auto load_slab = [&](int64_t kt, double (&xa)[A_PER], double (&xb)[B_PER]) {
     const int64_t k0 = kt * BK;
 #pragma unroll
     for (int q = 0; q < A_PER; ++q) {
       const bool ok = arow_ok && k0 + ak + AK * q < k;
-      const double v = pa[ok ? (int64_t)AK * q * lda : 0];
-      ra[q] = ok ? v : 0.0;
+      xa[q] = *(ok ? pa + (k0 * lda + (int64_t)AK * q * lda) : A);
     }
     const bool kok = k0 + bk < k;
 #pragma unroll
-    for (int q = 0; q < B_PER; ++q) {
-      const bool ok = bcol_ok[q] && kok;
-      const double v = pb[q][ok ? k0 : 0];
-      rb[q] = ok ? v : 0.0;
-    }
-    if (arow_ok) pa += (int64_t)BK * lda;
+    for (int q = 0; q < B_PER; ++q) xb[q] = *((bcol_ok[q] && kok) ? pb[q] + k0 : B);
   };
-  auto store_slab = [&](int buf) {
+  auto store_slab = [&](int64_t kt, int buf, const double (&xa)[A_PER], const double (&xb)[B_PER]) {
+    const int64_t k0 = kt * BK;
 #pragma unroll
-    for (int q = 0; q < A_PER; ++q) As[buf][ak + AK * q][ai] = ra[q];
+    for (int q = 0; q < A_PER; ++q)
+      As[buf][ak + AK * q][ai] = (arow_ok && k0 + ak + AK * q < k) ? xa[q] : 0.0;
+    const bool kok = k0 + bk < k;
 #pragma unroll
-    for (int q = 0; q < B_PER; ++q) Bs[buf][bk][bj + BJ * q] = rb[q];
+    for (int q = 0; q < B_PER; ++q) Bs[buf][bk][bj + BJ * q] = (bcol_ok[q] && kok) ? xb[q] : 0.0;
   };
 
   v4f64 acc[FM][FN];
@@ -228,29 +231,36 @@ __device__ __forceinline__ void dgemm_tile_cc(int64_t m, int64_t n, int64_t k, d
   const int lr = lane & 15, lk = lane >> 4;
   const int arow = wm * WTM + lr, bcol = wn * WTN + lr;
 
-  if (ktiles > 0) {
-    load_slab(0);
-    store_slab(0);
-  }
-  __syncthreads();
-  for (int64_t kt = 0; kt < ktiles; ++kt) {
-    const int cur = (int)(kt & 1);
-    if (kt + 1 < ktiles) load_slab(kt + 1);
+  if (ktiles > 0) {  // (k = 0: no loads; A and B may hold no element)
 #pragma unroll
-    for (int k4 = 0; k4 < BK / 4; ++k4) {
-      double af[FM], bf[FN];
+  for (int d = 0; d < D; ++d) load_slab(d, ra[d], rb[d]);
+#pragma unroll 1
+  for (int64_t kt0 = 0; kt0 < ktiles; kt0 += D) {
 #pragma unroll
-      for (int a = 0; a < FM; ++a) af[a] = As[cur][k4 * 4 + lk][arow + a * 16];
+    for (int d = 0; d < D; ++d) {
+      const int64_t kt = kt0 + d;
+      if (kt < ktiles) {
+        const int buf = (int)(kt & 1);
+        store_slab(kt, buf, ra[d], rb[d]);
+        __syncthreads();  // slab kt visible; everyone is done with slab kt-1
+        load_slab(kt + D, ra[d], rb[d]);
 #pragma unroll
-      for (int b = 0; b < FN; ++b) bf[b] = Bs[cur][k4 * 4 + lk][bcol + b * 16];
+        for (int k4 = 0; k4 < BK / 4; ++k4) {
+          double af[FM], bf[FN];
 #pragma unroll
-      for (int a = 0; a < FM; ++a)
+          for (int a = 0; a < FM; ++a) af[a] = As[buf][k4 * 4 + lk][arow + a * 16];
 #pragma unroll
-        for (int b = 0; b < FN; ++b) acc[a][b] = mfma_f64_16x16x4(bf[b], af[a], acc[a][b]);
+          for (int b = 0; b < FN; ++b) bf[b] = Bs[buf][k4 * 4 + lk][bcol + b * 16];
+#pragma unroll
+          for (int a = 0; a < FM; ++a)
+#pragma unroll
+            for (int b = 0; b < FN; ++b) acc[a][b] = mfma_f64_16x16x4(bf[b], af[a], acc[a][b]);
+        }
+      }
     }
-    if (kt + 1 < ktiles) store_slab(cur ^ 1);
-    __syncthreads();
   }
+  }
+  __syncthreads();  // LDS free for the next tile of a persistent caller
 #pragma unroll
   for (int a = 0; a < FM; ++a) {
     const int64_t gi = row0 + wm * WTM + a * 16 + lr;
@@ -287,8 +297,10 @@ __device__ __forceinline__ void dgemm_wave_tile32(int64_t m, int64_t n, int64_t 
                                                   int64_t ldc, int64_t row0, int64_t col0) {
   const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
   // per-lane element pointers at k = 4*lq of rows row0+lr, +16 and columns
-  // col0+lr, +16, advanced by one chunk (16 k) per step; out-of-range rows /
-  // columns point at element 0 and are masked
+  // col0+lr, +16, advanced by one chunk (16 k) per step. They run past the
+  // problem at its last chunks, so they are dereferenced only at elements
+  // inside it: every other element is read from A[0] / B[0] (present, as a
+  // tile with k > 0 loads) and masked when used.
   const bool i0 = row0 + lr < m, i1 = row0 + 16 + lr < m;
   const bool j0 = col0 + lr < n, j1 = col0 + 16 + lr < n;
   const double* pa0 = A + (i0 ? row0 + lr + 4 * lq * lda : 0);
@@ -310,10 +322,10 @@ __device__ __forceinline__ void dgemm_wave_tile32(int64_t m, int64_t n, int64_t 
 #pragma unroll
     for (int s = 0; s < 4; ++s) {
       const bool kok = s < kleft;
-      xa[0][s] = pa0[(i0 && kok) ? s * la : 0];
-      xa[1][s] = pa1[(i1 && kok) ? s * la : 0];
-      xb[0][s] = pb0[(j0 && kok) ? s : 0];
-      xb[1][s] = pb1[(j1 && kok) ? s : 0];
+      xa[0][s] = *((i0 && kok) ? pa0 + s * la : A);
+      xa[1][s] = *((i1 && kok) ? pa1 + s * la : A);
+      xb[0][s] = *((j0 && kok) ? pb0 + s : B);
+      xb[1][s] = *((j1 && kok) ? pb1 + s : B);
     }
     if (i0) pa0 += astep;
     if (i1) pa1 += astep;
@@ -344,7 +356,7 @@ __device__ __forceinline__ void dgemm_wave_tile32(int64_t m, int64_t n, int64_t 
   double xa0[2][4], xb0[2][4], xa1[2][4], xb1[2][4];
   if (nchunks > 0) load(kv(0), xa0, xb0);
   // the loads are issued unconditionally (past the end they are masked reads
-  // of element 0), so the wait before each multiply covers only its own set
+  // of A[0] / B[0]), so the wait before each multiply covers only its own set
 #pragma unroll 1
   for (int64_t c = 0; c < nchunks; c += 2) {
     load(kv(c + 1), xa1, xb1);

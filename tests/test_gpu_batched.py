@@ -83,6 +83,52 @@ def test_plain_batch_within_gamma_of_oracle(table, autotune):
     plan.close()
 
 
+@pytest.mark.parametrize("mode", [0, 1])
+def test_batch_reads_only_its_problems(mode):
+    """Arenas whose padding (lda > m, ldb > k, gaps) is NaN and whose last
+    problem ends at the last element: shapes where tiles reach past k (k < 16,
+    k not a multiple of 16, one-wavefront tiles at k = 512), past m and n, and
+    k = 0. A masked-out element that leaked into a sum would make it NaN."""
+    shapes = [(100, 120, 5), (30, 40, 37), (200, 96, 3), (32, 32, 512), (17, 300, 0), (96, 100, 1),
+              (5, 7, 9), (32, 32, 512)]
+    rows, ao, bo, co = [], 0, 0, 0
+    for i, (m, n, k) in enumerate(shapes):
+        last = i == len(shapes) - 1
+        lda, ldb = m + (0 if last else 3), max(k + (0 if last else 2), 1)
+        rows.append((m, n, k, ao + 1, bo + 1, co, lda, ldb, m))  # one NaN before each
+        ao += 1 + lda * k
+        bo += 1 + ldb * n
+        co += m * n
+    al, bl = ao, bo
+    g = torch.Generator(device="cuda").manual_seed(SEED + 31 + mode)
+    A = torch.full((al,), float("nan"), device="cuda", dtype=torch.float64)
+    B = torch.full((bl,), float("nan"), device="cuda", dtype=torch.float64)
+    C0 = torch.rand(co, generator=g, device="cuda", dtype=torch.float64) * 2 - 1
+    for (m, n, k, a0, b0, c0_, lda, ldb, ldc) in rows:
+        if k:
+            view(A, a0, m, k, lda).copy_(torch.rand(m, k, generator=g, device="cuda", dtype=torch.float64) - 0.5)
+        if k and n:
+            view(B, b0, k, n, ldb).copy_(torch.rand(k, n, generator=g, device="cuda", dtype=torch.float64) - 0.5)
+    assert rows[-1][3] + rows[-1][6] * rows[-1][2] == al and rows[-1][4] + rows[-1][7] * rows[-1][1] == bl
+    plan = capi.BatchPlan(rows, mode=mode, slices=7 if mode else 0, autotune=mode == 0, A=A, B=B)
+    C = C0.clone()
+    plan.run(A, B, C, 1.0, 1.0)
+    torch.cuda.synchronize()
+    a, b, c0, c = (x.cpu().numpy() for x in (A, B, C0, C))
+    assert np.isfinite(c).all()
+    for (m, n, k, a0, b0, c0_, lda, ldb, ldc) in rows:
+        Am, Bm, Cm0 = np_view(a, a0, m, k, lda), np_view(b, b0, k, n, ldb), np_view(c0, c0_, m, n, ldc)
+        got = np_view(c, c0_, m, n, ldc)
+        if mode == 0:
+            ref, mag = O.dd_gemm(Am, Bm, Cm0)
+            assert (np.abs(got - ref) <= gamma(k + 1) * mag).all(), (m, n, k)
+        else:
+            want = Cm0.copy(order="F")
+            O.ozaki_gemm(Am, Bm, want, 7, 1.0, 1.0)
+            assert np.array_equal(got.view(np.int64), want.view(np.int64)), (m, n, k)
+    plan.close()
+
+
 @pytest.mark.parametrize("s", [3, 7])
 def test_split_batch_bitwise_against_restatement(s):
     rows, al, bl, cl = edge_rows()
