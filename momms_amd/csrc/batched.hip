@@ -25,6 +25,8 @@
 #include <string>
 #include <vector>
 
+#include "batch_tiles.hpp"
+#include "batched.hpp"
 #include "dgemm.hpp"
 #include "dgemm_tile.hpp"
 #include "mfma.hpp"
@@ -35,9 +37,6 @@
 
 namespace momms {
 
-struct BatchItem {
-  int32_t prob, tm, tn, pad;
-};
 
 // Per-problem workspace of the split mode (byte offsets into the arena).
 struct SplitProb {
@@ -57,44 +56,60 @@ __global__ void reset_counters(unsigned* c, int n) {
   if ((int)threadIdx.x < n) c[threadIdx.x] = 0u;
 }
 
-// Pulls the next item index for the whole workgroup.
-__device__ __forceinline__ int next_item(unsigned* counter) {
-  __shared__ int s_item;
-  if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
-  __syncthreads();
-  const int it = s_item;
-  __syncthreads();
-  return it;
-}
 
-// Plain mode, one persistent kernel over one queue: an item with prob >= 0
-// is a 64 x 64 workgroup tile (dgemm_tile_deep); an item with prob = -1 - q
-// is a quad: wavefront w of the workgroup computes 32 x 32 tile 4q + w of
-// the wave list (dgemm_wave_tile32), so small problems run four tiles per
-// workgroup with no LDS and no barrier inside a tile. Both lists are sorted
-// by k, largest first, so a quad's tiles take about equally long.
-__global__ void __launch_bounds__(256)
+// Plain mode, one persistent kernel over kBatchQueues queues, one per XCD.
+// An item with prob >= 0 is a 64 x 64 workgroup tile (batch_tile64); an item
+// with prob = -1 - q is a quad: wavefront w of the workgroup computes 32 x 32
+// tile 4q + w of the wave list (batch_wave_tile32; entries with prob < 0 pad
+// a queue's wave list to whole quads), so small problems run four tiles per
+// workgroup with no LDS and no barrier inside a tile. Every problem sits in
+// one queue, whole, and workgroup b serves queue b % 8 first (workgroups are
+// dispatched to the 8 XCDs in turn), so the A and B panels a problem's
+// tiles share are fetched into one XCD's L2. A workgroup whose queue is
+// empty takes items from the next queues. Each queue is sorted by k, largest
+// first, so its tail is short and a quad's tiles take about equally long.
+// Only problems with lda, ldb, ldc <= kBatchLeanLd come here; the others go
+// to batched_wide_kernel.
+constexpr int kBatchQueues = 8;
+struct QueueBounds {
+  int off[kBatchQueues + 1];  // queue q holds items [off[q], off[q+1])
+};
+
+// Three workgroups per CU (at most 168 registers a lane, 39 KB of LDS
+// each): more resident waves hide the latency of short tiles; a fourth would
+// cost the tiles their deeper prefetch or spill.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
 batched_plain_kernel(const momms_gemm_desc* __restrict__ d, const BatchItem* __restrict__ items,
-                     int nitems, const BatchItem* __restrict__ waves, int nwaves, unsigned* counter,
-                     double alpha, const double* __restrict__ A, const double* __restrict__ B,
-                     double beta, double* __restrict__ C) {
-  for (;;) {
-    const int it = next_item(counter);
-    if (it >= nitems) return;
-    const BatchItem w = items[it];
-    if (w.prob >= 0) {
-      const momms_gemm_desc p = d[w.prob];
-      dgemm_tile_deep<64, 64, 3>(p.m, p.n, p.k, alpha, A + p.a_off, p.lda, B + p.b_off, p.ldb,
-                                 beta, C + p.c_off, p.ldc, (int64_t)w.tm * 64, (int64_t)w.tn * 64);
-    } else {
-      const int wi = 4 * (-1 - w.prob) + (int)(threadIdx.x >> 6);
-      if (wi < nwaves) {
-        const BatchItem t = waves[wi];
-        const momms_gemm_desc p = d[t.prob];
-        dgemm_wave_tile32(p.m, p.n, p.k, alpha, A + p.a_off, p.lda, B + p.b_off, p.ldb, beta,
-                          C + p.c_off, p.ldc, (int64_t)t.tm * 32, (int64_t)t.tn * 32);
+                     QueueBounds qb, const BatchItem* __restrict__ waves, int nwaves,
+                     unsigned* counters, double alpha, const double* __restrict__ A,
+                     const double* __restrict__ B, double beta, double* __restrict__ C) {
+  const int home = (int)(blockIdx.x % kBatchQueues);
+  for (int s = 0; s < kBatchQueues; ++s) {
+    const int q = (home + s) % kBatchQueues;
+    const int base = qb.off[q], count = qb.off[q + 1] - base;
+    run_queue(counters + q, count, [&](int it) {
+      const BatchItem w = items[base + uniform(it)];
+      const int prob = uniform(w.prob);
+      if (prob >= 0) {
+        const momms_gemm_desc p = d[prob];
+        batch_tile64<3>(p.m, p.n, p.k, alpha, A + p.a_off, p.lda, B + p.b_off, p.ldb, beta,
+                        C + p.c_off, p.ldc, (int64_t)uniform(w.tm) * 64,
+                        (int64_t)uniform(w.tn) * 64);
+      } else {
+        // the same for all lanes of a wavefront
+        const int wi = uniform(4 * (-1 - prob) + (int)(threadIdx.x >> 6));
+        if (wi < nwaves) {
+          const BatchItem t = waves[wi];
+          const int tp = uniform(t.prob);
+          if (tp >= 0) {
+            const momms_gemm_desc p = d[tp];
+            batch_wave_tile32<4>(p.m, p.n, p.k, alpha, A + p.a_off, p.lda, B + p.b_off, p.ldb,
+                                 beta, C + p.c_off, p.ldc, (int64_t)uniform(t.tm) * 32,
+                                 (int64_t)uniform(t.tn) * 32);
+          }
+        }
       }
-    }
+    });
   }
 }
 
@@ -169,9 +184,7 @@ batched_split_kernel(const momms_gemm_desc* __restrict__ d, const SplitProb* __r
                      const BatchItem* __restrict__ items, int nitems, unsigned* counter,
                      double alpha, double beta, double* __restrict__ C,
                      const char* __restrict__ ws, int slabs_per_chunk) {
-  for (;;) {
-    const int it = next_item(counter);
-    if (it >= nitems) return;
+  run_queue(counter, nitems, [&](int it) {
     const BatchItem w = items[it];
     const momms_gemm_desc p = d[w.prob];
     const SplitProb s = sp[w.prob];
@@ -181,7 +194,7 @@ batched_split_kernel(const momms_gemm_desc* __restrict__ d, const SplitProb* __r
                         reinterpret_cast<const int*>(ws + s.ea),
                         reinterpret_cast<const int*>(ws + s.eb), s.mpad / kBatchPanel,
                         s.npad / kBatchPanel, slabs_per_chunk, w.tm, w.tn);
-  }
+  });
 }
 
 // ---------------------------------------------------------------------------
@@ -223,8 +236,11 @@ struct momms_batch_plan {
   momms_gemm_desc* d_desc = nullptr;
   unsigned* d_counters = nullptr;
   momms::Launchable plain{};                 // plain mode: the queue of tiles and quads
+  momms::Launchable wide{};                  // plain mode: tiles of problems with a large ld
   momms::BatchItem* d_waves = nullptr;       // plain mode: the 32 x 32 wave tiles
-  int nwaves = 0;
+  int nwaves = 0;                            // plain mode: wave list length (whole quads)
+  int64_t nwave_tiles = 0;                   // plain mode: 32 x 32 tiles in it
+  momms::QueueBounds qb{};                   // plain mode: the per-XCD queues in plain.items
   int threshold = 0;                         // plain mode: min(m, n) for 64 x 64 tiles
   // split mode
   std::vector<momms::SplitProb> sp;
@@ -272,27 +288,61 @@ static std::vector<BatchItem> tile_items(const std::vector<momms_gemm_desc>& d,
 // The plain-mode queue for threshold T (replacing an earlier one).
 static hipError_t build_plain(momms_batch_plan* P, int T) {
   (void)hipFree(P->plain.items);
+  (void)hipFree(P->wide.items);
   (void)hipFree(P->d_waves);
   P->plain = Launchable{};
+  P->wide = Launchable{};
   P->d_waves = nullptr;
   P->threshold = T;
   const auto& d = P->descs;
-  std::vector<int> big, small;
+  std::vector<int> big, small, wide;
   for (size_t i = 0; i < d.size(); ++i) {
     if (!d[i].m || !d[i].n) continue;
-    (std::min(d[i].m, d[i].n) >= T ? big : small).push_back((int)i);
+    if (std::max(d[i].lda, std::max(d[i].ldb, d[i].ldc)) > kBatchLeanLd)
+      wide.push_back((int)i);
+    else
+      (std::min(d[i].m, d[i].n) >= T ? big : small).push_back((int)i);
   }
-  const auto tiles = tile_items(d, big, 64);
-  const auto waves = tile_items(d, small, 32);
-  // one queue by k, largest first: tiles and quads of consecutive wave tiles
-  std::vector<std::pair<int64_t, BatchItem>> q;
-  for (const auto& t : tiles) q.push_back({d[t.prob].k, t});
-  for (size_t w = 0; w < waves.size(); w += 4)
-    q.push_back({d[waves[w].prob].k, BatchItem{-1 - (int32_t)(w / 4), 0, 0, 0}});
-  std::stable_sort(q.begin(), q.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
-  std::vector<BatchItem> items;
-  items.reserve(q.size());
-  for (auto& x : q) items.push_back(x.second);
+  // problems to queues, largest work first onto the least loaded queue;
+  // a 64 x 64 tile counts as four 32 x 32 ones, each with a fixed cost of
+  // about 32 k-steps (prologue and C update)
+  std::vector<std::pair<double, int>> work;
+  for (int p : big)
+    work.push_back({4.0 * (double)(ceil_div(d[p].m, 64) * ceil_div(d[p].n, 64)) * (double)(d[p].k + 32), p});
+  for (int p : small)
+    work.push_back({(double)(ceil_div(d[p].m, 32) * ceil_div(d[p].n, 32)) * (double)(d[p].k + 32), p});
+  std::stable_sort(work.begin(), work.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+  std::vector<int> qbig[kBatchQueues], qsmall[kBatchQueues];
+  double load[kBatchQueues] = {};
+  std::vector<char> is_big(d.size(), 0);
+  for (int p : big) is_big[p] = 1;
+  for (const auto& w : work) {
+    const int q = (int)(std::min_element(load, load + kBatchQueues) - load);
+    load[q] += w.first;
+    (is_big[w.second] ? qbig[q] : qsmall[q]).push_back(w.second);
+  }
+  // per queue: its 64 x 64 tiles and quads of its wave tiles, by k
+  std::vector<BatchItem> items, waves;
+  int64_t ntiles = 0, nwave_tiles = 0;
+  for (int qi = 0; qi < kBatchQueues; ++qi) {
+    std::sort(qbig[qi].begin(), qbig[qi].end());
+    std::sort(qsmall[qi].begin(), qsmall[qi].end());
+    const auto tiles = tile_items(d, qbig[qi], 64);
+    auto wq = tile_items(d, qsmall[qi], 32);
+    ntiles += (int64_t)tiles.size();
+    nwave_tiles += (int64_t)wq.size();
+    std::vector<std::pair<int64_t, BatchItem>> q;
+    for (const auto& t : tiles) q.push_back({d[t.prob].k, t});
+    const int64_t w0 = (int64_t)waves.size();
+    for (size_t w = 0; w < wq.size(); w += 4)
+      q.push_back({d[wq[w].prob].k, BatchItem{-1 - (int32_t)((w0 + (int64_t)w) / 4), 0, 0, 0}});
+    while (wq.size() % 4) wq.push_back(BatchItem{-1, 0, 0, 0});
+    waves.insert(waves.end(), wq.begin(), wq.end());
+    std::stable_sort(q.begin(), q.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
+    P->qb.off[qi] = (int)items.size();
+    for (auto& x : q) items.push_back(x.second);
+  }
+  P->qb.off[kBatchQueues] = (int)items.size();
   hipError_t e;
   if ((e = upload((void**)&P->plain.items, items.data(), items.size() * sizeof(BatchItem))) != hipSuccess)
     return e;
@@ -302,7 +352,17 @@ static hipError_t build_plain(momms_batch_plan* P, int T) {
   P->nwaves = (int)waves.size();
   const int slots = occupancy(reinterpret_cast<const void*>(&batched_plain_kernel), 256, P->cus);
   P->plain.grid = std::max(1, std::min(P->plain.nitems, slots));
-  P->total_tiles = (int64_t)tiles.size() + (int64_t)waves.size();
+  const auto wide_tiles = tile_items(d, wide, 64);
+  if (!wide_tiles.empty()) {
+    if ((e = upload((void**)&P->wide.items, wide_tiles.data(), wide_tiles.size() * sizeof(BatchItem))) !=
+        hipSuccess)
+      return e;
+    P->wide.nitems = (int)wide_tiles.size();
+    const int wslots = occupancy(batched_wide_kernel_ptr(), 256, P->cus);
+    P->wide.grid = std::max(1, std::min(P->wide.nitems, wslots));
+  }
+  P->total_tiles = ntiles + nwave_tiles + (int64_t)wide_tiles.size();
+  P->nwave_tiles = nwave_tiles;
   return hipSuccess;
 }
 
@@ -320,15 +380,19 @@ static hipError_t build_plain(momms_batch_plan* P, int T) {
 
 static hipError_t enqueue(const momms_batch_plan* P, double alpha, const double* A,
                           const double* B, double beta, double* C, hipStream_t st) {
-  const int nctr = 1;
+  // plain mode: counters 0..7 the queues, 8 the wide problems
+  const int nctr = P->mode == 0 ? kBatchQueues + 1 : 1;
   hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, st, P->d_counters, nctr);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (P->mode == 0) {
-    if (!P->plain.nitems) return hipSuccess;
-    hipLaunchKernelGGL(batched_plain_kernel, dim3((unsigned)P->plain.grid), dim3(256), 0, st,
-                       P->d_desc, P->plain.items, P->plain.nitems, P->d_waves, P->nwaves,
-                       P->d_counters, alpha, A, B, beta, C);
+    if (P->plain.nitems)
+      hipLaunchKernelGGL(batched_plain_kernel, dim3((unsigned)P->plain.grid), dim3(256), 0, st,
+                         P->d_desc, P->plain.items, P->qb, P->d_waves, P->nwaves,
+                         P->d_counters, alpha, A, B, beta, C);
+    if (P->wide.nitems)
+      launch_batched_wide(P->wide.grid, st, P->d_desc, P->wide.items, P->wide.nitems,
+                          P->d_counters + kBatchQueues, alpha, A, B, beta, C);
     return hipGetLastError();
   }
   if (P->n_exp_tasks) {
@@ -359,6 +423,7 @@ static const void* split_kernel_ptr(int s) {
 static void free_plan(momms_batch_plan* P) {
   if (!P) return;
   (void)hipFree(P->plain.items);
+  (void)hipFree(P->wide.items);
   (void)hipFree(P->d_waves);
   (void)hipFree(P->mul.items);
   (void)hipFree(P->d_desc);
@@ -556,9 +621,9 @@ extern "C" int momms_hip_batch_plan_create(const momms_gemm_desc* descs, int64_t
     }
     std::snprintf(line, sizeof(line),
                   "{\"mode\":\"fp64\",\"threshold\":%d,\"tiles_64x64\":%lld,"
-                  "\"tiles_32x32_wavefront\":%d,\"queue\":%d,\"workgroups\":%d}",
-                  T, (long long)(P->total_tiles - P->nwaves), P->nwaves, P->plain.nitems,
-                  P->plain.grid);
+                  "\"tiles_32x32_wavefront\":%lld,\"queue\":%d,\"queues\":%d,\"workgroups\":%d}",
+                  T, (long long)(P->total_tiles - P->nwave_tiles), (long long)P->nwave_tiles,
+                  P->plain.nitems, kBatchQueues, P->plain.grid);
     P->info = line;
   } else {
     int s = slices;
@@ -606,8 +671,10 @@ extern "C" int momms_hip_batch_plan_create(const momms_gemm_desc* descs, int64_t
         }
       }
       (void)hipFree(T.plain.items);
+      (void)hipFree(T.wide.items);
       (void)hipFree(T.d_waves);
       T.plain.items = nullptr;
+      T.wide.items = nullptr;
       T.d_waves = nullptr;
       T.d_desc = nullptr;  // owned by P
       T.d_counters = nullptr;

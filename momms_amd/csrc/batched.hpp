@@ -1,0 +1,45 @@
+// Pieces of the batched engine shared by its translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "momms_hip.h"
+
+namespace momms {
+
+struct BatchItem {
+  int32_t prob, tm, tn, pad;
+};
+
+// A persistent workgroup's loop over one queue of `count` items: body(it)
+// for every item index it claims from `counter`. The claim of the next item
+// (a device-scope atomic, a long round trip) is issued when an item starts
+// and read when it ends, so it overlaps the item's work.
+template <typename Body>
+__device__ __forceinline__ void run_queue(unsigned* counter, int count, Body&& body) {
+  __shared__ int s_item;
+  if (threadIdx.x == 0) s_item = (int)atomicAdd(counter, 1u);
+  __syncthreads();
+  int it = s_item;
+  __syncthreads();
+  while (it < count) {
+    unsigned next = 0;
+    if (threadIdx.x == 0) next = atomicAdd(counter, 1u);
+    body(it);
+    if (threadIdx.x == 0) s_item = (int)next;
+    __syncthreads();
+    it = s_item;
+    __syncthreads();
+  }
+}
+
+// Problems of a batch with a leading dimension too large for the 32-bit
+// offsets of the lean tiles: 64 x 64 tiles with 64-bit addressing, one
+// queue (batched_wide.hip; its own translation unit, so that its register
+// allocation does not weigh on the lean kernel's).
+const void* batched_wide_kernel_ptr();
+void launch_batched_wide(int grid, hipStream_t st, const momms_gemm_desc* d, const BatchItem* items,
+                         int nitems, unsigned* counter, double alpha, const double* A,
+                         const double* B, double beta, double* C);
+
+}  // namespace momms

@@ -38,6 +38,7 @@
 #include <math.h>
 
 #include "crt_bf16.hpp"
+#include "c_update.hpp"
 #include "crt_recon.hpp"
 #include "panel_gemm.hpp"
 #include "mfma.hpp"
@@ -286,22 +287,26 @@ crt_reconstruct(int64_t m, int64_t n, int64_t k, int beta_bits, double alpha, do
       }
     }
     const int ej = EB[j];
+    int ea[kCrtRows];  // EA holds mpad entries: every row of the thread indexes it
+#pragma unroll
+    for (int e = 0; e < kCrtRows; ++e) ea[e] = EA[i0 + e];
+    int64_t idx[kCrtRows];
+    bool ok[kCrtRows];
+    double v[kCrtRows];
 #pragma unroll
     for (int e = 0; e < kCrtRows; ++e) {
       const int64_t i = i0 + e;
-      if (i >= m) break;
-      const int ei = EA[i];
-      double v;
-      if (ei == kNonFinite || ej == kNonFinite) {
+      ok[e] = i < m;
+      idx[e] = i * rsC + j * csC;
+      if (ok[e] && (ea[e] == kNonFinite || ej == kNonFinite)) {
         double s = 0.0;
         for (int64_t p = 0; p < k; ++p) s = __builtin_fma(A[i * rsA + p * csA], B[p * rsB + j * csB], s);
-        v = alpha * s;
+        v[e] = alpha * s;
       } else {
-        v = alpha * ldexp(crt_i128_to_f64(crt_sums_value(a[e], K)), ei + ej - 2 * beta_bits);
+        v[e] = alpha * ldexp(crt_i128_to_f64(crt_sums_value(a[e], K)), ea[e] + ej - 2 * beta_bits);
       }
-      double* cp = C + i * rsC + j * csC;
-      *cp = (beta == 0.0) ? v : __builtin_fma(beta, *cp, v);
     }
+    update_c(C, idx, ok, v, beta);
   }
 }
 

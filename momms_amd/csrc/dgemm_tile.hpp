@@ -2,6 +2,7 @@
 // used by the one-problem kernel (dgemm.hip) and the persistent batched
 // kernel (batched.hip). See dgemm.hip for how it maps the Goto tree.
 #pragma once
+#include "c_update.hpp"
 #include "mfma.hpp"
 
 namespace momms {
@@ -142,19 +143,21 @@ __device__ __forceinline__ void dgemm_tile(int64_t m, int64_t n, int64_t k, doub
 #pragma unroll
   for (int a = 0; a < FM; ++a) {
     const int64_t gi = row0 + wm * WTM + a * 16 + lr;
+    int64_t idx[FN * 4];
+    bool ok[FN * 4];
+    double v[FN * 4];
 #pragma unroll
     for (int b = 0; b < FN; ++b) {
       const int64_t gj0 = col0 + wn * WTN + b * 16 + lk;
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int64_t gj = gj0 + 4 * r;
-        if (gi < m && gj < n) {
-          double* cp = C + gi * rsC + gj * csC;
-          const double s = alpha * acc[a][b][r];
-          *cp = (beta == 0.0) ? s : __builtin_fma(beta, *cp, s);
-        }
+        idx[b * 4 + r] = gi * rsC + gj * csC;
+        ok[b * 4 + r] = gi < m && gj < n;
+        v[b * 4 + r] = alpha * acc[a][b][r];
       }
     }
+    update_c(C, idx, ok, v, beta);
   }
 }
 
@@ -177,12 +180,16 @@ __device__ __forceinline__ void dgemm_tile_deep(int64_t m, int64_t n, int64_t k,
                                                 double beta, double* __restrict__ C, int64_t ldc,
                                                 int64_t row0, int64_t col0) {
   constexpr int BK = 16, NT = 256, WTM = BM / 2, WTN = BN / 2, FM = WTM / 16, FN = WTN / 16;
-  constexpr int PAD = 16, LDA = BM + PAD, LDB = BN + PAD;
+  // A slab k-rows of BM doubles (lanes store and read along rows); B slab
+  // n-rows of BK doubles, k contiguous as in memory: a row stride of BK + 2
+  // doubles puts the 16 rows a fragment read touches (and the 16 k-values a
+  // store writes) on distinct bank pairs
+  constexpr int PAD = 16, LDA = BM + PAD, LDBK = BK + 2;
   constexpr int A_PER = BM * BK / NT, B_PER = BK * BN / NT;
   constexpr int AK = NT / BM, BJ = NT / BK;
   static_assert(NT % BM == 0 && A_PER * AK == BK && B_PER * BJ == BN, "slab split");
   __shared__ double As[2][BK][LDA];
-  __shared__ double Bs[2][BK][LDB];
+  __shared__ double Bs[2][BN][LDBK];
 
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % 2, wn = wave / 2;
@@ -220,7 +227,7 @@ __device__ __forceinline__ void dgemm_tile_deep(int64_t m, int64_t n, int64_t k,
       As[buf][ak + AK * q][ai] = (arow_ok && k0 + ak + AK * q < k) ? xa[q] : 0.0;
     const bool kok = k0 + bk < k;
 #pragma unroll
-    for (int q = 0; q < B_PER; ++q) Bs[buf][bk][bj + BJ * q] = (bcol_ok[q] && kok) ? xb[q] : 0.0;
+    for (int q = 0; q < B_PER; ++q) Bs[buf][bj + BJ * q][bk] = (bcol_ok[q] && kok) ? xb[q] : 0.0;
   };
 
   v4f64 acc[FM][FN];
@@ -231,52 +238,62 @@ __device__ __forceinline__ void dgemm_tile_deep(int64_t m, int64_t n, int64_t k,
   const int lr = lane & 15, lk = lane >> 4;
   const int arow = wm * WTM + lr, bcol = wn * WTN + lr;
 
+  // one slab: to LDS, barrier, (reload the set D slabs ahead), multiply
+  auto step = [&](int64_t kt, double (&xa)[A_PER], double (&xb)[B_PER], bool reload) {
+    const int buf = (int)(kt & 1);
+    store_slab(kt, buf, xa, xb);
+    __syncthreads();  // slab kt visible; everyone is done with slab kt-1
+    if (reload) load_slab(kt + D, xa, xb);
+    // keep the multiplies below the reload, so the wait before the next set
+    // is stored leaves this set's loads in flight
+    __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+    for (int k4 = 0; k4 < BK / 4; ++k4) {
+      double af[FM], bf[FN];
+#pragma unroll
+      for (int a = 0; a < FM; ++a) af[a] = As[buf][k4 * 4 + lk][arow + a * 16];
+#pragma unroll
+      for (int b = 0; b < FN; ++b) bf[b] = Bs[buf][bcol + b * 16][k4 * 4 + lk];
+#pragma unroll
+      for (int a = 0; a < FM; ++a)
+#pragma unroll
+        for (int b = 0; b < FN; ++b) acc[a][b] = mfma_f64_16x16x4(bf[b], af[a], acc[a][b]);
+    }
+  };
   if (ktiles > 0) {  // (k = 0: no loads; A and B may hold no element)
 #pragma unroll
-  for (int d = 0; d < D; ++d) load_slab(d, ra[d], rb[d]);
+    for (int d = 0; d < D; ++d) load_slab(d, ra[d], rb[d]);
+    // whole groups of D slabs with no branch inside, so the compiler's
+    // accounting of loads in flight stays exact across the loop
+    int64_t kt = 0;
 #pragma unroll 1
-  for (int64_t kt0 = 0; kt0 < ktiles; kt0 += D) {
+    for (; kt + D <= ktiles; kt += D) {
 #pragma unroll
-    for (int d = 0; d < D; ++d) {
-      const int64_t kt = kt0 + d;
-      if (kt < ktiles) {
-        const int buf = (int)(kt & 1);
-        store_slab(kt, buf, ra[d], rb[d]);
-        __syncthreads();  // slab kt visible; everyone is done with slab kt-1
-        load_slab(kt + D, ra[d], rb[d]);
-#pragma unroll
-        for (int k4 = 0; k4 < BK / 4; ++k4) {
-          double af[FM], bf[FN];
-#pragma unroll
-          for (int a = 0; a < FM; ++a) af[a] = As[buf][k4 * 4 + lk][arow + a * 16];
-#pragma unroll
-          for (int b = 0; b < FN; ++b) bf[b] = Bs[buf][k4 * 4 + lk][bcol + b * 16];
-#pragma unroll
-          for (int a = 0; a < FM; ++a)
-#pragma unroll
-            for (int b = 0; b < FN; ++b) acc[a][b] = mfma_f64_16x16x4(bf[b], af[a], acc[a][b]);
-        }
-      }
+      for (int d = 0; d < D; ++d) step(kt + d, ra[d], rb[d], true);
     }
-  }
+#pragma unroll
+    for (int d = 0; d + 1 < D; ++d)
+      if (kt + d < ktiles) step(kt + d, ra[d], rb[d], false);
   }
   __syncthreads();  // LDS free for the next tile of a persistent caller
+  int64_t idx[FM * FN * 4];
+  bool ok[FM * FN * 4];
+  double v[FM * FN * 4];
 #pragma unroll
   for (int a = 0; a < FM; ++a) {
     const int64_t gi = row0 + wm * WTM + a * 16 + lr;
-    if (gi >= m) continue;
 #pragma unroll
     for (int b = 0; b < FN; ++b)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
+        const int t = (a * FN + b) * 4 + r;
         const int64_t gj = col0 + wn * WTN + b * 16 + lk + 4 * r;
-        if (gj < n) {
-          double* cp = C + gi + gj * ldc;
-          const double v = alpha * acc[a][b][r];
-          *cp = (beta == 0.0) ? v : __builtin_fma(beta, *cp, v);
-        }
+        idx[t] = gi + gj * ldc;
+        ok[t] = gi < m && gj < n;
+        v[t] = alpha * acc[a][b][r];
       }
   }
+  update_c(C, idx, ok, v, beta);
 }
 
 // One 32 x 32 tile of C := beta*C + alpha*A*B (column-major A, B, C) computed
@@ -345,42 +362,55 @@ __device__ __forceinline__ void dgemm_wave_tile32(int64_t m, int64_t n, int64_t 
       acc[1][1] = mfma_f64_16x16x4(b1, a1, acc[1][1]);
     }
   };
-  // two register sets in turn: the loads of one chunk are in flight while
-  // the other chunk is multiplied
+  // WD register sets in turn: the loads of the next WD-1 chunks are in
+  // flight while one chunk is multiplied (a third set would cost the kernel
+  // its second wavefront per SIMD)
+  constexpr int WD = 2;
   const int64_t nchunks = ceil_div(k, 16);
   const int64_t k0 = k - 4 * lq;  // k-values from this lane group's first
   auto kv = [&](int64_t c) {
     const int64_t r = k0 - 16 * c;
     return (int)(r < 0 ? 0 : r < 4 ? r : 4);
   };
-  double xa0[2][4], xb0[2][4], xa1[2][4], xb1[2][4];
-  if (nchunks > 0) load(kv(0), xa0, xb0);
-  // the loads are issued unconditionally (past the end they are masked reads
-  // of A[0] / B[0]), so the wait before each multiply covers only its own set
+  double xa[WD][2][4], xb[WD][2][4];
+  if (nchunks > 0) {
+    // the loads are issued unconditionally (past the end they are masked
+    // reads of A[0] / B[0]) and whole groups of WD chunks run with no branch
+    // inside, so the wait before each multiply covers only its own set
+#pragma unroll
+    for (int d = 0; d < WD; ++d) load(kv(d), xa[d], xb[d]);
+    int64_t c = 0;
 #pragma unroll 1
-  for (int64_t c = 0; c < nchunks; c += 2) {
-    load(kv(c + 1), xa1, xb1);
-    mma(kv(c), xa0, xb0);
-    load(kv(c + 2), xa0, xb0);
-    if (c + 1 < nchunks) mma(kv(c + 1), xa1, xb1);
+    for (; c + WD <= nchunks; c += WD) {
+#pragma unroll
+      for (int d = 0; d < WD; ++d) {
+        mma(kv(c + d), xa[d], xb[d]);
+        __builtin_amdgcn_sched_barrier(0);
+        load(kv(c + d + WD), xa[d], xb[d]);
+        __builtin_amdgcn_sched_barrier(0);
+      }
+    }
+#pragma unroll
+    for (int d = 0; d + 1 < WD; ++d)
+      if (c + d < nchunks) mma(kv(c + d), xa[d], xb[d]);
   }
   // lane l holds C[i][j], i = l%16, j = l/16 + 4r of each 16x16 block
+  int64_t idx[16];
+  bool ok[16];
+  double v[16];
 #pragma unroll
-  for (int a = 0; a < 2; ++a) {
-    if (!(a ? i1 : i0)) continue;
-    double* crow = C + row0 + 16 * a + lr;
+  for (int a = 0; a < 2; ++a)
 #pragma unroll
     for (int b = 0; b < 2; ++b)
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
+        const int t = (a * 2 + b) * 4 + r;
         const int64_t j = col0 + 16 * b + lq + 4 * r;
-        if (j < n) {
-          double* cp = crow + j * ldc;
-          const double v = alpha * acc[a][b][r];
-          *cp = (beta == 0.0) ? v : __builtin_fma(beta, *cp, v);
-        }
+        idx[t] = row0 + 16 * a + lr + j * ldc;
+        ok[t] = (a ? i1 : i0) && j < n;
+        v[t] = alpha * acc[a][b][r];
       }
-  }
+  update_c(C, idx, ok, v, beta);
 }
 
 }  // namespace momms
@@ -518,19 +548,20 @@ __device__ __forceinline__ void dgemm_tile_dma(DgemmDmaSmem<STAGES>& sm, int64_t
 #pragma unroll
   for (int a = 0; a < FM; ++a) {
     const int64_t gi = row0 + wm * WTM + a * 16 + lr;
-    if (gi >= m) continue;
+    int64_t idx[FN * 4];
+    bool ok[FN * 4];
+    double v[FN * 4];
 #pragma unroll
     for (int b = 0; b < FN; ++b) {
 #pragma unroll
       for (int r = 0; r < 4; ++r) {
         const int64_t gj = col0 + wn * WTN + b * 16 + lk + 4 * r;
-        if (gj < n) {
-          double* cp = C + gi + gj * ldc;
-          const double s = alpha * acc[a][b][r];
-          *cp = (beta == 0.0) ? s : __builtin_fma(beta, *cp, s);
-        }
+        idx[b * 4 + r] = gi + gj * ldc;
+        ok[b * 4 + r] = gi < m && gj < n;
+        v[b * 4 + r] = alpha * acc[a][b][r];
       }
     }
+    update_c(C, idx, ok, v, beta);
   }
 }
 

@@ -5,6 +5,7 @@
 #pragma once
 #include <math.h>
 
+#include "c_update.hpp"
 #include "mfma.hpp"
 
 namespace momms {
@@ -255,16 +256,22 @@ __device__ __forceinline__ void split_tile(int64_t m, int64_t n, int64_t kpad, d
   const int64_t gi = tm * BM + wm * 32 + (lane & 31);
   if (gi >= m) return;
   const int ei = EA[gi];
+  // EB is padded to whole tiles, so every column of the tile indexes it
+  int eb[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) eb[r] = EB[tn * BN + wn * 32 + 8 * (r >> 2) + 4 * kh + (r & 3)];
+  int64_t idx[16];
+  bool ok[16];
+  double v[16];
 #pragma unroll
   for (int r = 0; r < 16; ++r) {
     const int64_t gj = tn * BN + wn * 32 + 8 * (r >> 2) + 4 * kh + (r & 3);
-    if (gj < n) {
-      double v = alpha * acc[r];
-      if (ei != kNonFinite && EB[gj] != kNonFinite) v = ldexp(v, ei + EB[gj]);
-      double* cp = C + gi * rsC + gj * csC;
-      *cp = (beta == 0.0) ? v : __builtin_fma(beta, *cp, v);
-    }
+    v[r] = alpha * acc[r];
+    if (ei != kNonFinite && eb[r] != kNonFinite) v[r] = ldexp(v[r], ei + eb[r]);
+    idx[r] = gi * rsC + gj * csC;
+    ok[r] = gj < n;
   }
+  update_c(C, idx, ok, v, beta);
 }
 
 }  // namespace momms

@@ -129,6 +129,46 @@ def test_batch_reads_only_its_problems(mode):
     plan.close()
 
 
+def test_batch_wide_leading_dimensions():
+    """Problems with lda, ldb or ldc above 2^21 (the 64-bit-addressed tiles)
+    in one batch with ordinary ones; padding is NaN."""
+    W = (1 << 21) + 5
+    specs = [(70, 50, 3, W, 3, 70), (40, 3, 20, 40, W, 40), (33, 2, 17, 33, 17, W),
+             (64, 64, 64, 64, 64, 64), (20, 30, 40, 21, 41, 22), (9, 9, 0, 9, 1, 9)]
+    rows, ao, bo, co = [], 0, 0, 0
+    for (m, n, k, lda, ldb, ldc) in specs:
+        rows.append((m, n, k, ao, bo, co, lda, ldb, ldc))
+        ao += lda * k
+        bo += ldb * n
+        co += ldc * n
+    g = torch.Generator(device="cuda").manual_seed(SEED + 77)
+    A = torch.full((ao,), float("nan"), device="cuda", dtype=torch.float64)
+    B = torch.full((bo,), float("nan"), device="cuda", dtype=torch.float64)
+    C0 = torch.full((co,), float("nan"), device="cuda", dtype=torch.float64)
+    for (m, n, k, a0, b0, c0_, lda, ldb, ldc) in rows:
+        if k:
+            view(A, a0, m, k, lda).copy_(torch.rand(m, k, generator=g, device="cuda", dtype=torch.float64) - 0.5)
+            view(B, b0, k, n, ldb).copy_(torch.rand(k, n, generator=g, device="cuda", dtype=torch.float64) - 0.5)
+        view(C0, c0_, m, n, ldc).copy_(torch.rand(m, n, generator=g, device="cuda", dtype=torch.float64) - 0.5)
+    plan = capi.BatchPlan(rows, mode=0, autotune=False)
+    C = C0.clone()
+    plan.run(A, B, C, 1.0, 1.0)
+    torch.cuda.synchronize()
+    for (m, n, k, a0, b0, c0_, lda, ldb, ldc) in rows:
+        Am = view(A, a0, m, k, lda).cpu().numpy() if k else np.zeros((m, 0))
+        Bm = view(B, b0, k, n, ldb).cpu().numpy() if k else np.zeros((0, n))
+        Cm0 = view(C0, c0_, m, n, ldc).cpu().numpy()
+        got = view(C, c0_, m, n, ldc).cpu().numpy()
+        ref, mag = O.dd_gemm(np.asfortranarray(Am), np.asfortranarray(Bm), np.asfortranarray(Cm0))
+        assert (np.abs(got - ref) <= gamma(k + 1) * mag).all(), (m, n, k, lda, ldb, ldc)
+    # nothing outside the problems was written
+    written = torch.zeros(co, dtype=torch.bool, device="cuda")
+    for (m, n, k, a0, b0, c0_, lda, ldb, ldc) in rows:
+        view(written, c0_, m, n, ldc).fill_(True)
+    assert torch.isnan(C[~written]).all()
+    plan.close()
+
+
 @pytest.mark.parametrize("s", [3, 7])
 def test_split_batch_bitwise_against_restatement(s):
     rows, al, bl, cl = edge_rows()
