@@ -1,4 +1,4 @@
-// Lean FP64 tiles of the batched kernel: buffer loads and stores with 32-bit
+// Lean FP64 tiles of the batched kernels: buffer loads and stores with 32-bit
 // offsets instead of 64-bit addresses, so a workgroup needs few registers
 // and four of them fit on a CU (the tiles of small problems are short and
 // bound by memory latency, which only more resident waves hide).
@@ -69,17 +69,18 @@ __device__ __forceinline__ void lean_update_c(__amdgpu_buffer_rsrc_t rc, Off&& o
   }
 }
 
-// 64 x 64 tile (row0, col0) of one problem by a workgroup of four waves
-// (2 x 2 of 32 x 32), 16-deep slabs through LDS with D slabs of loads in
+// BM x BN tile (row0, col0) of one problem by a workgroup of four waves
+// (2 x 2 of BM/2 x BN/2), 16-deep slabs through LDS with D slabs of loads in
 // flight. All arguments are uniform over the workgroup. Each element sums its
 // k-values in order, four per MFMA step.
-template <int D>
-__device__ __forceinline__ void batch_tile64(int64_t m, int64_t n, int64_t k, double alpha,
-                                             const double* A, int64_t lda, const double* B,
-                                             int64_t ldb, double beta, double* C, int64_t ldc,
-                                             int64_t row0, int64_t col0) {
-  constexpr int BM = 64, BN = 64, BK = 16, NT = 256, WTM = 32, WTN = 32, FM = 2, FN = 2;
+template <int BM, int BN, int D>
+__device__ __forceinline__ void batch_tile(int64_t m, int64_t n, int64_t k, double alpha,
+                                           const double* A, int64_t lda, const double* B,
+                                           int64_t ldb, double beta, double* C, int64_t ldc,
+                                           int64_t row0, int64_t col0) {
+  constexpr int BK = 16, NT = 256, WTM = BM / 2, WTN = BN / 2, FM = WTM / 16, FN = WTN / 16;
   constexpr int A_PER = BM * BK / NT, B_PER = BK * BN / NT, AK = NT / BM, BJ = NT / BK;
+  static_assert(NT % BM == 0 && A_PER * AK == BK && B_PER * BJ == BN, "slab split");
   // A slab: k-rows of BM doubles; B slab: n-rows of BK doubles (k contiguous
   // as in memory) with a stride of BK + 2, so the 16 rows a fragment read or
   // a store touches fall on distinct bank pairs
@@ -172,15 +173,24 @@ __device__ __forceinline__ void batch_tile64(int64_t m, int64_t n, int64_t k, do
   // lane l holds C[i][j], i = l%16, j = l/16 + 4r of each 16x16 block
   const int64_t coff = row0 + col0 * ldc;
   const auto rc = block_rsrc(C + coff, (n - 1) * ldc + m - coff);
-  // output t = (a*FN + b)*4 + r
-  lean_update_c<FM * FN * 4>(
-      rc,
-      [&](int t) {
-        const int a = t / (4 * FN), b = (t / 4) % FN, r = t % 4;
-        const int il = wm * WTM + a * 16 + lr, jl = wn * WTN + b * 16 + lk + 4 * r;
-        return (il < mr && jl < nr) ? (il + jl * lc) * 8u : kOob;
-      },
-      [&](int t) { return alpha * acc[t / (4 * FN)][(t / 4) % FN][t % 4]; }, beta);
+  // outputs t = (a*FN + b)*4 + r in groups of G (fewer for the wider tiles,
+  // whose accumulators leave fewer registers for values in flight)
+  constexpr int NO = FM * FN * 4, G = NO <= 16 ? NO : 8;
+#pragma unroll
+  for (int t0 = 0; t0 < NO; t0 += G) {
+    lean_update_c<G>(
+        rc,
+        [&](int u) {
+          const int t = t0 + u, a = t / (4 * FN), b = (t / 4) % FN, r = t % 4;
+          const int il = wm * WTM + a * 16 + lr, jl = wn * WTN + b * 16 + lk + 4 * r;
+          return (il < mr && jl < nr) ? (il + jl * lc) * 8u : kOob;
+        },
+        [&](int u) {
+          const int t = t0 + u;
+          return alpha * acc[t / (4 * FN)][(t / 4) % FN][t % 4];
+        },
+        beta);
+  }
 }
 
 // 32 x 32 tile (row0, col0) of one problem by a single wavefront, no LDS and
@@ -195,7 +205,7 @@ __device__ __forceinline__ void batch_wave_tile32(int64_t m, int64_t n, int64_t 
                                                   int64_t ldc, int64_t row0, int64_t col0) {
   constexpr int KC = 8;
   const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
-  // lane arithmetic in 32 bits (see batch_tile64)
+  // lane arithmetic in 32 bits (see batch_tile)
   const int mr = (int)(m - row0 < 32 ? m - row0 : 32), nr = (int)(n - col0 < 32 ? n - col0 : 32);
   const uint32_t la = (uint32_t)lda, lb = (uint32_t)ldb, lc = (uint32_t)ldc;
   const uint32_t lda8 = la * 8u;

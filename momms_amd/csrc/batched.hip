@@ -58,7 +58,7 @@ __global__ void reset_counters(unsigned* c, int n) {
 
 
 // Plain mode, one persistent kernel over kBatchQueues queues, one per XCD.
-// An item with prob >= 0 is a 64 x 64 workgroup tile (batch_tile64); an item
+// An item with prob >= 0 is a 64 x 64 workgroup tile (batch_tile<64, 64>); an item
 // with prob = -1 - q is a quad: wavefront w of the workgroup computes 32 x 32
 // tile 4q + w of the wave list (batch_wave_tile32; entries with prob < 0 pad
 // a queue's wave list to whole quads), so small problems run four tiles per
@@ -68,13 +68,9 @@ __global__ void reset_counters(unsigned* c, int n) {
 // tiles share are fetched into one XCD's L2. A workgroup whose queue is
 // empty takes items from the next queues. Each queue is sorted by k, largest
 // first, so its tail is short and a quad's tiles take about equally long.
-// Only problems with lda, ldb, ldc <= kBatchLeanLd come here; the others go
-// to batched_wide_kernel.
-constexpr int kBatchQueues = 8;
-struct QueueBounds {
-  int off[kBatchQueues + 1];  // queue q holds items [off[q], off[q+1])
-};
-
+// Only problems with lda, ldb, ldc <= kBatchLeanLd and a smaller side below
+// the plan's threshold T2 come here; the others go to batched_wide_kernel
+// and batched_big_kernel.
 // Three workgroups per CU (at most 168 registers a lane, 39 KB of LDS
 // each): more resident waves hide the latency of short tiles; a fourth would
 // cost the tiles their deeper prefetch or spill.
@@ -92,7 +88,7 @@ batched_plain_kernel(const momms_gemm_desc* __restrict__ d, const BatchItem* __r
       const int prob = uniform(w.prob);
       if (prob >= 0) {
         const momms_gemm_desc p = d[prob];
-        batch_tile64<3>(p.m, p.n, p.k, alpha, A + p.a_off, p.lda, B + p.b_off, p.ldb, beta,
+        batch_tile<64, 64, 3>(p.m, p.n, p.k, alpha, A + p.a_off, p.lda, B + p.b_off, p.ldb, beta,
                         C + p.c_off, p.ldc, (int64_t)uniform(w.tm) * 64,
                         (int64_t)uniform(w.tn) * 64);
       } else {
@@ -201,16 +197,20 @@ batched_split_kernel(const momms_gemm_desc* __restrict__ d, const SplitProb* __r
 // host side
 
 // Plain mode: a problem whose smaller side is at least the plan's threshold
-// T takes 64 x 64 workgroup tiles, the others 32 x 32 one-wavefront tiles,
-// all in one persistent kernel (batched_plain_kernel). The autotuner picks T
-// by timing the whole batch with each candidate.
+// T2 takes 128 x 128 tiles (batched_big_kernel), one with it at least T
+// 64 x 64 workgroup tiles, the others 32 x 32 one-wavefront tiles (both in
+// batched_plain_kernel). The autotuner picks (T, T2) by timing the whole
+// batch with each pair of candidates.
 static const int kThresholds[] = {32, 48, 64, 96, 1 << 30};
+static const int kBigThresholds[] = {128, 192, 256, 1 << 30};
 constexpr int kDefaultThreshold = 48;
+constexpr int kDefaultBigThreshold = 256;
 
-// Tuned threshold, remembered for the process (keyed by a coarse class of the
-// batch: problem count and mean k), so plans for similar batches reuse it.
+// Tuned thresholds, remembered for the process (keyed by a coarse class of
+// the batch: problem count and mean k), so plans for similar batches reuse
+// them.
 static std::mutex g_tune_mu;
-static std::map<int, int> g_tuned;
+static std::map<int, std::pair<int, int>> g_tuned;
 
 static int occupancy(const void* kernel, int threads, int cus) {
   int per = 0;
@@ -235,8 +235,12 @@ struct momms_batch_plan {
   std::vector<momms_gemm_desc> descs;
   momms_gemm_desc* d_desc = nullptr;
   unsigned* d_counters = nullptr;
-  momms::Launchable plain{};                 // plain mode: the queue of tiles and quads
+  momms::Launchable plain{};                 // plain mode: the queue of 64 x 64 tiles and quads
   momms::Launchable wide{};                  // plain mode: tiles of problems with a large ld
+  momms::Launchable big{};                   // plain mode: 128 x 128 tiles
+  momms::QueueBounds qb_big{};               // plain mode: the per-XCD queues in big.items
+  int threshold_big = 0;                     // plain mode: min(m, n) for 128 x 128 tiles
+  int64_t tiles_big = 0;
   momms::BatchItem* d_waves = nullptr;       // plain mode: the 32 x 32 wave tiles
   int nwaves = 0;                            // plain mode: wave list length (whole quads)
   int64_t nwave_tiles = 0;                   // plain mode: 32 x 32 tiles in it
@@ -286,49 +290,92 @@ static std::vector<BatchItem> tile_items(const std::vector<momms_gemm_desc>& d,
 }
 
 // The plain-mode queue for threshold T (replacing an earlier one).
-static hipError_t build_plain(momms_batch_plan* P, int T) {
-  (void)hipFree(P->plain.items);
-  (void)hipFree(P->wide.items);
-  (void)hipFree(P->d_waves);
-  P->plain = Launchable{};
-  P->wide = Launchable{};
-  P->d_waves = nullptr;
-  P->threshold = T;
-  const auto& d = P->descs;
-  std::vector<int> big, small, wide;
-  for (size_t i = 0; i < d.size(); ++i) {
-    if (!d[i].m || !d[i].n) continue;
-    if (std::max(d[i].lda, std::max(d[i].ldb, d[i].ldc)) > kBatchLeanLd)
-      wide.push_back((int)i);
-    else
-      (std::min(d[i].m, d[i].n) >= T ? big : small).push_back((int)i);
-  }
-  // problems to queues, largest work first onto the least loaded queue;
-  // a 64 x 64 tile counts as four 32 x 32 ones, each with a fixed cost of
-  // about 32 k-steps (prologue and C update)
+// Problems to queues, largest work first onto the least loaded queue: the
+// work of a problem is its count of 32 x 32 tile-units (a 64 x 64 tile is 4,
+// a 128 x 128 one 16), each with a fixed cost of about 32 k-steps (prologue
+// and C update).
+static void assign_queues(const std::vector<momms_gemm_desc>& d, const std::vector<int>& probs,
+                          const std::vector<int>& edge, std::vector<int> (&out)[kBatchQueues]) {
   std::vector<std::pair<double, int>> work;
-  for (int p : big)
-    work.push_back({4.0 * (double)(ceil_div(d[p].m, 64) * ceil_div(d[p].n, 64)) * (double)(d[p].k + 32), p});
-  for (int p : small)
-    work.push_back({(double)(ceil_div(d[p].m, 32) * ceil_div(d[p].n, 32)) * (double)(d[p].k + 32), p});
+  for (size_t i = 0; i < probs.size(); ++i) {
+    const int p = probs[i], e = edge[i];
+    const double units = (double)(ceil_div(d[p].m, e) * ceil_div(d[p].n, e)) * (e / 32) * (e / 32);
+    work.push_back({units * (double)(d[p].k + 32), (int)i});
+  }
   std::stable_sort(work.begin(), work.end(), [](const auto& a, const auto& b) { return a.first > b.first; });
-  std::vector<int> qbig[kBatchQueues], qsmall[kBatchQueues];
   double load[kBatchQueues] = {};
-  std::vector<char> is_big(d.size(), 0);
-  for (int p : big) is_big[p] = 1;
   for (const auto& w : work) {
     const int q = (int)(std::min_element(load, load + kBatchQueues) - load);
     load[q] += w.first;
-    (is_big[w.second] ? qbig[q] : qsmall[q]).push_back(w.second);
+    out[q].push_back((int)w.second);
+  }
+}
+
+// The plain-mode queues for thresholds T and T2 (replacing earlier ones): a
+// problem whose smaller side is at least T2 takes 128 x 128 tiles
+// (batched_big_kernel), one with it at least T 64 x 64 tiles, the others
+// 32 x 32 one-wavefront tiles (both in batched_plain_kernel).
+static hipError_t build_plain(momms_batch_plan* P, int T, int T2) {
+  (void)hipFree(P->plain.items);
+  (void)hipFree(P->big.items);
+  (void)hipFree(P->wide.items);
+  (void)hipFree(P->d_waves);
+  P->plain = Launchable{};
+  P->big = Launchable{};
+  P->wide = Launchable{};
+  P->d_waves = nullptr;
+  P->threshold = T;
+  P->threshold_big = T2;
+  const auto& d = P->descs;
+  std::vector<int> big, mid, wide;  // mid: 64 x 64 or wave tiles
+  std::vector<int> mid_edge;
+  for (size_t i = 0; i < d.size(); ++i) {
+    if (!d[i].m || !d[i].n) continue;
+    const int64_t side = std::min(d[i].m, d[i].n);
+    if (std::max(d[i].lda, std::max(d[i].ldb, d[i].ldc)) > kBatchLeanLd) {
+      wide.push_back((int)i);
+    } else if (side >= T2) {
+      big.push_back((int)i);
+    } else {
+      mid.push_back((int)i);
+      mid_edge.push_back(side >= T ? 64 : 32);
+    }
+  }
+  hipError_t e;
+  // 128 x 128 tiles, per queue by k
+  {
+    std::vector<int> qi_[kBatchQueues];
+    assign_queues(d, big, std::vector<int>(big.size(), 128), qi_);
+    std::vector<BatchItem> items;
+    for (int qi = 0; qi < kBatchQueues; ++qi) {
+      std::vector<int> probs;
+      for (int i : qi_[qi]) probs.push_back(big[i]);
+      std::sort(probs.begin(), probs.end());
+      const auto tiles = tile_items(d, probs, 128);
+      P->qb_big.off[qi] = (int)items.size();
+      items.insert(items.end(), tiles.begin(), tiles.end());
+    }
+    P->qb_big.off[kBatchQueues] = (int)items.size();
+    if (!items.empty()) {
+      if ((e = upload((void**)&P->big.items, items.data(), items.size() * sizeof(BatchItem))) != hipSuccess)
+        return e;
+      P->big.nitems = (int)items.size();
+      const int slots = occupancy(batched_big_kernel_ptr(), 256, P->cus);
+      P->big.grid = std::max(1, std::min(P->big.nitems, slots));
+    }
   }
   // per queue: its 64 x 64 tiles and quads of its wave tiles, by k
+  std::vector<int> qmid[kBatchQueues];
+  assign_queues(d, mid, mid_edge, qmid);
   std::vector<BatchItem> items, waves;
   int64_t ntiles = 0, nwave_tiles = 0;
   for (int qi = 0; qi < kBatchQueues; ++qi) {
-    std::sort(qbig[qi].begin(), qbig[qi].end());
-    std::sort(qsmall[qi].begin(), qsmall[qi].end());
-    const auto tiles = tile_items(d, qbig[qi], 64);
-    auto wq = tile_items(d, qsmall[qi], 32);
+    std::vector<int> p64, p32;
+    for (int i : qmid[qi]) (mid_edge[i] == 64 ? p64 : p32).push_back(mid[i]);
+    std::sort(p64.begin(), p64.end());
+    std::sort(p32.begin(), p32.end());
+    const auto tiles = tile_items(d, p64, 64);
+    auto wq = tile_items(d, p32, 32);
     ntiles += (int64_t)tiles.size();
     nwave_tiles += (int64_t)wq.size();
     std::vector<std::pair<int64_t, BatchItem>> q;
@@ -343,15 +390,16 @@ static hipError_t build_plain(momms_batch_plan* P, int T) {
     for (auto& x : q) items.push_back(x.second);
   }
   P->qb.off[kBatchQueues] = (int)items.size();
-  hipError_t e;
-  if ((e = upload((void**)&P->plain.items, items.data(), items.size() * sizeof(BatchItem))) != hipSuccess)
-    return e;
-  if ((e = upload((void**)&P->d_waves, waves.data(), waves.size() * sizeof(BatchItem))) != hipSuccess)
-    return e;
-  P->plain.nitems = (int)items.size();
-  P->nwaves = (int)waves.size();
-  const int slots = occupancy(reinterpret_cast<const void*>(&batched_plain_kernel), 256, P->cus);
-  P->plain.grid = std::max(1, std::min(P->plain.nitems, slots));
+  if (!items.empty()) {
+    if ((e = upload((void**)&P->plain.items, items.data(), items.size() * sizeof(BatchItem))) != hipSuccess)
+      return e;
+    if ((e = upload((void**)&P->d_waves, waves.data(), waves.size() * sizeof(BatchItem))) != hipSuccess)
+      return e;
+    P->plain.nitems = (int)items.size();
+    P->nwaves = (int)waves.size();
+    const int slots = occupancy(reinterpret_cast<const void*>(&batched_plain_kernel), 256, P->cus);
+    P->plain.grid = std::max(1, std::min(P->plain.nitems, slots));
+  }
   const auto wide_tiles = tile_items(d, wide, 64);
   if (!wide_tiles.empty()) {
     if ((e = upload((void**)&P->wide.items, wide_tiles.data(), wide_tiles.size() * sizeof(BatchItem))) !=
@@ -361,7 +409,8 @@ static hipError_t build_plain(momms_batch_plan* P, int T) {
     const int wslots = occupancy(batched_wide_kernel_ptr(), 256, P->cus);
     P->wide.grid = std::max(1, std::min(P->wide.nitems, wslots));
   }
-  P->total_tiles = ntiles + nwave_tiles + (int64_t)wide_tiles.size();
+  P->tiles_big = P->big.nitems;
+  P->total_tiles = P->big.nitems + ntiles + nwave_tiles + (int64_t)wide_tiles.size();
   P->nwave_tiles = nwave_tiles;
   return hipSuccess;
 }
@@ -380,12 +429,16 @@ static hipError_t build_plain(momms_batch_plan* P, int T) {
 
 static hipError_t enqueue(const momms_batch_plan* P, double alpha, const double* A,
                           const double* B, double beta, double* C, hipStream_t st) {
-  // plain mode: counters 0..7 the queues, 8 the wide problems
-  const int nctr = P->mode == 0 ? kBatchQueues + 1 : 1;
+  // plain mode: counters 0..7 the queues, 8 the wide problems, 9..16 the
+  // queues of 128 x 128 tiles
+  const int nctr = P->mode == 0 ? 2 * kBatchQueues + 1 : 1;
   hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, st, P->d_counters, nctr);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
   if (P->mode == 0) {
+    if (P->big.nitems)
+      launch_batched_big(P->big.grid, st, P->d_desc, P->big.items, P->qb_big,
+                         P->d_counters + kBatchQueues + 1, alpha, A, B, beta, C);
     if (P->plain.nitems)
       hipLaunchKernelGGL(batched_plain_kernel, dim3((unsigned)P->plain.grid), dim3(256), 0, st,
                          P->d_desc, P->plain.items, P->qb, P->d_waves, P->nwaves,
@@ -423,6 +476,7 @@ static const void* split_kernel_ptr(int s) {
 static void free_plan(momms_batch_plan* P) {
   if (!P) return;
   (void)hipFree(P->plain.items);
+  (void)hipFree(P->big.items);
   (void)hipFree(P->wide.items);
   (void)hipFree(P->d_waves);
   (void)hipFree(P->mul.items);
@@ -577,7 +631,7 @@ extern "C" int momms_hip_batch_plan_create(const momms_gemm_desc* descs, int64_t
     if ((e = hipMalloc((void**)&scratchC, cext * sizeof(double))) != hipSuccess) MOMMS_FAIL(e);
   }
 
-  char line[256];
+  char line[384];
   if (mode == 0) {
     int64_t ksum = 0, live = 0;
     for (auto& p : P->descs)
@@ -588,42 +642,49 @@ extern "C" int momms_hip_batch_plan_create(const momms_gemm_desc* descs, int64_t
     const int kcls = live ? (int)std::min<int64_t>(8, (int64_t)std::log2(1.0 + (double)ksum / live)) : 0;
     const int ccls = (int)std::min<int64_t>(15, (int64_t)std::log2(1.0 + (double)live));
     const int key = ccls * 16 + kcls;
-    int T = kDefaultThreshold;
+    int T = kDefaultThreshold, T2 = kDefaultBigThreshold;
     bool known = false;
     {
       std::lock_guard<std::mutex> g(g_tune_mu);
       auto it = g_tuned.find(key);
       if (it != g_tuned.end()) {
-        T = it->second;
+        T = it->second.first;
+        T2 = it->second.second;
         known = true;
       }
     }
     if (!known && autotune && live) {
       float best = 1e30f;
-      for (int cand : kThresholds) {
-        if ((e = build_plain(P, cand)) != hipSuccess) {
-          (void)hipFree(scratchC);
-          MOMMS_FAIL(e);
+      for (int cand2 : kBigThresholds)
+        for (int cand : kThresholds) {
+          if (cand > cand2 && cand != (1 << 30)) continue;
+          if ((e = build_plain(P, cand, cand2)) != hipSuccess) {
+            (void)hipFree(scratchC);
+            MOMMS_FAIL(e);
+          }
+          (void)time_enqueue(P, A, B, scratchC, st, 1);  // warm up
+          const float ms = time_enqueue(P, A, B, scratchC, st, 3);
+          if (ms < best) {
+            best = ms;
+            T = cand;
+            T2 = cand2;
+          }
         }
-        (void)time_enqueue(P, A, B, scratchC, st, 1);  // warm up
-        const float ms = time_enqueue(P, A, B, scratchC, st, 3);
-        if (ms < best) {
-          best = ms;
-          T = cand;
-        }
-      }
       std::lock_guard<std::mutex> g(g_tune_mu);
-      g_tuned[key] = T;
+      g_tuned[key] = {T, T2};
     }
-    if ((e = build_plain(P, T)) != hipSuccess) {
+    if ((e = build_plain(P, T, T2)) != hipSuccess) {
       (void)hipFree(scratchC);
       MOMMS_FAIL(e);
     }
     std::snprintf(line, sizeof(line),
-                  "{\"mode\":\"fp64\",\"threshold\":%d,\"tiles_64x64\":%lld,"
-                  "\"tiles_32x32_wavefront\":%lld,\"queue\":%d,\"queues\":%d,\"workgroups\":%d}",
-                  T, (long long)(P->total_tiles - P->nwave_tiles), (long long)P->nwave_tiles,
-                  P->plain.nitems, kBatchQueues, P->plain.grid);
+                  "{\"mode\":\"fp64\",\"threshold\":%d,\"threshold_128\":%d,\"tiles_128x128\":%lld,"
+                  "\"tiles_64x64\":%lld,\"tiles_32x32_wavefront\":%lld,\"queue\":%d,\"queues\":%d,"
+                  "\"workgroups\":%d,\"workgroups_128\":%d}",
+                  T, T2, (long long)P->tiles_big,
+                  (long long)(P->total_tiles - P->nwave_tiles - P->tiles_big),
+                  (long long)P->nwave_tiles, P->plain.nitems, kBatchQueues, P->plain.grid,
+                  P->big.grid);
     P->info = line;
   } else {
     int s = slices;
@@ -639,7 +700,7 @@ extern "C" int momms_hip_batch_plan_create(const momms_gemm_desc* descs, int64_t
       std::vector<int> all;
       for (int64_t i = 0; i < count; ++i)
         if (P->descs[i].m && P->descs[i].n) all.push_back((int)i);
-      e = build_plain(&T, kDefaultThreshold);
+      e = build_plain(&T, kDefaultThreshold, kDefaultBigThreshold);
       double* plainC = nullptr;
       if (e == hipSuccess) e = hipMalloc((void**)&plainC, std::max<size_t>(cext, 1) * sizeof(double));
       if (e == hipSuccess) e = enqueue(&T, 1.0, A, B, 0.0, plainC, st);
@@ -671,9 +732,11 @@ extern "C" int momms_hip_batch_plan_create(const momms_gemm_desc* descs, int64_t
         }
       }
       (void)hipFree(T.plain.items);
+      (void)hipFree(T.big.items);
       (void)hipFree(T.wide.items);
       (void)hipFree(T.d_waves);
       T.plain.items = nullptr;
+      T.big.items = nullptr;
       T.wide.items = nullptr;
       T.d_waves = nullptr;
       T.d_desc = nullptr;  // owned by P

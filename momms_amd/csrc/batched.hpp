@@ -33,6 +33,33 @@ __device__ __forceinline__ void run_queue(unsigned* counter, int count, Body&& b
   }
 }
 
+// Per-XCD work queues of a persistent launch: workgroup b serves queue
+// b % kBatchQueues first (workgroups are dispatched to the 8 XCDs in turn),
+// then takes items from the next queues.
+constexpr int kBatchQueues = 8;
+struct QueueBounds {
+  int off[kBatchQueues + 1];  // queue q holds items [off[q], off[q+1])
+};
+
+// body(q, it) for every item of every queue, home queue first.
+template <typename Body>
+__device__ __forceinline__ void run_queues(const QueueBounds& qb, unsigned* counters, Body&& body) {
+  const int home = (int)(blockIdx.x % kBatchQueues);
+  for (int s = 0; s < kBatchQueues; ++s) {
+    const int q = (home + s) % kBatchQueues;
+    const int base = qb.off[q];
+    run_queue(counters + q, qb.off[q + 1] - base, [&](int it) { body(base + it); });
+  }
+}
+
+// Problems whose smaller side reaches the plan's second threshold: 128 x 128
+// lean tiles (batch_tile<128, 128>), two workgroups per CU, per-XCD queues
+// (batched_big.hip).
+const void* batched_big_kernel_ptr();
+void launch_batched_big(int grid, hipStream_t st, const momms_gemm_desc* d, const BatchItem* items,
+                        QueueBounds qb, unsigned* counters, double alpha, const double* A,
+                        const double* B, double beta, double* C);
+
 // Problems of a batch with a leading dimension too large for the 32-bit
 // offsets of the lean tiles: 64 x 64 tiles with 64-bit addressing, one
 // queue (batched_wide.hip; its own translation unit, so that its register
