@@ -145,16 +145,31 @@ crt_pack(const double* __restrict__ X, int64_t count, int64_t len, int64_t sv, i
   const int tid = threadIdx.x;
   const int64_t v0 = (int64_t)blockIdx.x * 64, p0 = (int64_t)blockIdx.y * 64;
   const bool v_unit = (sv == 1);
+  if (count > 0 && len > 0) {  // (the same for the whole grid: X has an element 0)
+    // all 16 loads first (X[0] for elements outside X, masked after), then the
+    // LDS stores, so the loads of a thread are in flight together
+    double x[16];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int e = tid + 256 * r;
-    const int a = e & 63, b = e >> 6;  // a runs fastest
-    const int vv = v_unit ? a : b;
-    const int pp = v_unit ? b : a;
-    const int64_t gv = v0 + vv, gp = p0 + pp;
-    double x = 0.0;
-    if (gv < count && gp < len) x = X[gv * sv + gp * sp];
-    tile[vv][pp] = x;
+    for (int r = 0; r < 16; ++r) {
+      const int e = tid + 256 * r;
+      const int a = e & 63, b = e >> 6;  // a runs fastest
+      const int64_t gv = v0 + (v_unit ? a : b), gp = p0 + (v_unit ? b : a);
+      x[r] = X[(gv < count && gp < len) ? gv * sv + gp * sp : 0];
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int e = tid + 256 * r;
+      const int a = e & 63, b = e >> 6;
+      const int vv = v_unit ? a : b, pp = v_unit ? b : a;
+      tile[vv][pp] = (v0 + vv < count && p0 + pp < len) ? x[r] : 0.0;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int e = tid + 256 * r;
+      const int a = e & 63, b = e >> 6;
+      tile[v_unit ? a : b][v_unit ? b : a] = 0.0;
+    }
   }
   __syncthreads();
   const int64_t nslabs = lpad / kSlab;
@@ -270,43 +285,64 @@ crt_reconstruct(int64_t m, int64_t n, int64_t k, int beta_bits, double alpha, do
                 const int* __restrict__ EB, const double* __restrict__ A, int64_t rsA,
                 int64_t csA, const double* __restrict__ B, int64_t rsB, int64_t csB) {
   constexpr CrtRecon K = crt_recon_constants(N);
-  const int64_t i0 = ((int64_t)blockIdx.x * blockDim.x + threadIdx.x) * kCrtRows;
-  if (i0 >= m) return;
+  // a thread reconstructs rows i0 .. i0+7 of column j (16-byte residue loads);
+  // the results go through LDS so that the C update of the block runs along
+  // columns, 64 consecutive rows per wavefront access
+  __shared__ double vt[256 * (kCrtRows + 1)];
+  const int tid = threadIdx.x;
+  const int64_t rb = (int64_t)blockIdx.x * (256 * kCrtRows);
+  const int64_t i0 = rb + (int64_t)tid * kCrtRows;
+  const bool active = i0 < m;  // (rows up to i0 + 7 < mpad then)
   for (int64_t j = blockIdx.y; j < n; j += gridDim.y) {
-    CrtSums a[kCrtRows];
-#pragma unroll
-    for (int e = 0; e < kCrtRows; ++e) crt_sums_init(a[e]);
-#pragma unroll
-    for (int l = 0; l < N; ++l) {
-      const uint4 w = *reinterpret_cast<const uint4*>(RC + (int64_t)l * mpad * npad + j * mpad + i0);
-      const uint32_t h[4] = {w.x, w.y, w.z, w.w};
-#pragma unroll
-      for (int e = 0; e < kCrtRows; ++e) {
-        const int16_t r = (int16_t)(h[e / 2] >> (16 * (e % 2)));
-        crt_sums_add(a[e], (float)r, l, K);
-      }
-    }
-    const int ej = EB[j];
-    int ea[kCrtRows];  // EA holds mpad entries: every row of the thread indexes it
-#pragma unroll
-    for (int e = 0; e < kCrtRows; ++e) ea[e] = EA[i0 + e];
-    int64_t idx[kCrtRows];
-    bool ok[kCrtRows];
     double v[kCrtRows];
 #pragma unroll
-    for (int e = 0; e < kCrtRows; ++e) {
-      const int64_t i = i0 + e;
-      ok[e] = i < m;
-      idx[e] = i * rsC + j * csC;
-      if (ok[e] && (ea[e] == kNonFinite || ej == kNonFinite)) {
-        double s = 0.0;
-        for (int64_t p = 0; p < k; ++p) s = __builtin_fma(A[i * rsA + p * csA], B[p * rsB + j * csB], s);
-        v[e] = alpha * s;
-      } else {
-        v[e] = alpha * ldexp(crt_i128_to_f64(crt_sums_value(a[e], K)), ea[e] + ej - 2 * beta_bits);
+    for (int e = 0; e < kCrtRows; ++e) v[e] = 0.0;
+    if (active) {
+      CrtSums a[kCrtRows];
+#pragma unroll
+      for (int e = 0; e < kCrtRows; ++e) crt_sums_init(a[e]);
+#pragma unroll
+      for (int l = 0; l < N; ++l) {
+        const uint4 w = *reinterpret_cast<const uint4*>(RC + (int64_t)l * mpad * npad + j * mpad + i0);
+        const uint32_t h[4] = {w.x, w.y, w.z, w.w};
+#pragma unroll
+        for (int e = 0; e < kCrtRows; ++e) {
+          const int16_t r = (int16_t)(h[e / 2] >> (16 * (e % 2)));
+          crt_sums_add(a[e], (float)r, l, K);
+        }
+      }
+      const int ej = EB[j];
+#pragma unroll
+      for (int e = 0; e < kCrtRows; ++e) {
+        const int64_t i = i0 + e;
+        const int ei = EA[i];  // EA holds mpad entries
+        if (i < m && (ei == kNonFinite || ej == kNonFinite)) {
+          double s = 0.0;
+          for (int64_t p = 0; p < k; ++p) s = __builtin_fma(A[i * rsA + p * csA], B[p * rsB + j * csB], s);
+          v[e] = alpha * s;
+        } else {
+          v[e] = alpha * ldexp(crt_i128_to_f64(crt_sums_value(a[e], K)), ei + ej - 2 * beta_bits);
+        }
       }
     }
-    update_c(C, idx, ok, v, beta);
+    // row rb + q sits at vt[(q / 8) * 9 + q % 8] (the pad keeps the stores of
+    // a wavefront on distinct banks)
+#pragma unroll
+    for (int e = 0; e < kCrtRows; ++e) vt[tid * (kCrtRows + 1) + e] = v[e];
+    __syncthreads();
+    int64_t idx[kCrtRows];
+    bool ok[kCrtRows];
+    double w[kCrtRows];
+#pragma unroll
+    for (int e = 0; e < kCrtRows; ++e) {
+      const int q = tid + 256 * e;
+      const int64_t i = rb + q;
+      ok[e] = i < m;
+      idx[e] = i * rsC + j * csC;
+      w[e] = vt[(q / kCrtRows) * (kCrtRows + 1) + q % kCrtRows];
+    }
+    __syncthreads();  // vt free for the next column
+    update_c(C, idx, ok, w, beta);
   }
 }
 

@@ -127,14 +127,31 @@ pack_bf16_panels(const uint16_t* __restrict__ X, int64_t count, int64_t len, int
   const int tid = threadIdx.x;
   const int64_t v0 = (int64_t)blockIdx.x * 64, p0 = (int64_t)blockIdx.y * 64;
   const bool v_unit = (sv == 1);
+  if (count > 0 && len > 0) {  // (the same for the whole grid: X has an element 0)
+    // all 16 loads first (X[0] for elements outside X, masked after), then the
+    // LDS stores, so the loads of a thread are in flight together
+    uint16_t x[16];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int e = tid + 256 * r;
-    const int a = e & 63, b = e >> 6;  // a runs fastest
-    const int vv = v_unit ? a : b;
-    const int pp = v_unit ? b : a;
-    const int64_t gv = v0 + vv, gp = p0 + pp;
-    tile[vv][pp] = (gv < count && gp < len) ? X[gv * sv + gp * sp] : (uint16_t)0;
+    for (int r = 0; r < 16; ++r) {
+      const int e = tid + 256 * r;
+      const int a = e & 63, b = e >> 6;  // a runs fastest
+      const int64_t gv = v0 + (v_unit ? a : b), gp = p0 + (v_unit ? b : a);
+      x[r] = X[(gv < count && gp < len) ? gv * sv + gp * sp : 0];
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int e = tid + 256 * r;
+      const int a = e & 63, b = e >> 6;
+      const int vv = v_unit ? a : b, pp = v_unit ? b : a;
+      tile[vv][pp] = (v0 + vv < count && p0 + pp < len) ? x[r] : (uint16_t)0;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int e = tid + 256 * r;
+      const int a = e & 63, b = e >> 6;
+      tile[v_unit ? a : b][v_unit ? b : a] = (uint16_t)0;
+    }
   }
   __syncthreads();
   const int64_t nslabs = lpad / kSlab;

@@ -37,17 +37,32 @@ __device__ __forceinline__ void slice_block(const double* __restrict__ X, int64_
   __shared__ double tile[64][65];
   const int tid = threadIdx.x;
   const bool v_unit = (sv == 1);
-  // load: 64 x 64 doubles, 16 per thread
+    // load: 64 x 64 doubles, 16 per thread
+  if (count > 0 && len > 0) {  // (the same for the whole grid: X has an element 0)
+    // all 16 loads first (X[0] for elements outside X, masked after), then the
+    // LDS stores, so the loads of a thread are in flight together
+    double x[16];
 #pragma unroll
-  for (int r = 0; r < 16; ++r) {
-    const int e = tid + 256 * r;
-    const int a = e & 63, b = e >> 6;  // a runs fastest
-    const int vv = v_unit ? a : b;
-    const int pp = v_unit ? b : a;
-    const int64_t gv = v0 + vv, gp = p0 + pp;
-    double x = 0.0;
-    if (gv < count && gp < len) x = X[gv * sv + gp * sp];
-    tile[vv][pp] = x;
+    for (int r = 0; r < 16; ++r) {
+      const int e = tid + 256 * r;
+      const int a = e & 63, b = e >> 6;  // a runs fastest
+      const int64_t gv = v0 + (v_unit ? a : b), gp = p0 + (v_unit ? b : a);
+      x[r] = X[(gv < count && gp < len) ? gv * sv + gp * sp : 0];
+    }
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int e = tid + 256 * r;
+      const int a = e & 63, b = e >> 6;
+      const int vv = v_unit ? a : b, pp = v_unit ? b : a;
+      tile[vv][pp] = (v0 + vv < count && p0 + pp < len) ? x[r] : 0.0;
+    }
+  } else {
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int e = tid + 256 * r;
+      const int a = e & 63, b = e >> 6;
+      tile[v_unit ? a : b][v_unit ? b : a] = 0.0;
+    }
   }
   __syncthreads();
   const int64_t nslabs = lpad / 16;
