@@ -128,9 +128,16 @@ batched_exponents(const momms_gemm_desc* __restrict__ d, const SplitProb* __rest
   if (v < count) {
     const double* x = t.kind == 0 ? A + p.a_off + v : B + p.b_off + v * p.ldb;
     const int64_t se = t.kind == 0 ? p.lda : 1;
-    for (int64_t e = q; e < p.k; e += 4) {
-      const uint64_t b = (uint64_t)__double_as_longlong(x[e * se]) & 0x7fffffffffffffffull;
-      best = b > best ? b : best;
+    // eight loads in flight per step (indices past k read element k-1 again)
+    for (int64_t e0 = q; e0 < p.k; e0 += 32) {
+      uint64_t b[8];
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int64_t e = e0 + 4 * u;
+        b[u] = (uint64_t)__double_as_longlong(x[(e < p.k ? e : p.k - 1) * se]) & 0x7fffffffffffffffull;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) best = b[u] > best ? b[u] : best;
     }
   }
   part[q][vv] = best;

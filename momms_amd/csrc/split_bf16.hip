@@ -104,9 +104,16 @@ __global__ void absmax_strided(const double* __restrict__ X, int64_t count, int6
   const int64_t p0 = (int64_t)blockIdx.y * per_block;
   const int64_t p1 = p0 + per_block < len ? p0 + per_block : len;
   uint64_t best = 0;
-  for (int64_t p = p0; p < p1; ++p) {
-    const uint64_t b = (uint64_t)__double_as_longlong(X[v * stride_vec + p * stride_elem]) & 0x7fffffffffffffffull;
-    best = b > best ? b : best;
+  // eight loads in flight per step (indices past p1 read element p1-1 again)
+  for (int64_t pb = p0; pb < p1; pb += 8) {
+    uint64_t b[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int64_t p = pb + u < p1 ? pb + u : p1 - 1;
+      b[u] = (uint64_t)__double_as_longlong(X[v * stride_vec + p * stride_elem]) & 0x7fffffffffffffffull;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) best = b[u] > best ? b[u] : best;
   }
   if (p1 > p0) atomicMax((unsigned long long*)&mx[v], (unsigned long long)best);
 }
@@ -119,9 +126,16 @@ __global__ void absmax_contig(const double* __restrict__ X, int64_t count, int64
   if (v >= count) return;
   const double* x = X + v * stride_vec;
   uint64_t best = 0;
-  for (int64_t p = lane; p < len; p += 64) {
-    const uint64_t b = (uint64_t)__double_as_longlong(x[p]) & 0x7fffffffffffffffull;
-    best = b > best ? b : best;
+  // eight loads in flight per step (indices past len read element len-1 again)
+  for (int64_t pb = lane; pb < len; pb += 512) {
+    uint64_t b[8];
+#pragma unroll
+    for (int u = 0; u < 8; ++u) {
+      const int64_t p = pb + 64 * u < len ? pb + 64 * u : len - 1;
+      b[u] = (uint64_t)__double_as_longlong(x[p]) & 0x7fffffffffffffffull;
+    }
+#pragma unroll
+    for (int u = 0; u < 8; ++u) best = b[u] > best ? b[u] : best;
   }
   for (int off = 32; off > 0; off >>= 1) {
     const uint64_t o = __shfl_xor(best, off);
