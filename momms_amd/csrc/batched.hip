@@ -3,14 +3,16 @@
 // No reference counterpart: the batching addition of the engine. A batch is a
 // table of column-major problems (momms_gemm_desc: m, n, k, element offsets
 // into shared A/B/C arenas, leading dimensions). A plan, built once per table,
-//   - sorts every problem into a size bucket (by max(m, n)) and gives each
-//     bucket a tile shape (plain FP64) chosen by timing the candidates on the
-//     batch itself (the autotuner), or a slice count (split-BF16) chosen as
-//     the smallest one whose sampled error stays within 2x plain FP64's;
-//   - flattens the work into tile items, largest k first (longest job first
-//     keeps the tail of the queue short);
-//   - runs persistent workgroups that pull items from a device counter until
-//     the queue is empty: one workgroup per CU slot, no per-problem launch.
+//   - gives every problem a tile shape by its smaller side (plain FP64:
+//     128 x 128 from T2 up, 64 x 64 from T up, one-wavefront 32 x 32 below;
+//     (T, T2) chosen by timing the candidates on the batch itself, the
+//     autotuner) or a slice count (split-BF16: the smallest one whose sampled
+//     error stays within 2x plain FP64's);
+//   - places every problem whole on one of eight queues, one per XCD,
+//     balancing their work, and sorts each queue by k, largest first
+//     (longest job first keeps the tail of a queue short);
+//   - runs persistent workgroups that pull items from the queues' device
+//     counters until all are empty: no per-problem launch.
 // The whole sequence (counter reset by a kernel, slicing for the split mode,
 // the persistent multiplies) can be captured once into a hipGraph and then
 // replayed with one hipGraphLaunch.
@@ -79,34 +81,29 @@ batched_plain_kernel(const momms_gemm_desc* __restrict__ d, const BatchItem* __r
                      QueueBounds qb, const BatchItem* __restrict__ waves, int nwaves,
                      unsigned* counters, double alpha, const double* __restrict__ A,
                      const double* __restrict__ B, double beta, double* __restrict__ C) {
-  const int home = (int)(blockIdx.x % kBatchQueues);
-  for (int s = 0; s < kBatchQueues; ++s) {
-    const int q = (home + s) % kBatchQueues;
-    const int base = qb.off[q], count = qb.off[q + 1] - base;
-    run_queue(counters + q, count, [&](int it) {
-      const BatchItem w = items[base + uniform(it)];
-      const int prob = uniform(w.prob);
-      if (prob >= 0) {
-        const momms_gemm_desc p = d[prob];
-        batch_tile<64, 64, 3>(p.m, p.n, p.k, alpha, A + p.a_off, p.lda, B + p.b_off, p.ldb, beta,
-                        C + p.c_off, p.ldc, (int64_t)uniform(w.tm) * 64,
-                        (int64_t)uniform(w.tn) * 64);
-      } else {
-        // the same for all lanes of a wavefront
-        const int wi = uniform(4 * (-1 - prob) + (int)(threadIdx.x >> 6));
-        if (wi < nwaves) {
-          const BatchItem t = waves[wi];
-          const int tp = uniform(t.prob);
-          if (tp >= 0) {
-            const momms_gemm_desc p = d[tp];
-            batch_wave_tile32<4>(p.m, p.n, p.k, alpha, A + p.a_off, p.lda, B + p.b_off, p.ldb,
-                                 beta, C + p.c_off, p.ldc, (int64_t)uniform(t.tm) * 32,
-                                 (int64_t)uniform(t.tn) * 32);
-          }
+  run_queues(qb, counters, [&](int i) {
+    const BatchItem w = items[uniform(i)];
+    const int prob = uniform(w.prob);
+    if (prob >= 0) {
+      const momms_gemm_desc p = d[prob];
+      batch_tile<64, 64, 3>(p.m, p.n, p.k, alpha, A + p.a_off, p.lda, B + p.b_off, p.ldb, beta,
+                            C + p.c_off, p.ldc, (int64_t)uniform(w.tm) * 64,
+                            (int64_t)uniform(w.tn) * 64);
+    } else {
+      // the same for all lanes of a wavefront
+      const int wi = uniform(4 * (-1 - prob) + (int)(threadIdx.x >> 6));
+      if (wi < nwaves) {
+        const BatchItem t = waves[wi];
+        const int tp = uniform(t.prob);
+        if (tp >= 0) {
+          const momms_gemm_desc p = d[tp];
+          batch_wave_tile32<4>(p.m, p.n, p.k, alpha, A + p.a_off, p.lda, B + p.b_off, p.ldb,
+                               beta, C + p.c_off, p.ldc, (int64_t)uniform(t.tm) * 32,
+                               (int64_t)uniform(t.tn) * 32);
         }
       }
-    });
-  }
+    }
+  });
 }
 
 // Exponents of 64 vectors per task: 4 threads per vector share its k range.

@@ -296,123 +296,6 @@ __device__ __forceinline__ void dgemm_tile_deep(int64_t m, int64_t n, int64_t k,
   update_c(C, idx, ok, v, beta);
 }
 
-// One 32 x 32 tile of C := beta*C + alpha*A*B (column-major A, B, C) computed
-// by a single wavefront, its operands loaded straight from global memory into
-// the MFMA operand registers (no LDS, no workgroup barrier), so small
-// problems run many tiles per CU at once. k goes in chunks of 16; inside a
-// chunk, lane group q = l/16 takes k0 + 4q .. k0 + 4q + 3 over the four MFMA
-// steps (the MFMA sums over its four lane groups, so any fixed assignment of
-// k to (group, step) is the same product). A lane's four B values of a chunk
-// are then four consecutive doubles of one column, and its A values one per
-// step along a row; every element's order of summation is fixed.
-// Out-of-range rows, columns and k-values load element 0 of the operand and
-// use 0 in its place.
-__device__ __forceinline__ void dgemm_wave_tile32(int64_t m, int64_t n, int64_t k, double alpha,
-                                                  const double* __restrict__ A, int64_t lda,
-                                                  const double* __restrict__ B, int64_t ldb,
-                                                  double beta, double* __restrict__ C,
-                                                  int64_t ldc, int64_t row0, int64_t col0) {
-  const int lane = threadIdx.x & 63, lr = lane & 15, lq = lane >> 4;
-  // per-lane element pointers at k = 4*lq of rows row0+lr, +16 and columns
-  // col0+lr, +16, advanced by one chunk (16 k) per step. They run past the
-  // problem at its last chunks, so they are dereferenced only at elements
-  // inside it: every other element is read from A[0] / B[0] (present, as a
-  // tile with k > 0 loads) and masked when used.
-  const bool i0 = row0 + lr < m, i1 = row0 + 16 + lr < m;
-  const bool j0 = col0 + lr < n, j1 = col0 + 16 + lr < n;
-  const double* pa0 = A + (i0 ? row0 + lr + 4 * lq * lda : 0);
-  const double* pa1 = A + (i1 ? row0 + 16 + lr + 4 * lq * lda : 0);
-  const double* pb0 = B + (j0 ? 4 * lq + (col0 + lr) * ldb : 0);
-  const double* pb1 = B + (j1 ? 4 * lq + (col0 + 16 + lr) * ldb : 0);
-  const int la = (int)lda;  // < 2^24 (checked by the plan)
-  const int64_t astep = i0 || i1 ? 16 * lda : 0;
-  v4f64 acc[2][2];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b) acc[a][b] = v4f64{0.0, 0.0, 0.0, 0.0};
-
-  // raw loads of one chunk; the out-of-range mask is applied when the values
-  // are used, so the loads of the next chunk stay in flight across the MFMAs
-  // of this one
-  auto load = [&](int kleft, double (&xa)[2][4], double (&xb)[2][4]) {
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const bool kok = s < kleft;
-      xa[0][s] = *((i0 && kok) ? pa0 + s * la : A);
-      xa[1][s] = *((i1 && kok) ? pa1 + s * la : A);
-      xb[0][s] = *((j0 && kok) ? pb0 + s : B);
-      xb[1][s] = *((j1 && kok) ? pb1 + s : B);
-    }
-    if (i0) pa0 += astep;
-    if (i1) pa1 += astep;
-    if (j0) pb0 += 16;
-    if (j1) pb1 += 16;
-  };
-
-  auto mma = [&](int kv, const double (&xa)[2][4], const double (&xb)[2][4]) {
-#pragma unroll
-    for (int s = 0; s < 4; ++s) {
-      const bool kok = s < kv;
-      const double a0 = (i0 && kok) ? xa[0][s] : 0.0, a1 = (i1 && kok) ? xa[1][s] : 0.0;
-      const double b0 = (j0 && kok) ? xb[0][s] : 0.0, b1 = (j1 && kok) ? xb[1][s] : 0.0;
-      acc[0][0] = mfma_f64_16x16x4(b0, a0, acc[0][0]);
-      acc[0][1] = mfma_f64_16x16x4(b1, a0, acc[0][1]);
-      acc[1][0] = mfma_f64_16x16x4(b0, a1, acc[1][0]);
-      acc[1][1] = mfma_f64_16x16x4(b1, a1, acc[1][1]);
-    }
-  };
-  // WD register sets in turn: the loads of the next WD-1 chunks are in
-  // flight while one chunk is multiplied (a third set would cost the kernel
-  // its second wavefront per SIMD)
-  constexpr int WD = 2;
-  const int64_t nchunks = ceil_div(k, 16);
-  const int64_t k0 = k - 4 * lq;  // k-values from this lane group's first
-  auto kv = [&](int64_t c) {
-    const int64_t r = k0 - 16 * c;
-    return (int)(r < 0 ? 0 : r < 4 ? r : 4);
-  };
-  double xa[WD][2][4], xb[WD][2][4];
-  if (nchunks > 0) {
-    // the loads are issued unconditionally (past the end they are masked
-    // reads of A[0] / B[0]) and whole groups of WD chunks run with no branch
-    // inside, so the wait before each multiply covers only its own set
-#pragma unroll
-    for (int d = 0; d < WD; ++d) load(kv(d), xa[d], xb[d]);
-    int64_t c = 0;
-#pragma unroll 1
-    for (; c + WD <= nchunks; c += WD) {
-#pragma unroll
-      for (int d = 0; d < WD; ++d) {
-        mma(kv(c + d), xa[d], xb[d]);
-        __builtin_amdgcn_sched_barrier(0);
-        load(kv(c + d + WD), xa[d], xb[d]);
-        __builtin_amdgcn_sched_barrier(0);
-      }
-    }
-#pragma unroll
-    for (int d = 0; d + 1 < WD; ++d)
-      if (c + d < nchunks) mma(kv(c + d), xa[d], xb[d]);
-  }
-  // lane l holds C[i][j], i = l%16, j = l/16 + 4r of each 16x16 block
-  int64_t idx[16];
-  bool ok[16];
-  double v[16];
-#pragma unroll
-  for (int a = 0; a < 2; ++a)
-#pragma unroll
-    for (int b = 0; b < 2; ++b)
-#pragma unroll
-      for (int r = 0; r < 4; ++r) {
-        const int t = (a * 2 + b) * 4 + r;
-        const int64_t j = col0 + 16 * b + lq + 4 * r;
-        idx[t] = row0 + 16 * a + lr + j * ldc;
-        ok[t] = (a ? i1 : i0) && j < n;
-        v[t] = alpha * acc[a][b][r];
-      }
-  update_c(C, idx, ok, v, beta);
-}
-
 }  // namespace momms
 
 namespace momms {
