@@ -17,10 +17,11 @@ moduli, N residue GEMMs). Prints ONE JSON line:
   crt, split   both forms: count, rate, fraction of the FP64 peak, errors
   plain        the FP64-MFMA GEMM: TFLOP/s, fraction of the FP64 peak, errors
   configs      the other configurations' lines (sweep, slices/moduli x
-               exponent spread, FP32/BF16, batched) unless --quick
+               exponent spread, FP32/BF16, batched, hierarchical layout
+               pack / unpack-add) unless --quick
   cpu_baseline the restated Goto + 6x8 CPU path on this host's cores
 
---only {plain,split,multiply,crt,batched,batched_plain,copy} runs one kernel family in a loop
+--only {plain,split,multiply,crt,batched,batched_plain,copy,hier} runs one kernel family in a loop
 (for profilers) and prints nothing else but its timing line.
 """
 from __future__ import annotations
@@ -224,6 +225,34 @@ def batched(capi, torch, gen, steps):
     return res
 
 
+def hier(capi, torch, gen, steps, n=8192):
+    """Device-resident hierarchical matrices at n x n: pack a column-major
+    matrix into the Goto A (MR=4, MC=120, KC=192) and Goto C (MR=4, NR=12,
+    MC=120, NC=3000) buffers of the reference's l3 experiment and add the C
+    buffer back; algorithmic bytes = reads + writes of each element once
+    (padding places written as zeros)."""
+    M, N, K = capi.STEP_M, capi.STEP_N, capi.STEP_K
+    goto = [(M, 4), (N, 12), (M, 120), (K, 192), (N, 3000)]
+    out = {}
+    X = rand_colmajor(n, n, gen)
+    for name, (yd, xd, lrs, lcs) in {"goto_a": (M, K, 1, 4), "goto_c": (M, N, 12, 1)}.items():
+        L = capi.hier_layout(n, n, goto, yd, xd, 4, 12, lrs, lcs)
+        P = torch.empty(L.capacity, dtype=torch.float64, device="cuda")
+        ms = time_loop(lambda: capi.hier_pack(L, X, P), steps, 3)
+        b = 8.0 * (n * n + L.capacity)
+        out[f"pack_{name}"] = {"ms": round(ms, 4), "bytes": b, "GBps": round(b / ms / 1e6, 1),
+                               "frac_hbm": round(b / ms / 1e6 / PEAK_HBM, 4),
+                               "capacity": L.capacity}
+        if name == "goto_c":
+            ms = time_loop(lambda: capi.hier_unpack_add(L, P, X), steps, 3)
+            b = 8.0 * 3 * n * n
+            out[f"unpack_add_{name}"] = {"ms": round(ms, 4), "bytes": b,
+                                         "GBps": round(b / ms / 1e6, 1),
+                                         "frac_hbm": round(b / ms / 1e6 / PEAK_HBM, 4)}
+        del P
+    return out
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -238,7 +267,7 @@ def main() -> int:
     ap.add_argument("--tile", type=int, default=None, help="FP64 tile configuration (--only plain)")
     ap.add_argument("--moduli", type=int, default=14, help="moduli of the residue form (--only crt)")
     ap.add_argument("--only", choices=["plain", "split", "multiply", "crt", "batched",
-                                       "batched_plain", "copy"])
+                                       "batched_plain", "copy", "hier"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported. This engine "
@@ -288,6 +317,10 @@ def main() -> int:
                        args.steps, args.warmup)
         print(json.dumps({"only": "crt", "moduli": args.moduli, "ms": ms,
                           "tflops": flops / ms / 1e9}))
+        return 0
+    if args.only == "hier":
+        del A, B, C0, C
+        print(json.dumps({"only": "hier", **hier(capi, torch, gen, args.steps, n)}))
         return 0
     if args.only == "batched":
         del A, B, C0, C
@@ -453,6 +486,8 @@ def main() -> int:
         cfg["reduced_precision"] = reduced_precision(capi, torch, gen, 3)
         log("bench: config 5 (4096 uneven GEMMs)")
         cfg["batched"] = batched(capi, torch, gen, 5)
+        log("bench: hierarchical layout pack / unpack-add at 8192^2")
+        cfg["hier_layout"] = hier(capi, torch, gen, 10)
         result["configs"] = cfg
         torch.cuda.empty_cache()
 

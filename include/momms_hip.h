@@ -211,6 +211,67 @@ int momms_hip_dgemm_batched_loop(const momms_gemm_desc* descs, int64_t count,
                                  const double* alpha, const double* A, const double* B,
                                  const double* beta, double* C, void* stream);
 
+/* ---- hierarchical (multi-level blocked) matrices --------------------------------
+ * The device-resident counterpart of momms' Hierarch<T, LH, LW, LRS, LCS>
+ * (src/matrix/hierarch.rs:18-184): a matrix stored as nested blocks whose
+ * sizes come from a control tree's hierarchy description (the
+ * AlgorithmStep list GemmNode::hierarchy_description returns, innermost
+ * first, src/composables/mod.rs:24-36), with LH x LW leaves of strides
+ * LRS, LCS. The offset of element (y, x) is the sum of a y part and an x
+ * part (get_offset_y/_x, hierarch.rs:153-184): walking the levels outermost
+ * first, b = y / y_blksz[i] adds b * y_stride[i] and leaves y - b * y_blksz[i]
+ * for the next level; what is left at the leaf adds y * lrs; likewise x with
+ * x_blksz, x_stride and lcs. The size is padded to whole top-level blocks. */
+#define MOMMS_HIER_MAX_LEVELS 8
+enum { MOMMS_STEP_M = 0, MOMMS_STEP_N = 1, MOMMS_STEP_K = 2 };
+
+/* AlgorithmStep::{M, N, K}{bsz} (src/composables/mod.rs:25-29). */
+typedef struct momms_algo_step {
+  int32_t kind; /* MOMMS_STEP_M, _N or _K */
+  int32_t pad;
+  int64_t bsz;
+} momms_algo_step;
+
+typedef struct momms_hier_layout {
+  int64_t h, w;         /* logical height and width */
+  int64_t h_pad, w_pad; /* padded to whole top-level blocks */
+  int64_t ny, nx;       /* block levels of y and x, outermost first */
+  int64_t capacity;     /* elements: h_pad * w_pad */
+  int64_t lrs, lcs;     /* leaf row and column stride */
+  int64_t y_blksz[MOMMS_HIER_MAX_LEVELS], y_stride[MOMMS_HIER_MAX_LEVELS];
+  int64_t x_blksz[MOMMS_HIER_MAX_LEVELS], x_stride[MOMMS_HIER_MAX_LEVELS];
+} momms_hier_layout;
+
+/* Hierarch::new (hierarch.rs:90-140): the layout of an h x w matrix whose
+ * rows follow steps of kind ydim and columns steps of kind xdim; lh, lw give
+ * the top-level block when no step of that kind exists. Host only. Returns
+ * -1 for a bad argument (non-positive block size or negative leaf stride),
+ * -2 for more than MOMMS_HIER_MAX_LEVELS levels in one dimension, -3 for a
+ * layout in which two elements would share a place: a block that does not
+ * split evenly into the next level's, or a leaf (innermost y block x
+ * innermost x block) not stored densely by columns (lrs 1, lcs its height)
+ * or by rows (lcs 1, lrs its width). The reference accepts the last kind
+ * and overlaps elements. */
+int momms_hier_layout_init(momms_hier_layout* L, int64_t h, int64_t w, const momms_algo_step* steps,
+                           int64_t nsteps, int32_t ydim, int32_t xdim, int64_t lh, int64_t lw,
+                           int64_t lrs, int64_t lcs);
+
+/* Packer<Matrix, Hierarch>::pack (src/composables/pack.rs:153-309): P (device,
+ * L->capacity doubles) := A (h x w, strides rsA, csA) in layout L, padding
+ * zero (also whole padding leaves, which the reference leaves unwritten).
+ * L is one momms_hier_layout_init filled; the entry points check its fields
+ * for consistency, and a place outside [0, capacity) of a hand-made layout is
+ * neither written here nor read by momms_hip_hier_unpack_add (it unpacks as
+ * 0). Padded width at most 524280. */
+int momms_hip_hier_pack(const momms_hier_layout* L, const double* A, int64_t rsA, int64_t csA,
+                        double* P, void* stream);
+
+/* Unpacker<Matrix, Hierarch>::add (src/composables/unpack.rs:61-98, :174-213),
+ * the last step of UnpackC (unpack.rs:214-270): C[y][x] += P[offset(y, x)]
+ * for every element of the h x w matrix. */
+int momms_hip_hier_unpack_add(const momms_hier_layout* L, const double* P, double* C,
+                              int64_t rsC, int64_t csC, void* stream);
+
 /* ---- accuracy statement -----------------------------------------------------
  * Recomputes `samples` entries of beta*C0 + alpha*A*B in double-double on the
  * GPU (Dot2, Ogita-Rump-Oishi 2005) and returns, on the host, the largest

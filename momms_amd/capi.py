@@ -78,6 +78,10 @@ _SIGS: dict[str, list] = {
                                    _c_i64, _c_int, _c_vp, ctypes.c_size_t, _c_vp],
     "momms_hip_crt_bf16_products": [_c_i64, _c_i64, _c_i64, _c_int, _c_vp, ctypes.c_size_t,
                                     _c_vp],
+    "momms_hier_layout_init": [_c_vp, _c_i64, _c_i64, _c_vp, _c_i64, ctypes.c_int32,
+                               ctypes.c_int32, _c_i64, _c_i64, _c_i64, _c_i64],
+    "momms_hip_hier_pack": [_c_vp, _c_vp, _c_i64, _c_i64, _c_vp, _c_vp],
+    "momms_hip_hier_unpack_add": [_c_vp, _c_vp, _c_vp, _c_i64, _c_i64, _c_vp],
     "momms_hip_crt_bf16_reconstruct": [_c_i64, _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64,
                                        _c_vp, _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64,
                                        _c_int, _c_vp, ctypes.c_size_t, _c_vp],
@@ -450,6 +454,76 @@ def sampled_error(A, B, C0, C, alpha=1.0, beta=1.0, samples=65536, seed=0, ref_o
                                             stream_handle(stream))
     check("momms_hip_gemm_check_sampled", st)
     return {"max_scaled": ms.value, "max_rel": mr.value, "samples": samples}
+
+
+HIER_MAX_LEVELS = 8
+STEP_M, STEP_N, STEP_K = 0, 1, 2  # AlgorithmStep kinds
+
+
+class AlgoStep(ctypes.Structure):
+    """momms_algo_step: AlgorithmStep::{M, N, K}{bsz} (src/composables/mod.rs:25-29)."""
+
+    _fields_ = [("kind", ctypes.c_int32), ("pad", ctypes.c_int32), ("bsz", _c_i64)]
+
+
+class HierLayout(ctypes.Structure):
+    """momms_hier_layout: the shape of a Hierarch (src/matrix/hierarch.rs:18-184)."""
+
+    _fields_ = [("h", _c_i64), ("w", _c_i64), ("h_pad", _c_i64), ("w_pad", _c_i64),
+                ("ny", _c_i64), ("nx", _c_i64), ("capacity", _c_i64), ("lrs", _c_i64),
+                ("lcs", _c_i64), ("y_blksz", _c_i64 * HIER_MAX_LEVELS),
+                ("y_stride", _c_i64 * HIER_MAX_LEVELS), ("x_blksz", _c_i64 * HIER_MAX_LEVELS),
+                ("x_stride", _c_i64 * HIER_MAX_LEVELS)]
+
+    def as_list(self) -> list[int]:
+        """[h, w, h_pad, w_pad, ny, nx, capacity, lrs, lcs, y_blksz..., y_stride...,
+        x_blksz..., x_stride...] (levels beyond ny / nx are 0)."""
+        head = [self.h, self.w, self.h_pad, self.w_pad, self.ny, self.nx, self.capacity,
+                self.lrs, self.lcs]
+        return head + list(self.y_blksz) + list(self.y_stride) + list(self.x_blksz) + \
+            list(self.x_stride)
+
+
+def hier_layout(h: int, w: int, steps, ydim: int, xdim: int, lh: int, lw: int, lrs: int,
+                lcs: int) -> HierLayout:
+    """Hierarch::new (src/matrix/hierarch.rs:90-140) for an h x w matrix: rows
+    follow the steps of kind `ydim`, columns those of kind `xdim`; `steps` is a
+    control tree's hierarchy description, [(kind, bsz), ...] innermost first;
+    lh x lw leaves with strides lrs, lcs. Host only."""
+    arr = (AlgoStep * max(len(steps), 1))(*[AlgoStep(int(k), 0, int(b)) for k, b in steps])
+    L = HierLayout()
+    st = lib().momms_hier_layout_init(ctypes.byref(L), h, w, arr, len(steps), ydim, xdim, lh, lw,
+                                      lrs, lcs)
+    if st != 0:
+        why = {-2: "more than 8 levels in one dimension",
+               -3: "blocks do not nest or the leaf is not dense"}.get(st, "bad argument")
+        raise ValueError(f"momms_hier_layout_init returned {st}: {why}")
+    return L
+
+
+def hier_pack(L: HierLayout, A, P, stream=None):
+    """Packer<Matrix, Hierarch>::pack (src/composables/pack.rs:153-309): P (a
+    float64 GPU tensor of L.capacity elements) := A in layout L, padding 0."""
+    _f64((A, "A"), (P, "P"))
+    a, ra, ca = _mat(A, L.h, L.w, "A")
+    if P.numel() < L.capacity or not P.is_cuda or (P.numel() > 1 and not P.is_contiguous()):
+        raise ValueError(f"P must be a contiguous GPU tensor of >= {L.capacity} elements")
+    check("momms_hip_hier_pack",
+          lib().momms_hip_hier_pack(ctypes.byref(L), a, ra, ca, P.data_ptr(), stream_handle(stream)))
+    return P
+
+
+def hier_unpack_add(L: HierLayout, P, C, stream=None):
+    """Unpacker<Matrix, Hierarch>::add (src/composables/unpack.rs:174-213):
+    C[y][x] += P[offset(y, x)] over the h x w matrix C (in place)."""
+    _f64((P, "P"), (C, "C"))
+    c, rc, cc = _mat(C, L.h, L.w, "C")
+    if P.numel() < L.capacity or not P.is_cuda or (P.numel() > 1 and not P.is_contiguous()):
+        raise ValueError(f"P must be a contiguous GPU tensor of >= {L.capacity} elements")
+    check("momms_hip_hier_unpack_add",
+          lib().momms_hip_hier_unpack_add(ctypes.byref(L), P.data_ptr(), c, rc, cc,
+                                          stream_handle(stream)))
+    return C
 
 
 def sample_positions(seed: int, samples: int, m: int, n: int):

@@ -11,6 +11,7 @@
  *            oracle_residual       randomized residual check
  *   ozaki.c  oracle_ozaki_*        split-BF16 algorithm restatement (slices)
  *   crt.c    oracle_crt_*          split-BF16 algorithm restatement (residues)
+ *   hier.c   oracle_hier_*         hierarchical matrix layout, pack, unpack-add
  */
 #ifndef MOMMS_ORACLE_H
 #define MOMMS_ORACLE_H
@@ -60,5 +61,19 @@ void oracle_crt_integers(int64_t count, int64_t k, const double* X, int64_t sv, 
 int oracle_crt_gemm(int64_t m, int64_t n, int64_t k, const double* alpha, const double* A,
                     int64_t rsA, int64_t csA, const double* B, int64_t rsB, int64_t csB,
                     const double* betaC, double* C, int64_t rsC, int64_t csC, int N);
+
+/* hierarchical layout: a layout is int64[9 + 4 * ORACLE_HIER_LEVELS] =
+   [h, w, h_pad, w_pad, ny, nx, capacity, lrs, lcs, y_blksz[], y_stride[],
+    x_blksz[], x_stride[]] (levels outermost first) */
+#define ORACLE_HIER_LEVELS 8
+int oracle_hier_layout(int64_t h, int64_t w, const int32_t* kind, const int64_t* bsz,
+                       int64_t nsteps, int32_t ydim, int32_t xdim, int64_t lh, int64_t lw,
+                       int64_t lrs, int64_t lcs, int64_t* out);
+int64_t oracle_hier_offset(const int64_t* L, int64_t y, int64_t x);
+/* out[y * w_pad + x] = offset(y, x) for y < h_pad, x < w_pad */
+void oracle_hier_offsets(const int64_t* L, int64_t* out);
+void oracle_hier_pack(const int64_t* L, const double* A, int64_t rs, int64_t cs, double* P);
+void oracle_hier_unpack_add(const int64_t* L, const double* P, double* C, int64_t rs,
+                            int64_t cs);
 
 #endif

@@ -29,6 +29,7 @@ def lib() -> ctypes.CDLL:
         _lib = ctypes.CDLL(str(_PATH))
         _lib.oracle_residual.restype = ctypes.c_double
         _lib.oracle_ozaki_chunk.restype = ctypes.c_int
+        _lib.oracle_hier_offset.restype = ctypes.c_int64
     return _lib
 
 
@@ -209,4 +210,46 @@ def crt_gemm(A, B, C, N, alpha=1.0, beta=1.0):
                                ctypes.c_int(N))
     if st != 0:
         raise ValueError(f"oracle_crt_gemm: status {st} (too few moduli for k={k}?)")
+    return C
+
+
+HIER_LEVELS = 8
+M, N, K = 0, 1, 2  # hierarchy step kinds
+
+
+def hier_layout(h, w, steps, ydim, xdim, lh, lw, lrs, lcs) -> np.ndarray:
+    """Hierarch::new restated: steps = [(kind, bsz), ...] innermost first.
+    Returns the int64 layout vector (see oracle.h)."""
+    kind = np.array([s[0] for s in steps] or [0], dtype=np.int32)
+    bsz = np.array([s[1] for s in steps] or [1], dtype=np.int64)
+    out = np.zeros(9 + 4 * HIER_LEVELS, dtype=np.int64)
+    st = lib().oracle_hier_layout(_i64(h), _i64(w), _p(kind), _p(bsz), _i64(len(steps)),
+                                  ctypes.c_int32(ydim), ctypes.c_int32(xdim), _i64(lh), _i64(lw),
+                                  _i64(lrs), _i64(lcs), _p(out))
+    if st != 0:
+        raise ValueError(f"oracle_hier_layout: status {st}")
+    return out
+
+
+def hier_offset(L, y, x) -> int:
+    return int(lib().oracle_hier_offset(_p(L), _i64(y), _i64(x)))
+
+
+def hier_offsets(L) -> np.ndarray:
+    """offset(y, x) over the padded domain, [h_pad, w_pad]."""
+    out = np.empty((int(L[2]), int(L[3])), dtype=np.int64)
+    lib().oracle_hier_offsets(_p(L), _p(out))
+    return out
+
+
+def hier_pack(L, A) -> np.ndarray:
+    ra, ca = strides(A)
+    P = np.empty(int(L[6]), dtype=np.float64)
+    lib().oracle_hier_pack(_p(L), _p(A), _i64(ra), _i64(ca), _p(P))
+    return P
+
+
+def hier_unpack_add(L, P, C):
+    rc, cc = strides(C)
+    lib().oracle_hier_unpack_add(_p(L), _p(P), _p(C), _i64(rc), _i64(cc))
     return C

@@ -1,0 +1,97 @@
+"""Parity of the device-resident hierarchical layout (momms_hip_hier_pack,
+momms_hip_hier_unpack_add) with the oracle: bit-exact (pure data movement),
+on the reference's Goto / L3b layouts and random nested hierarchies, ragged
+and empty sizes, three storage orders; at 8192 x 8192 through the closed form
+of the Goto A buffer and a pack -> unpack round trip."""
+import random
+
+import numpy as np
+import pytest
+import torch
+
+import pyoracle as po
+from momms_amd import capi
+from test_gpu_dgemm import host, layout_view
+from test_hier import FORMS, GOTO, dense_leaf, goto_a, random_nested_steps
+
+pytestmark = pytest.mark.gpu
+M, K = po.M, po.K
+
+
+def oracle_layout(L: capi.HierLayout) -> np.ndarray:
+    return np.array(L.as_list(), dtype=np.int64)
+
+
+def check_pack_unpack(L, rng, layout):
+    A = layout_view(rng, L.h, L.w, layout)
+    P = torch.full((max(L.capacity, 1),), float("nan"), dtype=torch.float64, device="cuda")
+    capi.hier_pack(L, A, P)
+    C = layout_view(rng, L.h, L.w, layout)
+    C0 = host(C)
+    capi.hier_unpack_add(L, P, C)
+    torch.cuda.synchronize()
+    OL = oracle_layout(L)
+    want_p = po.hier_pack(OL, host(A))
+    np.testing.assert_array_equal(P.cpu().numpy()[:L.capacity], want_p)
+    want_c = po.hier_unpack_add(OL, want_p, C0.copy())
+    np.testing.assert_array_equal(host(C), want_c)
+
+
+@pytest.mark.parametrize("name", sorted(FORMS))
+@pytest.mark.parametrize("h,w", [(1, 1), (121, 193), (250, 130), (0, 9), (7, 0)])
+@pytest.mark.parametrize("layout", ["col", "row", "strided"])
+def test_pack_unpack_match_oracle(name, h, w, layout):
+    _, steps, yd, xd, lh, lw, lrs, lcs = FORMS[name]
+    L = capi.hier_layout(h, w, steps, yd, xd, lh, lw, lrs, lcs)
+    print(f"hier {name} {h}x{w} {layout}", flush=True)
+    check_pack_unpack(L, np.random.default_rng(h * 1000 + w), layout)
+
+
+@pytest.mark.parametrize("seed", range(10))
+def test_random_nested_layouts(seed):
+    rng = random.Random(100 + seed)
+    steps, _ = random_nested_steps(rng)
+    yd, xd = rng.sample([0, 1, 2], 2)
+    h, w = rng.randint(0, 600), rng.randint(0, 600)
+    hp = po.hier_layout(h, w, steps, yd, xd, 3, 5, 1, 1)
+    lrs, lcs = dense_leaf(rng, steps, yd, xd, int(hp[2]), int(hp[3]))
+    L = capi.hier_layout(h, w, steps, yd, xd, 3, 5, lrs, lcs)
+    print(f"hier random seed {seed}: {h}x{w} steps {steps}", flush=True)
+    check_pack_unpack(L, np.random.default_rng(seed), ["col", "row", "strided"][seed % 3])
+
+
+def test_goto_a_8192_closed_form_and_round_trip():
+    n = 8192
+    L = capi.hier_layout(n, n, GOTO, M, K, 4, 192, 1, 4)
+    print(f"hier goto_a {n}x{n}: capacity {L.capacity}", flush=True)
+    g = torch.Generator(device="cuda").manual_seed(7)
+    A = torch.empty((n, n), dtype=torch.float64, device="cuda").T  # column-major
+    A.uniform_(-1, 1, generator=g)
+    P = torch.full((L.capacity,), float("nan"), dtype=torch.float64, device="cuda")
+    capi.hier_pack(L, A, P)
+    # every element at its closed-form place (GotoA, computed on the GPU)
+    y = torch.arange(L.h_pad, device="cuda").view(-1, 1)
+    x = torch.arange(L.w_pad, device="cuda").view(1, -1)
+    off = (x // 192) * L.h_pad * 192 + (y // 120) * 192 * 120 + ((y % 120) // 4) * 192 * 4 \
+        + (y % 4) + (x % 192) * 4
+    assert torch.equal(P[off[:n, :n]], A)
+    pad = torch.ones((L.h_pad, L.w_pad), dtype=torch.bool, device="cuda")
+    pad[:n, :n] = False
+    assert int(pad.sum()) == L.capacity - n * n
+    assert bool((P[off[pad]] == 0).all())
+    # pack -> unpack into zeros gives A back; unpacking twice adds it twice
+    C = torch.zeros((n, n), dtype=torch.float64, device="cuda").T
+    capi.hier_unpack_add(L, P, C)
+    assert torch.equal(C, A)
+    capi.hier_unpack_add(L, P, C)
+    assert torch.equal(C, 2 * A)
+
+
+def test_pack_small_matches_closed_form_on_a_row_major_source():
+    L = capi.hier_layout(300, 500, GOTO, M, K, 4, 192, 1, 4)
+    A = torch.randn((300, 500), dtype=torch.float64, device="cuda")
+    P = torch.empty((L.capacity,), dtype=torch.float64, device="cuda")
+    capi.hier_pack(L, A, P)
+    hp, wp, off = goto_a(300, 500)
+    Pn = P.cpu().numpy()
+    np.testing.assert_array_equal(Pn[off[:300, :500]], A.cpu().numpy())
