@@ -262,7 +262,7 @@ int momms_hier_layout_init(momms_hier_layout* L, int64_t h, int64_t w, const mom
  * L is one momms_hier_layout_init filled; the entry points check its fields
  * for consistency, and a place outside [0, capacity) of a hand-made layout is
  * neither written here nor read by momms_hip_hier_unpack_add (it unpacks as
- * 0). Padded width at most 524280. */
+ * 0). */
 int momms_hip_hier_pack(const momms_hier_layout* L, const double* A, int64_t rsA, int64_t csA,
                         double* P, void* stream);
 

@@ -10,10 +10,9 @@
 // column instead of per element), kept in the library's per-stream scratch.
 // A place outside [0, capacity) (a hand-made layout; never one that
 // momms_hier_layout_init returns) is neither written nor read: it unpacks as 0.
-// A workgroup takes 256 consecutive rows of 8 consecutive columns, so the
-// strided side is read or written along its unit stride when that is the
-// row stride (the column-major case) and the layout side stays within the
-// leaves those rows share.
+// A workgroup moves 256 consecutive indices along the strided matrix's unit
+// stride times G consecutive indices across it through LDS, so that both the
+// strided side and the layout side are accessed in long contiguous runs.
 #include <hip/hip_runtime.h>
 
 #include "mfma.hpp"
@@ -22,8 +21,6 @@
 #include "status.hpp"
 
 namespace momms {
-
-constexpr int kHierCols = 8;  // columns per workgroup
 
 // offset tables: t[i] for i < h_pad is the y part of row i, t[h_pad + j] the x
 // part of column j (hierarch.rs:153-184)
@@ -45,61 +42,168 @@ hier_tables(momms_hier_layout L, int64_t* __restrict__ t) {
   t[i] = off + v * (is_y ? L.lrs : L.lcs);
 }
 
+// A workgroup moves a tile of 256 lane indices u (rows when LX is false,
+// columns when it is true) by G minor indices v through LDS:
+//  - on the strided side, lane u holds u and steps through the G minor
+//    indices, so every access runs along the matrix's unit stride;
+//  - on the layout side, element e = q*256 + tid of the tile is taken in
+//    layout order: runs of R consecutive lane indices (the leaf's contiguous
+//    run along u, a power of two dividing 256; 1 when the leaf runs along v)
+//    by the G minor indices, so a wave touches 64 consecutive places.
+// Placement is from the offset tables in either order; the order only decides
+// which places one wave instruction touches.
+template <int G>
+struct HierTile {
+  static constexpr int kStride = 256 + 4;  // conflict-free reads in layout order
+  double v[G * kStride];
+  int64_t ou[256];
+  int64_t ov[G];
+};
+
+template <int G>
+__device__ __forceinline__ void layout_order(int q, int log2r, int& uu, int& c) {
+  const uint32_t e = (uint32_t)q * 256 + threadIdx.x;
+  const uint32_t i = (e >> log2r) / G;  // group of R lane indices x G
+  const uint32_t r = e - i * (G << log2r);
+  uu = (int)((i << log2r) + (r & ((1u << log2r) - 1)));
+  c = (int)(r >> log2r);
+}
+
 // h == 0 or w == 0 (grid-uniform): nothing is read and every place is padding
+template <int G, bool LX>
 __global__ void __launch_bounds__(256)
-hier_pack_kernel(int64_t h, int64_t w, int64_t h_pad, int64_t w_pad, uint64_t cap,
-                 const int64_t* __restrict__ oy, const int64_t* __restrict__ ox,
+hier_pack_kernel(int64_t h, int64_t w, int64_t h_pad, int64_t w_pad, uint64_t cap, uint32_t nbu,
+                 int log2r, const int64_t* __restrict__ oy, const int64_t* __restrict__ ox,
                  const double* __restrict__ A, int64_t rs, int64_t cs, double* __restrict__ P) {
-  const int64_t y = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (y >= h_pad) return;
-  const int64_t x0 = (int64_t)blockIdx.y * kHierCols;
-  const int64_t py = oy[y];
-  double v[kHierCols];
+  __shared__ HierTile<G> T;
+  const uint32_t bu = blockIdx.x % nbu, bv = blockIdx.x / nbu;
+  const int64_t u0 = (int64_t)bu * 256, v0 = (int64_t)bv * G;
+  const int64_t u = u0 + threadIdx.x;
+  const int64_t U = LX ? w_pad : h_pad, V = LX ? h_pad : w_pad;
+  const int64_t hu = LX ? w : h, hv = LX ? h : w;  // logical extents
+  const int64_t su = LX ? cs : rs, sv = LX ? rs : cs;
+  const int64_t* ou = LX ? ox : oy;
+  const int64_t* ov = LX ? oy : ox;
+  double v[G];
   if (h > 0 && w > 0) {
-    // all loads first (A[0] stands in for padding and is replaced by 0),
-    // then the stores
-    const int64_t yc = y < h ? y : h - 1;
+    // all loads first (clamped into the matrix; padding is replaced by 0)
+    const int64_t uc = u < hu ? u : hu - 1;
 #pragma unroll
-    for (int c = 0; c < kHierCols; ++c) {
-      const int64_t x = x0 + c < w ? x0 + c : w - 1;
-      v[c] = A[yc * rs + x * cs];
+    for (int c = 0; c < G; ++c) {
+      const int64_t j = v0 + c < hv ? v0 + c : hv - 1;
+      v[c] = A[uc * su + j * sv];
     }
 #pragma unroll
-    for (int c = 0; c < kHierCols; ++c)
-      if (y >= h || x0 + c >= w) v[c] = 0.0;
+    for (int c = 0; c < G; ++c)
+      if (u >= hu || v0 + c >= hv) v[c] = 0.0;
   } else {
 #pragma unroll
-    for (int c = 0; c < kHierCols; ++c) v[c] = 0.0;
+    for (int c = 0; c < G; ++c) v[c] = 0.0;
   }
+  T.ou[threadIdx.x] = ou[u < U ? u : U - 1];
+  if (threadIdx.x < G) T.ov[threadIdx.x] = ov[v0 + threadIdx.x < V ? v0 + threadIdx.x : V - 1];
 #pragma unroll
-  for (int c = 0; c < kHierCols; ++c) {
-    if (x0 + c >= w_pad) break;
-    const int64_t o = py + ox[x0 + c];
-    if ((uint64_t)o < cap) P[o] = v[c];
+  for (int c = 0; c < G; ++c) T.v[c * HierTile<G>::kStride + threadIdx.x] = v[c];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < G; ++q) {
+    int uu, c;
+    layout_order<G>(q, log2r, uu, c);
+    const int64_t o = T.ou[uu] + T.ov[c];
+    if (u0 + uu < U && v0 + c < V && (uint64_t)o < cap) P[o] = T.v[c * HierTile<G>::kStride + uu];
   }
 }
 
+template <int G, bool LX>
 __global__ void __launch_bounds__(256)
-hier_unpack_add_kernel(int64_t h, int64_t w, uint64_t cap, const int64_t* __restrict__ oy,
-                       const int64_t* __restrict__ ox, const double* __restrict__ P,
-                       double* __restrict__ C, int64_t rs, int64_t cs) {
-  const int64_t y = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (y >= h) return;
-  const int64_t x0 = (int64_t)blockIdx.y * kHierCols;
-  const int64_t py = oy[y];
-  double p[kHierCols], c0[kHierCols];
+hier_unpack_add_kernel(int64_t h, int64_t w, uint64_t cap, uint32_t nbu, int log2r,
+                       const int64_t* __restrict__ oy, const int64_t* __restrict__ ox,
+                       const double* __restrict__ P, double* __restrict__ C, int64_t rs,
+                       int64_t cs) {
+  __shared__ HierTile<G> T;
+  const uint32_t bu = blockIdx.x % nbu, bv = blockIdx.x / nbu;
+  const int64_t u0 = (int64_t)bu * 256, v0 = (int64_t)bv * G;
+  const int64_t u = u0 + threadIdx.x;
+  const int64_t hu = LX ? w : h, hv = LX ? h : w;
+  const int64_t su = LX ? cs : rs, sv = LX ? rs : cs;
+  const int64_t* ou = LX ? ox : oy;
+  const int64_t* ov = LX ? oy : ox;
+  // C first (its loads overlap the layout side), clamped into the matrix
+  const int64_t uc = u < hu ? u : hu - 1;
+  double c0[G];
 #pragma unroll
-  for (int c = 0; c < kHierCols; ++c) {
-    const int64_t x = x0 + c < w ? x0 + c : w - 1;  // past the edge: re-read column w-1
-    const int64_t o = py + ox[x];
+  for (int c = 0; c < G; ++c) {
+    const int64_t j = v0 + c < hv ? v0 + c : hv - 1;
+    c0[c] = C[uc * su + j * sv];
+  }
+  T.ou[threadIdx.x] = ou[uc];
+  if (threadIdx.x < G) T.ov[threadIdx.x] = ov[v0 + threadIdx.x < hv ? v0 + threadIdx.x : hv - 1];
+  __syncthreads();
+  double p[G];
+#pragma unroll
+  for (int q = 0; q < G; ++q) {
+    int uu, c;
+    layout_order<G>(q, log2r, uu, c);
+    const int64_t o = T.ou[uu] + T.ov[c];
     const bool in = (uint64_t)o < cap;
-    p[c] = P[in ? o : 0];
-    p[c] = in ? p[c] : 0.0;
-    c0[c] = C[y * rs + x * cs];
+    p[q] = P[in ? o : 0];
+    p[q] = in ? p[q] : 0.0;
   }
 #pragma unroll
-  for (int c = 0; c < kHierCols; ++c)
-    if (x0 + c < w) C[y * rs + (x0 + c) * cs] = p[c] + c0[c];
+  for (int q = 0; q < G; ++q) {
+    int uu, c;
+    layout_order<G>(q, log2r, uu, c);
+    T.v[c * HierTile<G>::kStride + uu] = p[q];
+  }
+  __syncthreads();
+  if (u >= hu) return;
+#pragma unroll
+  for (int c = 0; c < G; ++c)
+    if (v0 + c < hv) C[u * su + (v0 + c) * sv] = T.v[c * HierTile<G>::kStride + threadIdx.x] + c0[c];
+}
+
+// Lanes along the source's unit stride (rows of a column-major matrix, the
+// default; columns of a row-major one). G tiles the leaf's contiguous run
+// along the minor index when the leaf runs that way (its width X for leaves
+// stored by rows and lanes down columns, its height Y in the transposed
+// case), else 16; R is the power-of-two part (at most 256) of the leaf's run
+// along the lane index, 1 when the leaf runs along the minor index.
+struct HierShape {
+  bool lx;
+  int g;
+  int log2r;
+};
+
+static HierShape hier_shape(const momms_hier_layout& L, int64_t rs, int64_t cs) {
+  HierShape sh;
+  sh.lx = cs == 1 && rs != 1;
+  const int64_t Y = L.ny ? L.y_blksz[L.ny - 1] : L.h_pad, X = L.nx ? L.x_blksz[L.nx - 1] : L.w_pad;
+  const bool run_u = sh.lx ? L.lcs == 1 : L.lrs == 1;  // leaf contiguous along the lanes
+  const int64_t Ru = run_u ? (sh.lx ? X : Y) : 1, Rv = run_u ? 1 : (sh.lx ? Y : X);
+  sh.g = Rv % 16 == 0 || Rv == 1 ? 16 : Rv % 12 == 0 ? 12 : 8;
+  sh.log2r = 0;
+  while (sh.log2r < 8 && Ru % ((int64_t)2 << sh.log2r) == 0) ++sh.log2r;
+  return sh;
+}
+
+template <int G, bool LX>
+static void launch_pack(const momms_hier_layout& L, int log2r, const int64_t* t, const double* A,
+                        int64_t rs, int64_t cs, double* P, hipStream_t s) {
+  const int64_t U = LX ? L.w_pad : L.h_pad, V = LX ? L.h_pad : L.w_pad;
+  const uint32_t nbu = (uint32_t)ceil_div(U, 256);
+  const uint32_t blocks = nbu * (uint32_t)ceil_div(V, G);
+  hipLaunchKernelGGL((hier_pack_kernel<G, LX>), dim3(blocks), dim3(256), 0, s, L.h, L.w, L.h_pad,
+                     L.w_pad, (uint64_t)L.capacity, nbu, log2r, t, t + L.h_pad, A, rs, cs, P);
+}
+
+template <int G, bool LX>
+static void launch_unpack(const momms_hier_layout& L, int log2r, const int64_t* t,
+                          const double* P, double* C, int64_t rs, int64_t cs, hipStream_t s) {
+  const int64_t U = LX ? L.w : L.h, V = LX ? L.h : L.w;
+  const uint32_t nbu = (uint32_t)ceil_div(U, 256);
+  const uint32_t blocks = nbu * (uint32_t)ceil_div(V, G);
+  hipLaunchKernelGGL((hier_unpack_add_kernel<G, LX>), dim3(blocks), dim3(256), 0, s, L.h, L.w,
+                     (uint64_t)L.capacity, nbu, log2r, t, t + L.h_pad, P, C, rs, cs);
 }
 
 static hipError_t hier_launch_tables(const momms_hier_layout& L, hipStream_t st, int64_t** t) {
@@ -117,8 +221,9 @@ static int check_layout(const char* fn, int arg, const momms_hier_layout* L) {
       L->h_pad > ((int64_t)1 << 31) || L->w_pad > ((int64_t)1 << 31) ||
       L->capacity != L->h_pad * L->w_pad)
     return bad_arg(fn, arg, "layout is inconsistent");
-  if (L->w_pad > 65535 * (int64_t)kHierCols)
-    return bad_arg(fn, arg, "layout is wider than 524280 padded columns");
+  // one launch of at most 2^32 - 1 workgroups of 256 lanes x 8 indices
+  if (ceil_div(L->h_pad, 256) * ceil_div(L->w_pad, 8) >= ((int64_t)1 << 32))
+    return bad_arg(fn, arg, "layout too large for one launch");
   for (int64_t l = 0; l < L->ny; ++l)
     if (L->y_blksz[l] <= 0) return bad_arg(fn, arg, "layout has a non-positive block size");
   for (int64_t l = 0; l < L->nx; ++l)
@@ -200,10 +305,16 @@ extern "C" int momms_hip_hier_pack(const momms_hier_layout* L, const double* A, 
   int64_t* t = nullptr;
   hipError_t e = hier_launch_tables(*L, s, &t);
   if (e != hipSuccess) return hip_status(fn, e);
-  dim3 grid((unsigned)ceil_div(L->h_pad, 256), (unsigned)ceil_div(L->w_pad, kHierCols));
-  hipLaunchKernelGGL(hier_pack_kernel, grid, dim3(256), 0, s, L->h, L->w, L->h_pad, L->w_pad,
-                     (uint64_t)L->capacity, t,
-                     t + L->h_pad, A, rsA, csA, P);
+  const HierShape sh = hier_shape(*L, rsA, csA);
+  if (sh.lx) {
+    if (sh.g == 16) launch_pack<16, true>(*L, sh.log2r, t, A, rsA, csA, P, s);
+    else if (sh.g == 12) launch_pack<12, true>(*L, sh.log2r, t, A, rsA, csA, P, s);
+    else launch_pack<8, true>(*L, sh.log2r, t, A, rsA, csA, P, s);
+  } else {
+    if (sh.g == 16) launch_pack<16, false>(*L, sh.log2r, t, A, rsA, csA, P, s);
+    else if (sh.g == 12) launch_pack<12, false>(*L, sh.log2r, t, A, rsA, csA, P, s);
+    else launch_pack<8, false>(*L, sh.log2r, t, A, rsA, csA, P, s);
+  }
   return hip_status(fn, hipGetLastError());
 }
 
@@ -219,9 +330,15 @@ extern "C" int momms_hip_hier_unpack_add(const momms_hier_layout* L, const doubl
   int64_t* t = nullptr;
   hipError_t e = hier_launch_tables(*L, s, &t);
   if (e != hipSuccess) return hip_status(fn, e);
-  dim3 grid((unsigned)ceil_div(L->h, 256), (unsigned)ceil_div(L->w, kHierCols));
-  hipLaunchKernelGGL(hier_unpack_add_kernel, grid, dim3(256), 0, s, L->h, L->w,
-                     (uint64_t)L->capacity, t, t + L->h_pad,
-                     P, C, rsC, csC);
+  const HierShape sh = hier_shape(*L, rsC, csC);
+  if (sh.lx) {
+    if (sh.g == 16) launch_unpack<16, true>(*L, sh.log2r, t, P, C, rsC, csC, s);
+    else if (sh.g == 12) launch_unpack<12, true>(*L, sh.log2r, t, P, C, rsC, csC, s);
+    else launch_unpack<8, true>(*L, sh.log2r, t, P, C, rsC, csC, s);
+  } else {
+    if (sh.g == 16) launch_unpack<16, false>(*L, sh.log2r, t, P, C, rsC, csC, s);
+    else if (sh.g == 12) launch_unpack<12, false>(*L, sh.log2r, t, P, C, rsC, csC, s);
+    else launch_unpack<8, false>(*L, sh.log2r, t, P, C, rsC, csC, s);
+  }
   return hip_status(fn, hipGetLastError());
 }

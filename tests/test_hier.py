@@ -192,6 +192,7 @@ def test_device_entry_points_validate_before_gpu_work():
     bad.capacity += 1
     assert l.momms_hip_hier_pack(ctypes.byref(bad), A, 1, 10, A, None) == -1
     assert "inconsistent" in capi.last_error()
-    wide = capi.hier_layout(1, 600000, [], M, N, 1, 8, 1, 1)
-    assert l.momms_hip_hier_pack(ctypes.byref(wide), A, 1, 1, A, None) == -1
-    assert "524280" in capi.last_error()
+    # 2^23 x 4096 workgroups: more than one launch holds
+    huge = capi.hier_layout(2 ** 31, 32768, [], M, N, 1, 1, 1, 2 ** 31)
+    assert l.momms_hip_hier_pack(ctypes.byref(huge), A, 1, 2 ** 31, A, None) == -1
+    assert "too large" in capi.last_error()
