@@ -229,8 +229,9 @@ def hier(capi, torch, gen, steps, n=8192):
     """Device-resident hierarchical matrices at n x n: pack a column-major
     matrix into the Goto A (MR=4, MC=120, KC=192) and Goto C (MR=4, NR=12,
     MC=120, NC=3000) buffers of the reference's l3 experiment and add the C
-    buffer back; algorithmic bytes = reads + writes of each element once
-    (padding places written as zeros)."""
+    buffer back (algorithmic bytes = reads + writes of each element once,
+    padding places written as zeros); then the GEMM C += A*B with A, B, C in
+    the Goto A/B/C buffers."""
     M, N, K = capi.STEP_M, capi.STEP_N, capi.STEP_K
     goto = [(M, 4), (N, 12), (M, 120), (K, 192), (N, 3000)]
     out = {}
@@ -250,6 +251,18 @@ def hier(capi, torch, gen, steps, n=8192):
                                          "GBps": round(b / ms / 1e6, 1),
                                          "frac_hbm": round(b / ms / 1e6 / PEAK_HBM, 4)}
         del P
+    # the GEMM on Goto A/B/C hierarchical buffers (C += A*B), against the plain
+    # GEMM's 2mnk
+    specs = [(M, K, 1, 4), (K, N, 12, 1), (M, N, 12, 1)]
+    Ls = [capi.hier_layout(n, n, goto, yd, xd, 4, 12, lrs, lcs) for yd, xd, lrs, lcs in specs]
+    Ps = [torch.empty(L.capacity, dtype=torch.float64, device="cuda") for L in Ls]
+    for L, P in zip(Ls, Ps):
+        capi.hier_pack(L, X, P)
+    ms = time_loop(lambda: capi.dgemm_hier(Ls[0], Ps[0], Ls[1], Ps[1], Ls[2], Ps[2], 1.0, 1.0),
+                   max(2, steps // 3), 1)
+    out["dgemm_hier_goto"] = {"ms": round(ms, 4), "tflops": round(2.0 * n ** 3 / ms / 1e9, 3),
+                              "frac_fp64_peak": round(2.0 * n ** 3 / ms / 1e9 / PEAK_FP64, 4)}
+    del Ps
     return out
 
 

@@ -272,6 +272,19 @@ int momms_hip_hier_pack(const momms_hier_layout* L, const double* A, int64_t rsA
 int momms_hip_hier_unpack_add(const momms_hier_layout* L, const double* P, double* C,
                               int64_t rsC, int64_t csC, void* stream);
 
+/* GemmNode::run over Hierarch<f64> operands, as the reference's l3
+ * experiment runs its control trees on them (src/exper/l3.rs:46-72): C :=
+ * beta*C + alpha*A*B on the logical elements of the hierarchical buffers PA
+ * (LA: m x k), PB (LB: k x n) and PC (LC: m x n); C's padding places are
+ * neither read nor written, C is not read when beta == 0. The operands are
+ * copied column-major into library scratch (8(mk + kn + mn) bytes per
+ * stream, kept), multiplied by momms_hip_dgemm (same kernels, same bits as
+ * on the column-major matrices), and the result written back into PC. */
+int momms_hip_dgemm_hier(const momms_hier_layout* LA, const double* PA,
+                         const momms_hier_layout* LB, const double* PB, const double* alpha,
+                         const double* beta, const momms_hier_layout* LC, double* PC,
+                         void* stream);
+
 /* ---- accuracy statement -----------------------------------------------------
  * Recomputes `samples` entries of beta*C0 + alpha*A*B in double-double on the
  * GPU (Dot2, Ogita-Rump-Oishi 2005) and returns, on the host, the largest

@@ -82,6 +82,7 @@ _SIGS: dict[str, list] = {
                                ctypes.c_int32, _c_i64, _c_i64, _c_i64, _c_i64],
     "momms_hip_hier_pack": [_c_vp, _c_vp, _c_i64, _c_i64, _c_vp, _c_vp],
     "momms_hip_hier_unpack_add": [_c_vp, _c_vp, _c_vp, _c_i64, _c_i64, _c_vp],
+    "momms_hip_dgemm_hier": [_c_vp, _c_vp, _c_vp, _c_vp, _c_dp, _c_dp, _c_vp, _c_vp, _c_vp],
     "momms_hip_crt_bf16_reconstruct": [_c_i64, _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64,
                                        _c_vp, _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64,
                                        _c_int, _c_vp, ctypes.c_size_t, _c_vp],
@@ -524,6 +525,22 @@ def hier_unpack_add(L: HierLayout, P, C, stream=None):
           lib().momms_hip_hier_unpack_add(ctypes.byref(L), P.data_ptr(), c, rc, cc,
                                           stream_handle(stream)))
     return C
+
+
+def dgemm_hier(LA: HierLayout, PA, LB: HierLayout, PB, LC: HierLayout, PC, alpha: float = 1.0,
+               beta: float = 1.0, stream=None):
+    """C := beta*C + alpha*A*B on hierarchical buffers (float64 GPU tensors of at
+    least LA/LB/LC.capacity elements), in place on PC."""
+    _f64((PA, "PA"), (PB, "PB"), (PC, "PC"))
+    for P, L, nm in ((PA, LA, "PA"), (PB, LB, "PB"), (PC, LC, "PC")):
+        if P.numel() < L.capacity or not P.is_cuda or (P.numel() > 1 and not P.is_contiguous()):
+            raise ValueError(f"{nm} must be a contiguous GPU tensor of >= {L.capacity} elements")
+    a, b = ctypes.c_double(alpha), ctypes.c_double(beta)
+    check("momms_hip_dgemm_hier",
+          lib().momms_hip_dgemm_hier(ctypes.byref(LA), PA.data_ptr(), ctypes.byref(LB),
+                                     PB.data_ptr(), ctypes.byref(a), ctypes.byref(b),
+                                     ctypes.byref(LC), PC.data_ptr(), stream_handle(stream)))
+    return PC
 
 
 def sample_positions(seed: int, samples: int, m: int, n: int):

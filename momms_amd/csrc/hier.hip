@@ -69,8 +69,10 @@ __device__ __forceinline__ void layout_order(int q, int log2r, int& uu, int& c) 
   c = (int)(r >> log2r);
 }
 
-// h == 0 or w == 0 (grid-uniform): nothing is read and every place is padding
-template <int G, bool LX>
+// h == 0 or w == 0 (grid-uniform): nothing is read and every place is padding.
+// PADW: padding places are written (zeros); otherwise only the matrix's own
+// places are.
+template <int G, bool LX, bool PADW>
 __global__ void __launch_bounds__(256)
 hier_pack_kernel(int64_t h, int64_t w, int64_t h_pad, int64_t w_pad, uint64_t cap, uint32_t nbu,
                  int log2r, const int64_t* __restrict__ oy, const int64_t* __restrict__ ox,
@@ -110,11 +112,13 @@ hier_pack_kernel(int64_t h, int64_t w, int64_t h_pad, int64_t w_pad, uint64_t ca
     int uu, c;
     layout_order<G>(q, log2r, uu, c);
     const int64_t o = T.ou[uu] + T.ov[c];
-    if (u0 + uu < U && v0 + c < V && (uint64_t)o < cap) P[o] = T.v[c * HierTile<G>::kStride + uu];
+    const bool place = PADW ? (u0 + uu < U && v0 + c < V) : (u0 + uu < hu && v0 + c < hv);
+    if (place && (uint64_t)o < cap) P[o] = T.v[c * HierTile<G>::kStride + uu];
   }
 }
 
-template <int G, bool LX>
+// ADD: C += P at the matrix's places; otherwise C := P there (C not read).
+template <int G, bool LX, bool ADD>
 __global__ void __launch_bounds__(256)
 hier_unpack_add_kernel(int64_t h, int64_t w, uint64_t cap, uint32_t nbu, int log2r,
                        const int64_t* __restrict__ oy, const int64_t* __restrict__ ox,
@@ -134,7 +138,7 @@ hier_unpack_add_kernel(int64_t h, int64_t w, uint64_t cap, uint32_t nbu, int log
 #pragma unroll
   for (int c = 0; c < G; ++c) {
     const int64_t j = v0 + c < hv ? v0 + c : hv - 1;
-    c0[c] = C[uc * su + j * sv];
+    c0[c] = ADD ? C[uc * su + j * sv] : 0.0;
   }
   T.ou[threadIdx.x] = ou[uc];
   if (threadIdx.x < G) T.ov[threadIdx.x] = ov[v0 + threadIdx.x < hv ? v0 + threadIdx.x : hv - 1];
@@ -159,7 +163,10 @@ hier_unpack_add_kernel(int64_t h, int64_t w, uint64_t cap, uint32_t nbu, int log
   if (u >= hu) return;
 #pragma unroll
   for (int c = 0; c < G; ++c)
-    if (v0 + c < hv) C[u * su + (v0 + c) * sv] = T.v[c * HierTile<G>::kStride + threadIdx.x] + c0[c];
+    if (v0 + c < hv)
+      C[u * su + (v0 + c) * sv] =
+          ADD ? T.v[c * HierTile<G>::kStride + threadIdx.x] + c0[c]
+              : T.v[c * HierTile<G>::kStride + threadIdx.x];
 }
 
 // Lanes along the source's unit stride (rows of a column-major matrix, the
@@ -186,24 +193,25 @@ static HierShape hier_shape(const momms_hier_layout& L, int64_t rs, int64_t cs) 
   return sh;
 }
 
-template <int G, bool LX>
+template <int G, bool LX, bool PADW>
 static void launch_pack(const momms_hier_layout& L, int log2r, const int64_t* t, const double* A,
                         int64_t rs, int64_t cs, double* P, hipStream_t s) {
   const int64_t U = LX ? L.w_pad : L.h_pad, V = LX ? L.h_pad : L.w_pad;
   const uint32_t nbu = (uint32_t)ceil_div(U, 256);
   const uint32_t blocks = nbu * (uint32_t)ceil_div(V, G);
-  hipLaunchKernelGGL((hier_pack_kernel<G, LX>), dim3(blocks), dim3(256), 0, s, L.h, L.w, L.h_pad,
-                     L.w_pad, (uint64_t)L.capacity, nbu, log2r, t, t + L.h_pad, A, rs, cs, P);
+  hipLaunchKernelGGL((hier_pack_kernel<G, LX, PADW>), dim3(blocks), dim3(256), 0, s, L.h, L.w,
+                     L.h_pad, L.w_pad, (uint64_t)L.capacity, nbu, log2r, t, t + L.h_pad, A, rs, cs,
+                     P);
 }
 
-template <int G, bool LX>
+template <int G, bool LX, bool ADD>
 static void launch_unpack(const momms_hier_layout& L, int log2r, const int64_t* t,
                           const double* P, double* C, int64_t rs, int64_t cs, hipStream_t s) {
   const int64_t U = LX ? L.w : L.h, V = LX ? L.h : L.w;
   const uint32_t nbu = (uint32_t)ceil_div(U, 256);
   const uint32_t blocks = nbu * (uint32_t)ceil_div(V, G);
-  hipLaunchKernelGGL((hier_unpack_add_kernel<G, LX>), dim3(blocks), dim3(256), 0, s, L.h, L.w,
-                     (uint64_t)L.capacity, nbu, log2r, t, t + L.h_pad, P, C, rs, cs);
+  hipLaunchKernelGGL((hier_unpack_add_kernel<G, LX, ADD>), dim3(blocks), dim3(256), 0, s, L.h,
+                     L.w, (uint64_t)L.capacity, nbu, log2r, t, t + L.h_pad, P, C, rs, cs);
 }
 
 static hipError_t hier_launch_tables(const momms_hier_layout& L, hipStream_t st, int64_t** t) {
@@ -229,6 +237,48 @@ static int check_layout(const char* fn, int arg, const momms_hier_layout* L) {
   for (int64_t l = 0; l < L->nx; ++l)
     if (L->x_blksz[l] <= 0) return bad_arg(fn, arg, "layout has a non-positive block size");
   return 0;
+}
+
+// strided matrix -> layout (all padding places written as 0 when padw)
+template <bool PADW>
+static hipError_t hier_pack_impl(const momms_hier_layout& L, const double* A, int64_t rs,
+                                 int64_t cs, double* P, hipStream_t s) {
+  if (L.capacity == 0 || (!PADW && (L.h == 0 || L.w == 0))) return hipSuccess;
+  int64_t* t = nullptr;
+  hipError_t e = hier_launch_tables(L, s, &t);
+  if (e != hipSuccess) return e;
+  const HierShape sh = hier_shape(L, rs, cs);
+  if (sh.lx) {
+    if (sh.g == 16) launch_pack<16, true, PADW>(L, sh.log2r, t, A, rs, cs, P, s);
+    else if (sh.g == 12) launch_pack<12, true, PADW>(L, sh.log2r, t, A, rs, cs, P, s);
+    else launch_pack<8, true, PADW>(L, sh.log2r, t, A, rs, cs, P, s);
+  } else {
+    if (sh.g == 16) launch_pack<16, false, PADW>(L, sh.log2r, t, A, rs, cs, P, s);
+    else if (sh.g == 12) launch_pack<12, false, PADW>(L, sh.log2r, t, A, rs, cs, P, s);
+    else launch_pack<8, false, PADW>(L, sh.log2r, t, A, rs, cs, P, s);
+  }
+  return hipGetLastError();
+}
+
+// layout -> strided matrix: C += P (ADD) or C := P at the matrix's places
+template <bool ADD>
+static hipError_t hier_unpack_impl(const momms_hier_layout& L, const double* P, double* C,
+                                   int64_t rs, int64_t cs, hipStream_t s) {
+  if (L.h == 0 || L.w == 0) return hipSuccess;
+  int64_t* t = nullptr;
+  hipError_t e = hier_launch_tables(L, s, &t);
+  if (e != hipSuccess) return e;
+  const HierShape sh = hier_shape(L, rs, cs);
+  if (sh.lx) {
+    if (sh.g == 16) launch_unpack<16, true, ADD>(L, sh.log2r, t, P, C, rs, cs, s);
+    else if (sh.g == 12) launch_unpack<12, true, ADD>(L, sh.log2r, t, P, C, rs, cs, s);
+    else launch_unpack<8, true, ADD>(L, sh.log2r, t, P, C, rs, cs, s);
+  } else {
+    if (sh.g == 16) launch_unpack<16, false, ADD>(L, sh.log2r, t, P, C, rs, cs, s);
+    else if (sh.g == 12) launch_unpack<12, false, ADD>(L, sh.log2r, t, P, C, rs, cs, s);
+    else launch_unpack<8, false, ADD>(L, sh.log2r, t, P, C, rs, cs, s);
+  }
+  return hipGetLastError();
 }
 
 }  // namespace momms
@@ -300,22 +350,7 @@ extern "C" int momms_hip_hier_pack(const momms_hier_layout* L, const double* A, 
   if ((st = check_layout(fn, 1, L))) return st;
   if ((st = check_operand(fn, 2, A, L->h, L->w, rsA, csA))) return st;
   if (!P && L->capacity) return bad_arg(fn, 5, "P is null");
-  if (L->capacity == 0) return 0;
-  hipStream_t s = (hipStream_t)stream;
-  int64_t* t = nullptr;
-  hipError_t e = hier_launch_tables(*L, s, &t);
-  if (e != hipSuccess) return hip_status(fn, e);
-  const HierShape sh = hier_shape(*L, rsA, csA);
-  if (sh.lx) {
-    if (sh.g == 16) launch_pack<16, true>(*L, sh.log2r, t, A, rsA, csA, P, s);
-    else if (sh.g == 12) launch_pack<12, true>(*L, sh.log2r, t, A, rsA, csA, P, s);
-    else launch_pack<8, true>(*L, sh.log2r, t, A, rsA, csA, P, s);
-  } else {
-    if (sh.g == 16) launch_pack<16, false>(*L, sh.log2r, t, A, rsA, csA, P, s);
-    else if (sh.g == 12) launch_pack<12, false>(*L, sh.log2r, t, A, rsA, csA, P, s);
-    else launch_pack<8, false>(*L, sh.log2r, t, A, rsA, csA, P, s);
-  }
-  return hip_status(fn, hipGetLastError());
+  return hip_status(fn, hier_pack_impl<true>(*L, A, rsA, csA, P, (hipStream_t)stream));
 }
 
 extern "C" int momms_hip_hier_unpack_add(const momms_hier_layout* L, const double* P, double* C,
@@ -325,20 +360,41 @@ extern "C" int momms_hip_hier_unpack_add(const momms_hier_layout* L, const doubl
   if ((st = check_layout(fn, 1, L))) return st;
   if (!P && L->capacity) return bad_arg(fn, 2, "P is null");
   if ((st = check_operand(fn, 3, C, L->h, L->w, rsC, csC))) return st;
-  if (L->h == 0 || L->w == 0) return 0;
+  return hip_status(fn, hier_unpack_impl<true>(*L, P, C, rsC, csC, (hipStream_t)stream));
+}
+
+extern "C" int momms_hip_dgemm_hier(const momms_hier_layout* LA, const double* PA,
+                                    const momms_hier_layout* LB, const double* PB,
+                                    const double* alpha, const double* beta,
+                                    const momms_hier_layout* LC, double* PC, void* stream) {
+  static const char* fn = "momms_hip_dgemm_hier";
+  int st;
+  if ((st = check_layout(fn, 1, LA))) return st;
+  if ((st = check_layout(fn, 3, LB))) return st;
+  if ((st = check_layout(fn, 7, LC))) return st;
+  if (!alpha) return bad_arg(fn, 5, "alpha is null");
+  if (!beta) return bad_arg(fn, 6, "beta is null");
+  const int64_t m = LC->h, n = LC->w, k = LA->w;
+  if (LA->h != m) return bad_arg(fn, 1, "A has a different height from C");
+  if (LB->h != k) return bad_arg(fn, 3, "B's height differs from A's width");
+  if (LB->w != n) return bad_arg(fn, 3, "B has a different width from C");
+  if (m == 0 || n == 0) return 0;
+  if (!PA && m * k) return bad_arg(fn, 2, "PA is null");
+  if (!PB && k * n) return bad_arg(fn, 4, "PB is null");
+  if (!PC) return bad_arg(fn, 8, "PC is null");
   hipStream_t s = (hipStream_t)stream;
-  int64_t* t = nullptr;
-  hipError_t e = hier_launch_tables(*L, s, &t);
+  // the operands column-major in scratch slot 1 (the GEMM's own scratch use,
+  // and the offset tables, are slot 0)
+  double* W = nullptr;
+  hipError_t e = stream_scratch(s, (size_t)(m * k + k * n + m * n) * sizeof(double), (void**)&W, 1);
   if (e != hipSuccess) return hip_status(fn, e);
-  const HierShape sh = hier_shape(*L, rsC, csC);
-  if (sh.lx) {
-    if (sh.g == 16) launch_unpack<16, true>(*L, sh.log2r, t, P, C, rsC, csC, s);
-    else if (sh.g == 12) launch_unpack<12, true>(*L, sh.log2r, t, P, C, rsC, csC, s);
-    else launch_unpack<8, true>(*L, sh.log2r, t, P, C, rsC, csC, s);
-  } else {
-    if (sh.g == 16) launch_unpack<16, false>(*L, sh.log2r, t, P, C, rsC, csC, s);
-    else if (sh.g == 12) launch_unpack<12, false>(*L, sh.log2r, t, P, C, rsC, csC, s);
-    else launch_unpack<8, false>(*L, sh.log2r, t, P, C, rsC, csC, s);
-  }
-  return hip_status(fn, hipGetLastError());
+  double *A = W, *B = W + m * k, *C = W + m * k + k * n;
+  if ((e = hier_unpack_impl<false>(*LA, PA, A, 1, m, s)) != hipSuccess) return hip_status(fn, e);
+  if ((e = hier_unpack_impl<false>(*LB, PB, B, 1, k > 0 ? k : 1, s)) != hipSuccess)
+    return hip_status(fn, e);
+  if (*beta != 0.0 && (e = hier_unpack_impl<false>(*LC, PC, C, 1, m, s)) != hipSuccess)
+    return hip_status(fn, e);
+  if ((st = momms_hip_dgemm(m, n, k, alpha, A, 1, m, B, 1, k > 0 ? k : 1, beta, C, 1, m, stream)))
+    return st;
+  return hip_status(fn, hier_pack_impl<false>(*LC, C, 1, m, PC, s));
 }

@@ -5,10 +5,11 @@
 
 namespace momms {
 
-hipError_t stream_scratch(hipStream_t st, size_t bytes, void** out) {
+hipError_t stream_scratch(hipStream_t st, size_t bytes, void** out, int slot) {
   struct Entry {
     int dev;
     hipStream_t st;
+    int slot;
     void* p;
     size_t bytes;
   };
@@ -19,7 +20,7 @@ hipError_t stream_scratch(hipStream_t st, size_t bytes, void** out) {
   if (e != hipSuccess) return e;
   std::lock_guard<std::mutex> lock(mu);
   for (Entry& x : pool) {
-    if (x.dev != dev || x.st != st) continue;
+    if (x.dev != dev || x.st != st || x.slot != slot) continue;
     if (x.bytes < bytes) {
       if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
       if ((e = hipFree(x.p)) != hipSuccess) return e;
@@ -33,7 +34,7 @@ hipError_t stream_scratch(hipStream_t st, size_t bytes, void** out) {
   }
   void* p = nullptr;
   if ((e = hipMalloc(&p, bytes)) != hipSuccess) return e;
-  pool.push_back({dev, st, p, bytes});
+  pool.push_back({dev, st, slot, p, bytes});
   *out = p;
   return hipSuccess;
 }

@@ -95,3 +95,60 @@ def test_pack_small_matches_closed_form_on_a_row_major_source():
     hp, wp, off = goto_a(300, 500)
     Pn = P.cpu().numpy()
     np.testing.assert_array_equal(Pn[off[:300, :500]], A.cpu().numpy())
+
+
+def colmajor(rng, h, w):
+    t = torch.empty((w, h), dtype=torch.float64, device="cuda").T
+    t.copy_(torch.from_numpy(rng.uniform(-1, 1, (h, w))))
+    return t
+
+
+GOTO_ABC = [(M, K, 4, 192, 1, 4), (K, po.N, 192, 12, 12, 1), (M, po.N, 4, 12, 12, 1)]
+
+
+@pytest.mark.parametrize("m,n,k", [(250, 130, 77), (1000, 700, 300), (64, 3001, 193), (5, 1, 1),
+                                   (120, 12, 0)])
+@pytest.mark.parametrize("alpha,beta", [(1.0, 1.0), (-0.5, 2.0), (1.0, 0.0)])
+def test_dgemm_hier_equals_dgemm_on_the_same_matrices(m, n, k, alpha, beta):
+    """The GEMM on Goto A/B/C hierarchical buffers (l3.rs:56-58) gives the bits
+    of momms_hip_dgemm on the column-major matrices, and leaves C's padding
+    places as they were."""
+    print(f"dgemm_hier {m}x{n}x{k} alpha={alpha} beta={beta}", flush=True)
+    rng = np.random.default_rng(m * 7 + n * 3 + k)
+    dims = [(m, k), (k, n), (m, n)]
+    Ls = [capi.hier_layout(h, w, GOTO, *spec) for (h, w), spec in zip(dims, GOTO_ABC)]
+    A, B, C = (colmajor(rng, h, w) for h, w in dims)
+    Ps = [torch.zeros(max(L.capacity, 1), dtype=torch.float64, device="cuda") for L in Ls]
+    for L, X, P in zip(Ls, (A, B, C), Ps):
+        capi.hier_pack(L, X, P)
+    # C's padding places hold NaN; with beta == 0 its own places do too
+    offC = torch.from_numpy(po.hier_offsets(oracle_layout(Ls[2]))).cuda()
+    pad = torch.ones(offC.shape, dtype=torch.bool, device="cuda")
+    pad[:m, :n] = False
+    Ps[2][offC[pad]] = float("nan")
+    if beta == 0.0:
+        Ps[2][offC[:m, :n].reshape(-1)] = float("nan")
+    capi.dgemm_hier(Ls[0], Ps[0], Ls[1], Ps[1], Ls[2], Ps[2], alpha, beta)
+    Cref = C.clone()
+    capi.dgemm(A, B, Cref, alpha, beta)
+    got = Ps[2][offC[:m, :n].reshape(-1)].view(m, n)
+    torch.cuda.synchronize()
+    assert torch.equal(got, Cref)
+    assert bool(torch.isnan(Ps[2][offC[pad]]).all())
+
+
+def test_dgemm_hier_goto_8192_matches_dgemm():
+    n = 8192
+    print(f"dgemm_hier goto {n}^3", flush=True)
+    g = torch.Generator(device="cuda").manual_seed(11)
+    X = [torch.empty((n, n), dtype=torch.float64, device="cuda").T.uniform_(-1, 1, generator=g)
+         for _ in range(3)]
+    Ls = [capi.hier_layout(n, n, GOTO, *spec) for spec in GOTO_ABC]
+    Ps = [torch.empty(L.capacity, dtype=torch.float64, device="cuda") for L in Ls]
+    for L, x, P in zip(Ls, X, Ps):
+        capi.hier_pack(L, x, P)
+    capi.dgemm_hier(Ls[0], Ps[0], Ls[1], Ps[1], Ls[2], Ps[2], 1.0, 1.0)
+    capi.dgemm(X[0], X[1], X[2], 1.0, 1.0)
+    back = torch.zeros((n, n), dtype=torch.float64, device="cuda").T
+    capi.hier_unpack_add(Ls[2], Ps[2], back)
+    assert torch.equal(back, X[2])

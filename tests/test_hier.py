@@ -196,3 +196,23 @@ def test_device_entry_points_validate_before_gpu_work():
     huge = capi.hier_layout(2 ** 31, 32768, [], M, N, 1, 1, 1, 2 ** 31)
     assert l.momms_hip_hier_pack(ctypes.byref(huge), A, 1, 2 ** 31, A, None) == -1
     assert "too large" in capi.last_error()
+
+
+def test_dgemm_hier_validates_before_gpu_work():
+    l = capi.lib()
+    LA = capi.hier_layout(10, 7, GOTO, M, K, 4, 192, 1, 4)
+    LB = capi.hier_layout(7, 5, GOTO, K, N, 192, 12, 12, 1)
+    LC = capi.hier_layout(10, 5, GOTO, M, N, 4, 12, 12, 1)
+    bad = capi.hier_layout(9, 5, GOTO, M, N, 4, 12, 12, 1)
+    p = ctypes.c_void_p(16)
+    one = ctypes.c_double(1.0)
+    r = ctypes.byref
+    f = l.momms_hip_dgemm_hier
+    assert f(None, p, r(LB), p, r(one), r(one), r(LC), p, None) == -1
+    assert f(r(LA), p, r(LB), p, None, r(one), r(LC), p, None) == -5
+    assert f(r(LA), p, r(LB), p, r(one), None, r(LC), p, None) == -6
+    assert f(r(LA), p, r(LB), p, r(one), r(one), r(bad), p, None) == -1
+    assert "height" in capi.last_error()
+    assert f(r(LA), p, r(LA), p, r(one), r(one), r(LC), p, None) == -3
+    assert f(r(LA), None, r(LB), p, r(one), r(one), r(LC), p, None) == -2
+    assert f(r(LA), p, r(LB), p, r(one), r(one), r(LC), None, None) == -8
