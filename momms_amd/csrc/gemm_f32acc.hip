@@ -13,6 +13,7 @@
 // threads (2 x 2 wavefronts), tile 128 x 128, each wavefront 64 x 64 as
 // 2 x 2 blocks of 32 x 32. Operands are swapped at the MFMA (B slab as the
 // "A" operand) so stores to column-major C are coalesced, as in dgemm.hip.
+#include "dgemm_tile.hpp"
 #include "mfma.hpp"
 #include "panel_gemm.hpp"
 #include "scratch.hpp"
@@ -165,6 +166,170 @@ gemm_f32acc_kernel(int64_t m, int64_t n, int64_t k, float alpha, const TIn* __re
   }
 }
 
+// FP32 in, column-major A, B and C with 16-byte aligned columns: a 128 x 128
+// tile (4 wavefronts of 64 x 64 as 2 x 2 blocks of v_mfma_f32_32x32x2_f32)
+// over 32-deep slabs moved HBM -> LDS by buffer_load ... lds (1 KiB per
+// wavefront load, no register staging), two slabs in flight in turn; the
+// two stages are distinct LDS objects and the loop is unrolled by two, so
+// the reads of one stage never wait for the loads into the other.
+//   A slab [32][128]: one wavefront load = two k-rows (lanes 0-31 the first,
+//     32-63 the second), four rows of A per lane.
+//   B slab: 16 blocks of 8 columns x 32 k; column c of a block keeps its
+//     k-quad q at 16-byte slot c*8 + (q ^ c).
+// k order inside a slab: MFMA step s (0..15) gives the lanes of half h the
+// k-value 8(s/4) + 4h + s%4. So a lane's B values for four steps are one
+// 16-byte read (quad 2(s/4) + h of its column; the 8 columns of a group of
+// 8 lanes sit in distinct banks), and the A values of a half-wavefront are
+// 32 consecutive floats of one k-row.
+// Out-of-range elements: the buffer resources cover exactly the matrix
+// extents (reads past them return 0); rows >= m of A and columns >= n of B
+// may read neighbouring values, which reach only outputs that are not stored;
+// the k >= K entries of the last slab are zeroed in LDS after they land.
+__device__ __forceinline__ void sgemm_issue(__amdgpu_buffer_rsrc_t ra, int64_t lda,
+                                            __amdgpu_buffer_rsrc_t rb, int64_t ldb, int64_t row0,
+                                            int64_t col0, int64_t k0, float* As, float* Bs) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int bc = lane >> 3, bq = (lane & 7) ^ (lane >> 3);
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int pr = wave * 4 + q;  // k-rows 2pr, 2pr+1
+    const int64_t e = (row0 + 4 * (lane & 31)) + (k0 + 2 * pr + (lane >> 5)) * lda;
+    dma16(ra, &As[pr * 256], (uint32_t)(e * 4));
+  }
+#pragma unroll
+  for (int q = 0; q < 4; ++q) {
+    const int b = wave * 4 + q;  // columns col0 + 8b ..
+    const int64_t e = (col0 + 8 * b + bc) * ldb + k0 + 4 * bq;
+    dma16(rb, &Bs[b * 256], (uint32_t)(e * 4));
+  }
+}
+
+// the k >= kv entries of a landed slab set to zero (the last, partial slab)
+__device__ __forceinline__ void sgemm_clear_tail(float* As, float* Bs, int kv) {
+  for (int e = threadIdx.x; e < (32 - kv) * 128; e += 256) As[kv * 128 + e] = 0.0f;
+  for (int e = threadIdx.x; e < 16 * 256; e += 256) {
+    const int slot = (e & 255) >> 2, c = slot >> 3, q = (slot & 7) ^ c;
+    if (4 * q + (e & 3) >= kv) Bs[e] = 0.0f;
+  }
+}
+
+__device__ __forceinline__ void sgemm_slab(v16f32 (&acc)[2][2], const float* As, const float* Bs) {
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave & 1, wn = wave >> 1, h = lane >> 5, lr = lane & 31;
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    v4f32 bq[2];
+#pragma unroll
+    for (int b = 0; b < 2; ++b) {
+      const int j = wn * 64 + b * 32 + lr, c = j & 7, q = 2 * g + h;
+      bq[b] = *reinterpret_cast<const v4f32*>(&Bs[(j >> 3) * 256 + (c * 8 + (q ^ c)) * 4]);
+    }
+#pragma unroll
+    for (int s = 0; s < 4; ++s) {
+      const int kk = 8 * g + 4 * h + s;
+      float af[2];
+#pragma unroll
+      for (int a = 0; a < 2; ++a) af[a] = As[kk * 128 + wm * 64 + a * 32 + lr];
+#pragma unroll
+      for (int a = 0; a < 2; ++a)
+#pragma unroll
+        for (int b = 0; b < 2; ++b) acc[a][b] = mfma_f32_32x32x2(bq[b][s], af[a], acc[a][b]);
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256, 2)
+sgemm_dma_kernel(int64_t m, int64_t n, int64_t k, float alpha, const float* __restrict__ A,
+                 int64_t lda, uint32_t a_bytes, const float* __restrict__ B, int64_t ldb,
+                 uint32_t b_bytes, float beta, float* __restrict__ C, int64_t ldc) {
+  __shared__ __attribute__((aligned(16))) float As0[32 * 128];
+  __shared__ __attribute__((aligned(16))) float Bs0[32 * 128];
+  __shared__ __attribute__((aligned(16))) float As1[32 * 128];
+  __shared__ __attribute__((aligned(16))) float Bs1[32 * 128];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int wm = wave & 1, wn = wave >> 1, h = lane >> 5, lr = lane & 31;
+  const int64_t tiles_m = ceil_div(m, 128), tiles_n = ceil_div(n, 128);
+  const int64_t t = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
+  int64_t tm, tn;
+  grouped_tile(t, tiles_m, tiles_n, 8, tm, tn);
+  const int64_t row0 = tm * 128, col0 = tn * 128;
+  const __amdgpu_buffer_rsrc_t ra = make_rsrc(A, a_bytes), rb = make_rsrc(B, b_bytes);
+  const int64_t ktiles = ceil_div(k, 32);
+
+  v16f32 acc[2][2];
+#pragma unroll
+  for (int a = 0; a < 2; ++a)
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
+
+  // slab kt: wait for it, then (everyone done with the other stage) start
+  // slab kt + 1 into the other stage and multiply this one
+#define MOMMS_SG_STAGE(KT, Ac, Bc, An, Bn)                             \
+  {                                                                    \
+    wait_vmcnt<0>();                                                   \
+    lds_barrier();                                                     \
+    if ((KT) * 32 + 32 > k) {                                          \
+      sgemm_clear_tail(Ac, Bc, (int)(k - (KT) * 32));                  \
+      lds_barrier();                                                   \
+    }                                                                  \
+    if ((KT) + 1 < ktiles)                                             \
+      sgemm_issue(ra, lda, rb, ldb, row0, col0, ((KT) + 1) * 32, An, Bn); \
+    sgemm_slab(acc, Ac, Bc);                                           \
+  }
+  if (ktiles > 0) sgemm_issue(ra, lda, rb, ldb, row0, col0, 0, As0, Bs0);
+  for (int64_t kt = 0; kt < ktiles; kt += 2) {
+    MOMMS_SG_STAGE(kt, As0, Bs0, As1, Bs1)
+    if (kt + 1 < ktiles) MOMMS_SG_STAGE(kt + 1, As1, Bs1, As0, Bs0)
+  }
+#undef MOMMS_SG_STAGE
+
+  // every C value of a block row read before the first store; lane holds row
+  // l%32 of each block, columns 8(r/4) + 4(l/32) + r%4
+#pragma unroll
+  for (int a = 0; a < 2; ++a) {
+    const int64_t gi = row0 + wm * 64 + a * 32 + lr;
+    float cv[32];
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t gj = col0 + wn * 64 + b * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+        const bool ok = gi < m && gj < n;
+        cv[b * 16 + r] = (beta != 0.0f) ? C[ok ? gi + gj * ldc : 0] : 0.0f;
+      }
+#pragma unroll
+    for (int b = 0; b < 2; ++b)
+#pragma unroll
+      for (int r = 0; r < 16; ++r) {
+        const int64_t gj = col0 + wn * 64 + b * 32 + 8 * (r >> 2) + 4 * h + (r & 3);
+        if (gi < m && gj < n) {
+          const float sv = alpha * acc[a][b][r];
+          C[gi + gj * ldc] = (beta == 0.0f) ? sv : __builtin_fmaf(beta, cv[b * 16 + r], sv);
+        }
+      }
+  }
+}
+
+// The DMA path's conditions: column-major A, B, C, columns 16-byte aligned
+// (leading dimensions multiples of 4, 16-byte aligned bases) and operand
+// extents a 32-bit buffer offset can address.
+static bool sgemm_dma_ok(int64_t m, int64_t n, int64_t k, const float* A, int64_t rsA,
+                         int64_t csA, const float* B, int64_t rsB, int64_t csB, int64_t rsC,
+                         uint32_t& a_bytes, uint32_t& b_bytes) {
+  if (rsA != 1 || rsB != 1 || rsC != 1 || k == 0) return false;
+  if ((csA & 3) || (csB & 3)) return false;
+  if ((reinterpret_cast<uintptr_t>(A) & 15) || (reinterpret_cast<uintptr_t>(B) & 15)) return false;
+  const uint64_t ea = (uint64_t)((k - 1) * csA + m) * 4;
+  const uint64_t eb = (uint64_t)((n - 1) * csB + k) * 4;
+  const uint64_t lim = (1ull << 32) - (1ull << 20);
+  if (ea > lim || eb > lim) return false;
+  a_bytes = (uint32_t)ea;
+  b_bytes = (uint32_t)eb;
+  return true;
+}
+
 // BF16 in, FP32 out, large problems: both operands packed into 256-wide
 // panels (pack_bf16_panels, into library scratch), then the DMA-ring panel
 // GEMM of panel_gemm.hpp with a 256 x 256 tile and the alpha / beta epilogue.
@@ -282,6 +447,16 @@ static int f32acc_entry(const char* fn, int64_t m, int64_t n, int64_t k, const f
   constexpr int BK = std::is_same<T, float>::value ? 16 : 32;
   // the packed-panel form once the 256 x 256 tiles cover the device and k is
   // deep enough to repay the packing pass
+  if constexpr (std::is_same<T, float>::value) {
+    uint32_t ab = 0, bb = 0;
+    if (ceil_div(m, 128) * ceil_div(n, 128) >= 256 &&
+        sgemm_dma_ok(m, n, k, A, rsA, csA, B, rsB, csB, rsC, ab, bb)) {
+      hipLaunchKernelGGL(sgemm_dma_kernel, dim3((unsigned)(ceil_div(m, 128) * ceil_div(n, 128))),
+                         dim3(256), 0, (hipStream_t)stream, m, n, k, *alpha, A, csA, ab, B, csB,
+                         bb, *beta, C, csC);
+      return hip_status(fn, hipGetLastError());
+    }
+  }
   if (!std::is_same<T, float>::value && ceil_div(m, 256) * ceil_div(n, 256) >= 128 && k >= 256)
     return hip_status(fn, launch_bf16_panels(m, n, k, *alpha, reinterpret_cast<const uint16_t*>(A),
                                              rsA, csA, reinterpret_cast<const uint16_t*>(B), rsB,

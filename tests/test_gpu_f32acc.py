@@ -95,3 +95,28 @@ def test_bf16_packed_panel_form(m, n, k, layout):
     capi.gemm_bf16(A, B, C, -0.75, 2.0)
     torch.cuda.synchronize()
     check(A, B, C0, C, -0.75, 2.0, k)
+
+
+@pytest.mark.parametrize("m,n,k,ld_pad", [(2100, 2050, 1000, 0), (2048, 2176, 31, 0),
+                                          (2052, 2048, 4097, 0), (2100, 2049, 513, 12),
+                                          (2049, 2048, 300, 0)])
+def test_sgemm_dma_form(m, n, k, ld_pad):
+    """Column-major FP32 shapes with at least 256 tiles of 128 x 128: the DMA
+    tile (leading dimensions multiples of 4) with ragged m, n, k (partial last
+    slab), padded leading dimensions, and the fallback when a leading
+    dimension is not a multiple of 4 (m = 2049)."""
+    print(f"sgemm dma {m}x{n}x{k} ld+{ld_pad}", flush=True)
+    rng = np.random.default_rng(SEED + m + n + k)
+
+    def colmajor(h, w):
+        big = torch.empty((w, h + ld_pad), dtype=torch.float32, device="cuda")
+        t = big.T[:h, :]
+        t.copy_(torch.from_numpy(rng.uniform(-1, 1, (h, w))))
+        return t
+
+    A, B, C0 = colmajor(m, k), colmajor(k, n), colmajor(m, n)
+    C = C0.clone() if ld_pad == 0 else colmajor(m, n)
+    C.copy_(C0)
+    capi.sgemm(A, B, C, 1.25, -0.5)
+    torch.cuda.synchronize()
+    check(A, B, C0, C, 1.25, -0.5, k)
