@@ -21,7 +21,7 @@ moduli, N residue GEMMs). Prints ONE JSON line:
                pack / unpack-add) unless --quick
   cpu_baseline the restated Goto + 6x8 CPU path on this host's cores
 
---only {plain,split,multiply,crt,batched,batched_plain,copy,hier} runs one kernel family in a loop
+--only {plain,split,multiply,crt,batched,batched_plain,copy,hier,fp32} runs one kernel family in a loop
 (for profilers) and prints nothing else but its timing line.
 """
 from __future__ import annotations
@@ -280,7 +280,7 @@ def main() -> int:
     ap.add_argument("--tile", type=int, default=None, help="FP64 tile configuration (--only plain)")
     ap.add_argument("--moduli", type=int, default=14, help="moduli of the residue form (--only crt)")
     ap.add_argument("--only", choices=["plain", "split", "multiply", "crt", "batched",
-                                       "batched_plain", "copy", "hier"])
+                                       "batched_plain", "copy", "hier", "fp32"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported. This engine "
@@ -330,6 +330,13 @@ def main() -> int:
                        args.steps, args.warmup)
         print(json.dumps({"only": "crt", "moduli": args.moduli, "ms": ms,
                           "tflops": flops / ms / 1e9}))
+        return 0
+    if args.only == "fp32":
+        A32, B32 = A.float(), B.float()
+        C32 = torch.zeros((n, n), device="cuda", dtype=torch.float32).T
+        del A, B, C0, C
+        ms = time_loop(lambda: capi.sgemm(A32, B32, C32, 1.0, 1.0), args.steps, args.warmup)
+        print(json.dumps({"only": "fp32", "ms": ms, "tflops": flops / ms / 1e9}))
         return 0
     if args.only == "hier":
         del A, B, C0, C
