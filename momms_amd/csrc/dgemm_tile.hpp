@@ -314,7 +314,9 @@ namespace momms {
 // columns >= n of B may read neighbouring values, which only reach outputs
 // that are not stored; the k >= K entries of the last slab are zeroed in LDS
 // after they land.
-// Same per-element summation order as dgemm_tile (k4 steps in slab order).
+// Per element, the k-values of a slab are summed in the order 0, 2, 4, 6, 1,
+// 3, 5, 7, 8, 10, ... (MFMA steps 2u + t cover 8u + 2lk + t over the four
+// lane groups lk); slabs in order.
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, uint32_t bytes) {
   return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), 0, bytes, 0x00020000);
@@ -345,7 +347,7 @@ template <int STAGES>
 struct DgemmDmaSmem {
   static constexpr int BM = 128, BN = 128, BK = 16, LDA = BM + 16;
   double A[STAGES][BK][LDA];
-  double B[STAGES][BN * BK];
+  __attribute__((aligned(16))) double B[STAGES][BN * BK];
 };
 
 template <int STAGES>
@@ -409,21 +411,30 @@ __device__ __forceinline__ void dgemm_tile_dma(DgemmDmaSmem<STAGES>& sm, int64_t
       lds_barrier();
     }
     if (kt + STAGES - 1 < ktiles) issue(kt + STAGES - 1, (int)((kt + STAGES - 1) % STAGES));
+    // k order inside the slab: MFMA step 2u + t gives lane group lk the
+    // k-value 8u + 2lk + t, so a lane's B values for two steps are its
+    // column's k-pair 4u + lk, one 16-byte read (the 8 columns of a group of
+    // 8 lanes sit in distinct banks)
 #pragma unroll
-    for (int k4 = 0; k4 < BK / 4; ++k4) {
-      const int kk = k4 * 4 + lk;
-      double af[FM], bf[FN];
-#pragma unroll
-      for (int a = 0; a < FM; ++a) af[a] = sm.A[cur][kk][wm * WTM + a * 16 + lr];
+    for (int u = 0; u < BK / 8; ++u) {
+      const int kp = 4 * u + lk;
+      v2f64 bp[FN];
 #pragma unroll
       for (int b = 0; b < FN; ++b) {
-        const int j = wn * WTN + b * 16 + lr, c = j & 7, kp = kk >> 1;
-        bf[b] = sm.B[cur][(j >> 3) * 128 + (c * 8 + (kp ^ c)) * 2 + (kk & 1)];
+        const int j = wn * WTN + b * 16 + lr, c = j & 7;
+        bp[b] = *reinterpret_cast<const v2f64*>(&sm.B[cur][(j >> 3) * 128 + (c * 8 + (kp ^ c)) * 2]);
       }
 #pragma unroll
-      for (int a = 0; a < FM; ++a)
+      for (int t = 0; t < 2; ++t) {
+        const int kk = 2 * kp + t;
+        double af[FM];
 #pragma unroll
-        for (int b = 0; b < FN; ++b) acc[a][b] = mfma_f64_16x16x4(bf[b], af[a], acc[a][b]);
+        for (int a = 0; a < FM; ++a) af[a] = sm.A[cur][kk][wm * WTM + a * 16 + lr];
+#pragma unroll
+        for (int a = 0; a < FM; ++a)
+#pragma unroll
+          for (int b = 0; b < FN; ++b) acc[a][b] = mfma_f64_16x16x4(bp[b][t], af[a], acc[a][b]);
+      }
     }
   }
   lds_barrier();  // LDS free for the next tile of a persistent caller

@@ -17,6 +17,7 @@
 namespace momms {
 
 typedef double v4f64 __attribute__((ext_vector_type(4)));
+typedef double v2f64 __attribute__((ext_vector_type(2)));
 typedef float v4f32 __attribute__((ext_vector_type(4)));
 typedef float v16f32 __attribute__((ext_vector_type(16)));
 typedef __bf16 v8bf16 __attribute__((ext_vector_type(8)));
