@@ -349,8 +349,9 @@ crt_reconstruct(int64_t m, int64_t n, int64_t k, int beta_bits, double alpha, do
 // ---------------------------------------------------------------------------
 // host side
 
-constexpr int kCrtFM = 2, kCrtFN = 4, kCrtWM = 4, kCrtWN = 2, kCrtSPS = 2;
+constexpr int kCrtFM = 2, kCrtFN = 4, kCrtWM = 4, kCrtWN = 2, kCrtSPS = 4;
 static_assert(32 * kCrtFM * kCrtWM == kCrtBM && 32 * kCrtFN * kCrtWN == kCrtBN, "tile");
+static_assert(kCrtKAlign % (kSlab * kCrtSPS) == 0, "panels are padded to whole stages");
 
 hipError_t crt_prepare(const CrtLayout& L, int64_t m, int64_t n, int64_t k, const double* A,
                        int64_t rsA, int64_t csA, const double* B, int64_t rsB, int64_t csB,

@@ -333,7 +333,7 @@ static bool sgemm_dma_ok(int64_t m, int64_t n, int64_t k, const float* A, int64_
 // BF16 in, FP32 out, large problems: both operands packed into 256-wide
 // panels (pack_bf16_panels, into library scratch), then the DMA-ring panel
 // GEMM of panel_gemm.hpp with a 256 x 256 tile and the alpha / beta epilogue.
-constexpr int kBfFM = 2, kBfFN = 4, kBfWM = 4, kBfWN = 2, kBfSPS = 2;
+constexpr int kBfFM = 2, kBfFN = 4, kBfWM = 4, kBfWN = 2, kBfSPS = 4;
 typedef PanelGemm<kBfFM, kBfFN, kBfWM, kBfWN, kBfSPS> BfPanel;
 
 __global__ void __launch_bounds__(64 * kBfWM * kBfWN, 1)

@@ -9,12 +9,19 @@
 // Workgroup: WM x WN wavefronts, each FM x FN blocks of 32 x 32 (tile BM x BN,
 // BM = 32*FM*WM, BN = 32*FN*WN). A stage is SPS consecutive slabs; stages
 // move HBM -> LDS by global_load_lds_dwordx4 (1 KiB per wavefront load, no
-// register staging) into two LDS buffers that alternate, the next stage in
-// flight while the current one is multiplied. The two buffers are distinct
-// __shared__ objects and the loop is unrolled by two, so every LDS read names
-// a buffer no DMA in flight writes to: the compiler then needs no wait for
-// the DMA before the reads (with one ring array indexed at run time it must
-// wait for every DMA in flight, which serializes loads and multiplies).
+// register staging) into two LDS buffers that alternate. The two buffers are
+// distinct __shared__ objects and the loop is unrolled by two, so every LDS
+// read names a buffer no DMA in flight writes to: the compiler then needs no
+// wait for the DMA before the reads (with one ring array indexed at run time
+// it must wait for every DMA in flight, which serializes loads and
+// multiplies).
+// Fragments are software-pipelined: a wavefront holds two register sets and
+// reads the fragments of slab s+1 between the MFMAs of slab s (one read after
+// each MFMA), so LDS latency hides behind the matrix cores. The stage barrier
+// sits before the last slab's MFMAs: after it the next stage's DMA is issued
+// into the buffer everyone has finished reading, and the first fragments of
+// the next stage are read while the last slab multiplies. The MFMA order per
+// accumulator is slab order, as in any k loop over the same panels.
 // The MFMA takes the B block as its "A" operand, so lane l holds row
 // i = l%32 of a block and columns 8(r/4) + 4(l/32) + r%4: column-major
 // stores along i are coalesced.
@@ -35,30 +42,6 @@ struct PanelGemm {
   static constexpr int KALIGN = kSlab * SPS;  // k padding of the panels
 };
 
-// The k loop: acc += A_panel * B_panel over nslabs slabs (a multiple of SPS).
-// With MOD, every 256 k-values (a chunk) each accumulator x becomes
-// x - p rint(x / p) (exact by fma for |x| < 2^24).
-#define MOMMS_PANEL_STAGE(KS, Ac, Bc, An, Bn)                                                \
-  {                                                                                          \
-    split_wait_vmcnt<0>(); /* this wavefront's part of stage KS has landed */                \
-    split_barrier();       /* everyone's part has; everyone is done with the other buffer */ \
-    if ((KS) + 1 < nst) issue((KS) + 1, An, Bn);                                             \
-    _Pragma("unroll") for (int ss = 0; ss < SPS; ++ss) {                                     \
-      v8bf16 af[FM], bf[FN];                                                                 \
-      _Pragma("unroll") for (int a = 0; a < FM; ++a) af[a] =                                 \
-          *reinterpret_cast<const v8bf16*>(&Ac[ss * G::ABLK + a_off + a * 32 * 8]);          \
-      _Pragma("unroll") for (int b = 0; b < FN; ++b) bf[b] =                                 \
-          *reinterpret_cast<const v8bf16*>(&Bc[ss * G::BBLK + b_off + b * 32 * 8]);          \
-      _Pragma("unroll") for (int a = 0; a < FM; ++a)                                         \
-          _Pragma("unroll") for (int b = 0; b < FN; ++b) acc[a][b] =                         \
-              mfma_bf16_32x32x16(bf[b], af[a], acc[a][b]);                                   \
-    }                                                                                        \
-    if (MOD && ++in_chunk == CHUNK_STAGES) {                                                 \
-      panel_reduce_mod<FM, FN>(acc, p, invp);                                                \
-      in_chunk = 0;                                                                          \
-    }                                                                                        \
-  }
-
 template <int FM, int FN>
 __device__ __forceinline__ void panel_reduce_mod(v16f32 (&acc)[FM][FN], float p, float invp) {
 #pragma unroll
@@ -71,6 +54,49 @@ __device__ __forceinline__ void panel_reduce_mod(v16f32 (&acc)[FM][FN], float p,
         acc[a][b][r] = __builtin_fmaf(-__builtin_rintf(x * invp), p, x);
       }
 }
+
+// One slab's MFMAs with the next slab's fragment reads placed between them:
+// MFMA, then (read, MFMA) pairs, then the remaining MFMAs. The reads name the
+// other register set, so only the waits at the region's start (for reads of
+// the previous region) remain.
+template <int FM, int FN>
+__device__ __forceinline__ void panel_slab_order() {
+  __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+#pragma unroll
+  for (int q = 0; q < FM + FN; ++q) {
+    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);  // DS read
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);  // MFMA
+  }
+  if constexpr (FM * FN > 1 + FM + FN)
+    __builtin_amdgcn_sched_group_barrier(0x008, FM * FN - 1 - FM - FN, 0);
+}
+
+// The k loop: acc += A_panel * B_panel over nslabs slabs (a multiple of SPS).
+// With MOD, every 256 k-values (a chunk) each accumulator x becomes
+// x - p rint(x / p) (exact by fma for |x| < 2^24).
+#define MOMMS_PANEL_STAGE(KS, Ac, Bc, An, Bn)                                     \
+  {                                                                               \
+    _Pragma("unroll") for (int ss = 0; ss < SPS; ++ss) {                          \
+      if (ss + 1 < SPS) {                                                         \
+        rd(Ac, Bc, ss + 1, an, bn);                                               \
+      } else {                                                                    \
+        __builtin_amdgcn_sched_barrier(0);                                        \
+        split_wait_vmcnt<0>(); /* this wavefront's part of stage KS+1 landed */   \
+        split_barrier();       /* everyone's has; everyone is done reading KS */  \
+        if ((KS) + 2 < nst) issue((KS) + 2, Ac, Bc);                              \
+        rd(An, Bn, 0, an, bn); /* past the last stage: unused, in bounds */       \
+      }                                                                           \
+      mm(af, bf);                                                                 \
+      panel_slab_order<FM, FN>();                                                 \
+      __builtin_amdgcn_sched_barrier(0);                                          \
+      _Pragma("unroll") for (int a = 0; a < FM; ++a) af[a] = an[a];               \
+      _Pragma("unroll") for (int b = 0; b < FN; ++b) bf[b] = bn[b];               \
+    }                                                                             \
+    if (MOD && ++in_chunk == CHUNK_STAGES) {                                      \
+      panel_reduce_mod<FM, FN>(acc, p, invp);                                     \
+      in_chunk = 0;                                                               \
+    }                                                                             \
+  }
 
 template <int FM, int FN, int WM, int WN, int SPS, bool MOD>
 __device__ __forceinline__ void panel_mainloop(v16f32 (&acc)[FM][FN],
@@ -101,8 +127,34 @@ __device__ __forceinline__ void panel_mainloop(v16f32 (&acc)[FM][FN],
   const int kh = lane >> 5;
   const int a_off = kh * G::BM * 8 + (wm * 32 * FM + (lane & 31)) * 8;
   const int b_off = kh * G::BN * 8 + (wn * 32 * FN + (lane & 31)) * 8;
+  v8bf16 af[FM], bf[FN], an[FM], bn[FN];
+  auto rd = [&](const uint16_t* Ac, const uint16_t* Bc, int ss, v8bf16 (&xa)[FM],
+                v8bf16 (&xb)[FN]) {
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+      xa[a] = *reinterpret_cast<const v8bf16*>(&Ac[ss * G::ABLK + a_off + a * 32 * 8]);
+#pragma unroll
+    for (int b = 0; b < FN; ++b)
+      xb[b] = *reinterpret_cast<const v8bf16*>(&Bc[ss * G::BBLK + b_off + b * 32 * 8]);
+  };
+  auto mm = [&](const v8bf16 (&xa)[FM], const v8bf16 (&xb)[FN]) {
+#pragma unroll
+    for (int a = 0; a < FM; ++a)
+#pragma unroll
+      for (int b = 0; b < FN; ++b) acc[a][b] = mfma_bf16_32x32x16(xb[b], xa[a], acc[a][b]);
+  };
 
-  if (nst > 0) issue(0, As0, Bs0);
+  if (nst <= 0) return;
+  // prologue: stages 0 and 1 in flight, wait for stage 0, first fragments
+  issue(0, As0, Bs0);
+  if (nst > 1) {
+    issue(1, As1, Bs1);
+    split_wait_vmcnt<G::LW>();
+  } else {
+    split_wait_vmcnt<0>();
+  }
+  split_barrier();
+  rd(As0, Bs0, 0, af, bf);
   int in_chunk = 0;
   (void)in_chunk;
   (void)p;
