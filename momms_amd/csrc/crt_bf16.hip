@@ -223,10 +223,10 @@ crt_multiply(int64_t tiles_m, int64_t tiles_n, int64_t nslabs, int64_t mpad, int
              const uint16_t* __restrict__ RA, const uint16_t* __restrict__ RB,
              int16_t* __restrict__ RC) {
   typedef PanelGemm<FM, FN, WM, WN, SPS> G;
-  __shared__ __attribute__((aligned(16))) uint16_t As0[G::ABLK * SPS];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs0[G::BBLK * SPS];
-  __shared__ __attribute__((aligned(16))) uint16_t As1[G::ABLK * SPS];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs1[G::BBLK * SPS];
+  __shared__ __attribute__((aligned(16))) uint16_t S0[G::SLOT];
+  __shared__ __attribute__((aligned(16))) uint16_t S1[G::SLOT];
+  __shared__ __attribute__((aligned(16))) uint16_t S2[G::SLOT];
+  __shared__ __attribute__((aligned(16))) uint16_t S3[G::SLOT];
 
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave % WM, wn = wave / WM, kh = lane >> 5;
@@ -247,7 +247,7 @@ crt_multiply(int64_t tiles_m, int64_t tiles_n, int64_t nslabs, int64_t mpad, int
       for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
   panel_mainloop<FM, FN, WM, WN, SPS, true>(
       acc, RA + (int64_t)l * mpad * kpad + tm * nslabs * G::ABLK,
-      RB + (int64_t)l * npad * kpad + tn * nslabs * G::BBLK, nslabs, p, invp, As0, Bs0, As1, Bs1);
+      RB + (int64_t)l * npad * kpad + tn * nslabs * G::BBLK, nslabs, p, invp, S0, S1, S2, S3);
   panel_reduce_mod<FM, FN>(acc, p, invp);
 
   // balanced residue in [-(p-1)/2, (p-1)/2]
@@ -349,7 +349,7 @@ crt_reconstruct(int64_t m, int64_t n, int64_t k, int beta_bits, double alpha, do
 // ---------------------------------------------------------------------------
 // host side
 
-constexpr int kCrtFM = 2, kCrtFN = 4, kCrtWM = 4, kCrtWN = 2, kCrtSPS = 4;
+constexpr int kCrtFM = 2, kCrtFN = 4, kCrtWM = 4, kCrtWN = 2, kCrtSPS = 2;
 static_assert(32 * kCrtFM * kCrtWM == kCrtBM && 32 * kCrtFN * kCrtWN == kCrtBN, "tile");
 static_assert(kCrtKAlign % (kSlab * kCrtSPS) == 0, "panels are padded to whole stages");
 

@@ -10,7 +10,7 @@
 namespace momms {
 
 constexpr int kCrtBM = 256, kCrtBN = 256;  // residue GEMM tile = panel sizes
-constexpr int kCrtKAlign = 64;             // k padding: whole four-slab stages
+constexpr int kCrtKAlign = 32;             // k padding: whole two-slab stages
 
 // Workspace layout for one (m, n, k, N) problem: packed residues of A and B,
 // the int16 residue matrices of C' (one per modulus), exponents and scratch.

@@ -333,7 +333,7 @@ static bool sgemm_dma_ok(int64_t m, int64_t n, int64_t k, const float* A, int64_
 // BF16 in, FP32 out, large problems: both operands packed into 256-wide
 // panels (pack_bf16_panels, into library scratch), then the DMA-ring panel
 // GEMM of panel_gemm.hpp with a 256 x 256 tile and the alpha / beta epilogue.
-constexpr int kBfFM = 2, kBfFN = 4, kBfWM = 4, kBfWN = 2, kBfSPS = 4;
+constexpr int kBfFM = 2, kBfFN = 4, kBfWM = 4, kBfWN = 2, kBfSPS = 2;
 typedef PanelGemm<kBfFM, kBfFN, kBfWM, kBfWN, kBfSPS> BfPanel;
 
 __global__ void __launch_bounds__(64 * kBfWM * kBfWN, 1)
@@ -342,10 +342,10 @@ bf16_panel_gemm(int64_t m, int64_t n, int64_t tiles_m, int64_t tiles_n, int64_t 
                 float beta, float* __restrict__ C, int64_t rsC, int64_t csC) {
   typedef BfPanel G;
   constexpr int FM = kBfFM, FN = kBfFN, WM = kBfWM;
-  __shared__ __attribute__((aligned(16))) uint16_t As0[G::ABLK * kBfSPS];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs0[G::BBLK * kBfSPS];
-  __shared__ __attribute__((aligned(16))) uint16_t As1[G::ABLK * kBfSPS];
-  __shared__ __attribute__((aligned(16))) uint16_t Bs1[G::BBLK * kBfSPS];
+  __shared__ __attribute__((aligned(16))) uint16_t S0[G::SLOT];
+  __shared__ __attribute__((aligned(16))) uint16_t S1[G::SLOT];
+  __shared__ __attribute__((aligned(16))) uint16_t S2[G::SLOT];
+  __shared__ __attribute__((aligned(16))) uint16_t S3[G::SLOT];
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave % WM, wn = wave / WM, kh = lane >> 5;
   const int64_t t = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
@@ -360,7 +360,7 @@ bf16_panel_gemm(int64_t m, int64_t n, int64_t tiles_m, int64_t tiles_n, int64_t 
       for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
   panel_mainloop<FM, FN, kBfWM, kBfWN, kBfSPS, false>(acc, PA + tm * nslabs * G::ABLK,
                                                       PB + tn * nslabs * G::BBLK, nslabs, 0.0f,
-                                                      0.0f, As0, Bs0, As1, Bs1);
+                                                      0.0f, S0, S1, S2, S3);
 #pragma unroll
   for (int a = 0; a < FM; ++a) {
     const int64_t gi = tm * G::BM + wm * 32 * FM + a * 32 + (lane & 31);

@@ -7,20 +7,25 @@
 // of bf16 (kh = which 8-wide half of the slab); panels are slab-contiguous.
 //
 // Workgroup: WM x WN wavefronts, each FM x FN blocks of 32 x 32 (tile BM x BN,
-// BM = 32*FM*WM, BN = 32*FN*WN). A stage is SPS consecutive slabs; stages
-// move HBM -> LDS by global_load_lds_dwordx4 (1 KiB per wavefront load, no
-// register staging) into two LDS buffers that alternate. The two buffers are
-// distinct __shared__ objects and the loop is unrolled by two, so every LDS
-// read names a buffer no DMA in flight writes to: the compiler then needs no
-// wait for the DMA before the reads (with one ring array indexed at run time
-// it must wait for every DMA in flight, which serializes loads and
-// multiplies).
+// BM = 32*FM*WM, BN = 32*FN*WN). A stage is SPS consecutive slabs of A and
+// B; stages move HBM -> LDS by global_load_lds_dwordx4 (1 KiB per wavefront
+// load, no register staging) through a ring of four LDS slots (distinct
+// __shared__ objects, A then B in each), three stages in flight while one is
+// multiplied: at stage KS's barrier every wavefront has finished reading slot
+// KS % 4, and the DMA of stage KS + 4 is issued into it.
+// The DMA is issued from inline assembly, so the compiler does not see an LDS
+// store in flight and adds no waits of its own before the fragment reads (it
+// would otherwise wait for every DMA before any read once the ring has more
+// slots than its alias tracking follows); the loop waits explicitly: before
+// stage KS + 1 is read, vmcnt <= 2 stages (KS + 2, KS + 3 may be in flight),
+// then a barrier. Past the last stage the ring is kept full with reloads of
+// stage 0 into slots nobody reads again, so every wait has the same count;
+// the loop ends with vmcnt(0), so no DMA outlives the workgroup's LDS.
 // Fragments are software-pipelined: a wavefront holds two register sets and
 // reads the fragments of slab s+1 between the MFMAs of slab s (one read after
 // each MFMA), so LDS latency hides behind the matrix cores. The stage barrier
-// sits before the last slab's MFMAs: after it the next stage's DMA is issued
-// into the buffer everyone has finished reading, and the first fragments of
-// the next stage are read while the last slab multiplies. The MFMA order per
+// sits before the last slab's MFMAs, and the first fragments of the next
+// stage are read while the last slab multiplies. The MFMA order per
 // accumulator is slab order, as in any k loop over the same panels.
 // The MFMA takes the B block as its "A" operand, so lane l holds row
 // i = l%32 of a block and columns 8(r/4) + 4(l/32) + r%4: column-major
@@ -40,7 +45,27 @@ struct PanelGemm {
   static_assert(LOADS % NW == 0, "every wavefront issues the same number of loads");
   static constexpr int LW = LOADS / NW;
   static constexpr int KALIGN = kSlab * SPS;  // k padding of the panels
+  static constexpr int SLOT = (ABLK + BBLK) * SPS;  // bf16 per LDS ring slot: A, then B
+  static constexpr int BOFF = ABLK * SPS;           // B's offset in a slot
 };
+
+// global -> LDS DMA of 16 bytes per lane (a wavefront writes 1 KiB at lds,
+// lane l at lds + 16 l), issued so that the compiler does not track it: the
+// caller waits for it (vmcnt) before any wavefront reads the destination.
+// No VGPR is written by the instruction. m0 is declared clobbered; the
+// compiler reserves m0 and warns that it may hold a value of its own across
+// the statement: the kernels built on this loop keep nothing in m0 (their
+// only m0 writes are these statements, each setting it right before its use).
+#pragma clang diagnostic push
+#pragma clang diagnostic ignored "-Winline-asm"
+__device__ __forceinline__ void lds_dma16_untracked(const void* src, const void* lds) {
+  const uint32_t m0v = __builtin_amdgcn_readfirstlane((uint32_t)(uintptr_t)lds);
+  asm volatile("s_mov_b32 m0, %1\n\ts_nop 0\n\tglobal_load_lds_dwordx4 %0, off"
+               :
+               : "v"(src), "s"(m0v)
+               : "memory", "m0");
+}
+#pragma clang diagnostic pop
 
 template <int FM, int FN>
 __device__ __forceinline__ void panel_reduce_mod(v16f32 (&acc)[FM][FN], float p, float invp) {
@@ -74,17 +99,17 @@ __device__ __forceinline__ void panel_slab_order() {
 // The k loop: acc += A_panel * B_panel over nslabs slabs (a multiple of SPS).
 // With MOD, every 256 k-values (a chunk) each accumulator x becomes
 // x - p rint(x / p) (exact by fma for |x| < 2^24).
-#define MOMMS_PANEL_STAGE(KS, Ac, Bc, An, Bn)                                     \
+#define MOMMS_PANEL_STAGE(KS, Sc, Sn)                                             \
   {                                                                               \
     _Pragma("unroll") for (int ss = 0; ss < SPS; ++ss) {                          \
       if (ss + 1 < SPS) {                                                         \
-        rd(Ac, Bc, ss + 1, an, bn);                                               \
+        rd(Sc, ss + 1, an, bn);                                                   \
       } else {                                                                    \
         __builtin_amdgcn_sched_barrier(0);                                        \
-        split_wait_vmcnt<0>(); /* this wavefront's part of stage KS+1 landed */   \
-        split_barrier();       /* everyone's has; everyone is done reading KS */  \
-        if ((KS) + 2 < nst) issue((KS) + 2, Ac, Bc);                              \
-        rd(An, Bn, 0, an, bn); /* past the last stage: unused, in bounds */       \
+        split_wait_vmcnt<2 * G::LW>(); /* this wavefront's stage KS+1 landed */   \
+        split_barrier(); /* everyone's has; everyone is done reading slot KS */   \
+        issue((KS) + 4, Sc);                                                      \
+        rd(Sn, 0, an, bn); /* past the last stage: unused */                      \
       }                                                                           \
       mm(af, bf);                                                                 \
       panel_slab_order<FM, FN>();                                                 \
@@ -98,13 +123,13 @@ __device__ __forceinline__ void panel_slab_order() {
     }                                                                             \
   }
 
+// S0..S3: the four ring slots, each G::SLOT bf16 of distinct __shared__ storage.
 template <int FM, int FN, int WM, int WN, int SPS, bool MOD>
 __device__ __forceinline__ void panel_mainloop(v16f32 (&acc)[FM][FN],
                                                const uint16_t* __restrict__ Abase,
                                                const uint16_t* __restrict__ Bbase,
-                                               int64_t nslabs, float p, float invp,
-                                               uint16_t* As0, uint16_t* Bs0, uint16_t* As1,
-                                               uint16_t* Bs1) {
+                                               int64_t nslabs, float p, float invp, uint16_t* S0,
+                                               uint16_t* S1, uint16_t* S2, uint16_t* S3) {
   typedef PanelGemm<FM, FN, WM, WN, SPS> G;
   static_assert(!MOD || (256 / kSlab) % SPS == 0, "a chunk is whole stages");
   constexpr int CHUNK_STAGES = 256 / kSlab / SPS;
@@ -112,30 +137,30 @@ __device__ __forceinline__ void panel_mainloop(v16f32 (&acc)[FM][FN],
   const int wm = wave % WM, wn = wave / WM;
   const int64_t nst = nslabs / SPS;
 
-  auto issue = [&](int64_t kst, uint16_t* Ad, uint16_t* Bd) {
+  auto issue = [&](int64_t kst, uint16_t* S) {
+    if (kst >= nst) kst = 0;  // keeps the ring full: stage 0 again, into a slot no one reads
 #pragma unroll
     for (int i = 0; i < G::LW; ++i) {
       const int c = wave + G::NW * i;
       if (c < G::LA)
-        lds_dma16(Abase + kst * SPS * G::ABLK + c * 512 + lane * 8, &Ad[c * 512]);
+        lds_dma16_untracked(Abase + kst * SPS * G::ABLK + c * 512 + lane * 8, &S[c * 512]);
       else
-        lds_dma16(Bbase + kst * SPS * G::BBLK + (c - G::LA) * 512 + lane * 8,
-                  &Bd[(c - G::LA) * 512]);
+        lds_dma16_untracked(Bbase + kst * SPS * G::BBLK + (c - G::LA) * 512 + lane * 8,
+                            &S[G::BOFF + (c - G::LA) * 512]);
     }
   };
 
   const int kh = lane >> 5;
   const int a_off = kh * G::BM * 8 + (wm * 32 * FM + (lane & 31)) * 8;
-  const int b_off = kh * G::BN * 8 + (wn * 32 * FN + (lane & 31)) * 8;
+  const int b_off = G::BOFF + kh * G::BN * 8 + (wn * 32 * FN + (lane & 31)) * 8;
   v8bf16 af[FM], bf[FN], an[FM], bn[FN];
-  auto rd = [&](const uint16_t* Ac, const uint16_t* Bc, int ss, v8bf16 (&xa)[FM],
-                v8bf16 (&xb)[FN]) {
+  auto rd = [&](const uint16_t* S, int ss, v8bf16 (&xa)[FM], v8bf16 (&xb)[FN]) {
 #pragma unroll
     for (int a = 0; a < FM; ++a)
-      xa[a] = *reinterpret_cast<const v8bf16*>(&Ac[ss * G::ABLK + a_off + a * 32 * 8]);
+      xa[a] = *reinterpret_cast<const v8bf16*>(&S[ss * G::ABLK + a_off + a * 32 * 8]);
 #pragma unroll
     for (int b = 0; b < FN; ++b)
-      xb[b] = *reinterpret_cast<const v8bf16*>(&Bc[ss * G::BBLK + b_off + b * 32 * 8]);
+      xb[b] = *reinterpret_cast<const v8bf16*>(&S[ss * G::BBLK + b_off + b * 32 * 8]);
   };
   auto mm = [&](const v8bf16 (&xa)[FM], const v8bf16 (&xb)[FN]) {
 #pragma unroll
@@ -145,26 +170,26 @@ __device__ __forceinline__ void panel_mainloop(v16f32 (&acc)[FM][FN],
   };
 
   if (nst <= 0) return;
-  // prologue: stages 0 and 1 in flight, wait for stage 0, first fragments
-  issue(0, As0, Bs0);
-  if (nst > 1) {
-    issue(1, As1, Bs1);
-    split_wait_vmcnt<G::LW>();
-  } else {
-    split_wait_vmcnt<0>();
-  }
+  // prologue: stages 0..3 in flight, wait for stage 0, its first fragments
+  issue(0, S0);
+  issue(1, S1);
+  issue(2, S2);
+  issue(3, S3);
+  split_wait_vmcnt<3 * G::LW>();
   split_barrier();
-  rd(As0, Bs0, 0, af, bf);
+  rd(S0, 0, af, bf);
   int in_chunk = 0;
   (void)in_chunk;
   (void)p;
   (void)invp;
-  for (int64_t ks = 0; ks < nst; ks += 2) {
-    MOMMS_PANEL_STAGE(ks, As0, Bs0, As1, Bs1)
-    if (ks + 1 < nst) MOMMS_PANEL_STAGE(ks + 1, As1, Bs1, As0, Bs0)
+  for (int64_t ks = 0; ks < nst; ks += 4) {
+    MOMMS_PANEL_STAGE(ks, S0, S1)
+    if (ks + 1 < nst) MOMMS_PANEL_STAGE(ks + 1, S1, S2)
+    if (ks + 2 < nst) MOMMS_PANEL_STAGE(ks + 2, S2, S3)
+    if (ks + 3 < nst) MOMMS_PANEL_STAGE(ks + 3, S3, S0)
   }
+  split_wait_vmcnt<0>();  // the ring's last DMAs (reloads) have landed
 }
-#undef MOMMS_PANEL_STAGE
 
 // Packs count vectors of length len (vector v, element p at X[v*sv + p*sp],
 // bf16 bit patterns) into the panel layout, padding with zeros to vpad
