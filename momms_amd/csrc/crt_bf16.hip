@@ -109,9 +109,10 @@ CrtLayout crt_layout(int64_t m, int64_t n, int64_t k, int N) {
   L.kpad = ceil_div(k > 0 ? k : 1, kCrtKAlign) * kCrtKAlign;
   size_t off = 0;
   L.off_ra = off;
-  off = align256(off + (size_t)N * L.mpad * L.kpad * 2);
+  const int64_t kstride = panel_stride(L.kpad / kSlab) * kSlab;  // k-values per panel
+  off = align256(off + (size_t)N * L.mpad * kstride * 2);
   L.off_rb = off;
-  off = align256(off + (size_t)N * L.npad * L.kpad * 2);
+  off = align256(off + (size_t)N * L.npad * kstride * 2);
   L.off_rc = off;
   off = align256(off + (size_t)N * L.mpad * L.npad * 2);
   L.off_ea = off;
@@ -131,8 +132,9 @@ CrtLayout crt_layout(int64_t m, int64_t n, int64_t k, int N) {
 //
 // Vector v (a row of A or a column of B), element p at X[v*sv + p*sp]. For
 // modulus l the residue goes to
-//   R[l*vpad*lpad + (pv*nslabs + ks)*(PANEL*16) + kh*PANEL*8 + (v%PANEL)*8 + p%8]
-// with pv = v / PANEL, ks = p / 16, kh = (p % 16) / 8; padding is 0. A 64 x 64
+//   R[(l*(vpad/PANEL) + pv)*S*(PANEL*16) + ks*(PANEL*16) + kh*PANEL*8 + (v%PANEL)*8 + p%8]
+// with pv = v / PANEL, ks = p / 16, kh = (p % 16) / 8 and S = panel_stride(lpad/16)
+// slabs per panel (one unused slab between panels); padding is 0. A 64 x 64
 // block of (v, p) goes through LDS so the global read (along whichever index
 // is unit-stride) and the 16-byte packed writes are both coalesced.
 
@@ -173,7 +175,7 @@ crt_pack(const double* __restrict__ X, int64_t count, int64_t len, int64_t sv, i
   }
   __syncthreads();
   const int64_t nslabs = lpad / kSlab;
-  const int64_t mod_stride = vpad * lpad;
+  const int64_t mod_stride = vpad * panel_stride(nslabs) * kSlab;
 #pragma unroll
   for (int r = 0; r < 2; ++r) {
     const int c = tid + 256 * r;
@@ -189,8 +191,8 @@ crt_pack(const double* __restrict__ X, int64_t count, int64_t len, int64_t sv, i
     for (int e = 0; e < 8; ++e) q[e] = finite ? rint(ldexp(tile[vv][pc * 8 + e], beta - E)) : 0.0;
     const int64_t pv = gv / PANEL, ks = gp / kSlab;
     const int kh = (int)((gp % kSlab) / 8);
-    uint16_t* dst = R + (pv * nslabs + ks) * (PANEL * kSlab) + (int64_t)kh * PANEL * 8 +
-                    (gv % PANEL) * 8;
+    uint16_t* dst = R + (pv * panel_stride(nslabs) + ks) * (PANEL * kSlab) +
+                    (int64_t)kh * PANEL * 8 + (gv % PANEL) * 8;
     for (int l = 0; l < nmod; ++l) {
       const double p = kCrt.pd[l], invp = kCrt.invpd[l], h = 0.5 * (p - 1.0);
       uint32_t bits[8];
@@ -235,7 +237,6 @@ crt_multiply(int64_t tiles_m, int64_t tiles_n, int64_t nslabs, int64_t mpad, int
   const int l = (int)(t / tiles);
   int64_t tm, tn;
   grouped_tile(t - (int64_t)l * tiles, tiles_m, tiles_n, 8, tm, tn);
-  const int64_t kpad = nslabs * kSlab;
   const float p = kCrt.pf[l], invp = kCrt.invpf[l];
 
   v16f32 acc[FM][FN];
@@ -246,8 +247,9 @@ crt_multiply(int64_t tiles_m, int64_t tiles_n, int64_t nslabs, int64_t mpad, int
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
   panel_mainloop<FM, FN, WM, WN, SPS, true>(
-      acc, RA + (int64_t)l * mpad * kpad + tm * nslabs * G::ABLK,
-      RB + (int64_t)l * npad * kpad + tn * nslabs * G::BBLK, nslabs, p, invp, S0, S1, S2, S3);
+      acc, RA + ((int64_t)l * tiles_m + tm) * panel_stride(nslabs) * G::ABLK,
+      RB + ((int64_t)l * tiles_n + tn) * panel_stride(nslabs) * G::BBLK, nslabs, p, invp, S0, S1,
+      S2, S3);
   panel_reduce_mod<FM, FN>(acc, p, invp);
 
   // balanced residue in [-(p-1)/2, (p-1)/2]

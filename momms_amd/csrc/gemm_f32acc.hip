@@ -358,8 +358,10 @@ bf16_panel_gemm(int64_t m, int64_t n, int64_t tiles_m, int64_t tiles_n, int64_t 
     for (int b = 0; b < FN; ++b)
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
-  panel_mainloop<FM, FN, kBfWM, kBfWN, kBfSPS, false>(acc, PA + tm * nslabs * G::ABLK,
-                                                      PB + tn * nslabs * G::BBLK, nslabs, 0.0f,
+  panel_mainloop<FM, FN, kBfWM, kBfWN, kBfSPS, false>(acc,
+                                                      PA + tm * panel_stride(nslabs) * G::ABLK,
+                                                      PB + tn * panel_stride(nslabs) * G::BBLK,
+                                                      nslabs, 0.0f,
                                                       0.0f, S0, S1, S2, S3);
 #pragma unroll
   for (int a = 0; a < FM; ++a) {
@@ -387,9 +389,10 @@ static hipError_t launch_bf16_panels(int64_t m, int64_t n, int64_t k, float alph
   const int64_t npad = ceil_div(n, BfPanel::BN) * BfPanel::BN;
   const int64_t kpad = ceil_div(k, BfPanel::KALIGN) * BfPanel::KALIGN;
   uint16_t* pa = nullptr;
-  hipError_t e = stream_scratch(st, (size_t)(mpad + npad) * kpad * 2, (void**)&pa);
+  const int64_t kstride = panel_stride(kpad / kSlab) * kSlab;  // k-values per panel, with the gap
+  hipError_t e = stream_scratch(st, (size_t)(mpad + npad) * kstride * 2, (void**)&pa);
   if (e != hipSuccess) return e;
-  uint16_t* pb = pa + mpad * kpad;
+  uint16_t* pb = pa + mpad * kstride;
   dim3 block(256);
   hipLaunchKernelGGL((pack_bf16_panels<BfPanel::BM>),
                      dim3((unsigned)ceil_div(mpad, 64), (unsigned)ceil_div(kpad, 64)), block, 0,

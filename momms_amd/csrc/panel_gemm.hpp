@@ -4,7 +4,13 @@
 //
 // Panel layout (written by the pack kernels): for a panel of PANEL rows of A
 // (columns of B) and 16-deep k-slab ks, one contiguous block [kh][PANEL][8]
-// of bf16 (kh = which 8-wide half of the slab); panels are slab-contiguous.
+// of bf16 (kh = which 8-wide half of the slab); the slabs of a panel are
+// contiguous, and panel pv starts at slab pv * (nslabs + 1) (panel_stride):
+// one unused slab between panels. Without it every panel is a power of two
+// long at power-of-two k, so the slabs that the concurrently running tiles
+// read at the same k all share their low address bits and, depending on the
+// physical pages, compete for the same L2 sets (measured: 2.7x the HBM
+// fetches and 6 % of the residue GEMMs' time).
 //
 // Workgroup: WM x WN wavefronts, each FM x FN blocks of 32 x 32 (tile BM x BN,
 // BM = 32*FM*WM, BN = 32*FN*WN). A stage is SPS consecutive slabs of A and
@@ -31,10 +37,15 @@
 // i = l%32 of a block and columns 8(r/4) + 4(l/32) + r%4: column-major
 // stores along i are coalesced.
 #pragma once
+#include <type_traits>
+
 #include "mfma.hpp"
 #include "split_tile.hpp"
 
 namespace momms {
+
+// slabs from one panel's start to the next (one unused slab between panels)
+__host__ __device__ constexpr int64_t panel_stride(int64_t nslabs) { return nslabs + 1; }
 
 template <int FM, int FN, int WM, int WN, int SPS>
 struct PanelGemm {
@@ -97,33 +108,16 @@ __device__ __forceinline__ void panel_slab_order() {
 }
 
 // The k loop: acc += A_panel * B_panel over nslabs slabs (a multiple of SPS).
-// With MOD, every 256 k-values (a chunk) each accumulator x becomes
-// x - p rint(x / p) (exact by fma for |x| < 2^24).
-#define MOMMS_PANEL_STAGE(KS, Sc, Sn)                                             \
-  {                                                                               \
-    _Pragma("unroll") for (int ss = 0; ss < SPS; ++ss) {                          \
-      if (ss + 1 < SPS) {                                                         \
-        rd(Sc, ss + 1, an, bn);                                                   \
-      } else {                                                                    \
-        __builtin_amdgcn_sched_barrier(0);                                        \
-        split_wait_vmcnt<2 * G::LW>(); /* this wavefront's stage KS+1 landed */   \
-        split_barrier(); /* everyone's has; everyone is done reading slot KS */   \
-        issue((KS) + 4, Sc);                                                      \
-        rd(Sn, 0, an, bn); /* past the last stage: unused */                      \
-      }                                                                           \
-      mm(af, bf);                                                                 \
-      panel_slab_order<FM, FN>();                                                 \
-      __builtin_amdgcn_sched_barrier(0);                                          \
-      _Pragma("unroll") for (int a = 0; a < FM; ++a) af[a] = an[a];               \
-      _Pragma("unroll") for (int b = 0; b < FN; ++b) bf[b] = bn[b];               \
-    }                                                                             \
-    if (MOD && ++in_chunk == CHUNK_STAGES) {                                      \
-      panel_reduce_mod<FM, FN>(acc, p, invp);                                     \
-      in_chunk = 0;                                                               \
-    }                                                                             \
-  }
-
-// S0..S3: the four ring slots, each G::SLOT bf16 of distinct __shared__ storage.
+// With MOD, each accumulator x is kept an exact FP32 integer by replacing it
+// with x - p rint(x / p) (exact by fma for |x| < 2^24) once every 256
+// k-values. The reductions are staggered: the loop is unrolled by one chunk
+// (CHUNK_STAGES stages = 256 k), and at stage j of a chunk the blocks q with
+// q * CHUNK_STAGES / (FM FN) == j are reduced before that stage's first slab,
+// whose MFMAs then start with the next block, so the VALU work overlaps the
+// other blocks' MFMAs instead of stalling both wavefronts of a SIMD at the
+// chunk end. Between two reductions of a block lie exactly 256 k-values
+// (|x| < 0.51 p + 256 * 255^2 < 2^24), and the caller reduces once more after
+// the loop, so the residues equal those of a reduction at every chunk end.
 template <int FM, int FN, int WM, int WN, int SPS, bool MOD>
 __device__ __forceinline__ void panel_mainloop(v16f32 (&acc)[FM][FN],
                                                const uint16_t* __restrict__ Abase,
@@ -131,8 +125,11 @@ __device__ __forceinline__ void panel_mainloop(v16f32 (&acc)[FM][FN],
                                                int64_t nslabs, float p, float invp, uint16_t* S0,
                                                uint16_t* S1, uint16_t* S2, uint16_t* S3) {
   typedef PanelGemm<FM, FN, WM, WN, SPS> G;
-  static_assert(!MOD || (256 / kSlab) % SPS == 0, "a chunk is whole stages");
   constexpr int CHUNK_STAGES = 256 / kSlab / SPS;
+  constexpr int NQ = FM * FN;
+  static_assert((256 / kSlab) % SPS == 0 && CHUNK_STAGES % 4 == 0 && CHUNK_STAGES <= 16,
+                "a chunk is whole stages and whole turns of the four-slot ring");
+  static_assert(!MOD || NQ >= CHUNK_STAGES, "every stage of a chunk reduces some block");
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
   const int64_t nst = nslabs / SPS;
@@ -162,11 +159,50 @@ __device__ __forceinline__ void panel_mainloop(v16f32 (&acc)[FM][FN],
     for (int b = 0; b < FN; ++b)
       xb[b] = *reinterpret_cast<const v8bf16*>(&S[ss * G::BBLK + b_off + b * 32 * 8]);
   };
-  auto mm = [&](const v8bf16 (&xa)[FM], const v8bf16 (&xb)[FN]) {
+  // one slab's MFMAs, block q = a FN + b, in the order first, first + 1, ... (mod NQ)
+  auto mm = [&](int first) {
 #pragma unroll
-    for (int a = 0; a < FM; ++a)
+    for (int t = 0; t < NQ; ++t) {
+      const int q = (first + t) % NQ, a = q / FN, b = q % FN;
+      acc[a][b] = mfma_bf16_32x32x16(bf[b], af[a], acc[a][b]);
+    }
+  };
+  // stage KS (J: its index in the chunk) from slot Sc; slot Sn holds stage KS + 1
+  auto stage = [&](int64_t KS, uint16_t* Sc, uint16_t* Sn, auto jc) {
+    constexpr int J = decltype(jc)::value;
+    int first = 0;
+    if constexpr (MOD) {
 #pragma unroll
-      for (int b = 0; b < FN; ++b) acc[a][b] = mfma_bf16_32x32x16(xb[b], xa[a], acc[a][b]);
+      for (int q = 0; q < NQ; ++q) {
+        if (q * CHUNK_STAGES / NQ != J) continue;
+        first = (q + 1) % NQ;
+        const int a = q / FN, b = q % FN;
+#pragma unroll
+        for (int r = 0; r < 16; ++r) {
+          const float x = acc[a][b][r];
+          acc[a][b][r] = __builtin_fmaf(-__builtin_rintf(x * invp), p, x);
+        }
+      }
+    }
+#pragma unroll
+    for (int ss = 0; ss < SPS; ++ss) {
+      if (ss + 1 < SPS) {
+        rd(Sc, ss + 1, an, bn);
+      } else {
+        __builtin_amdgcn_sched_barrier(0);
+        split_wait_vmcnt<2 * G::LW>();  // this wavefront's part of stage KS + 1 landed
+        split_barrier();                // everyone's has; everyone is done reading slot KS
+        issue(KS + 4, Sc);
+        rd(Sn, 0, an, bn);  // past the last stage: unused
+      }
+      mm(ss == 0 ? first : 0);
+      panel_slab_order<FM, FN>();
+      __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+      for (int a = 0; a < FM; ++a) af[a] = an[a];
+#pragma unroll
+      for (int b = 0; b < FN; ++b) bf[b] = bn[b];
+    }
   };
 
   if (nst <= 0) return;
@@ -178,22 +214,38 @@ __device__ __forceinline__ void panel_mainloop(v16f32 (&acc)[FM][FN],
   split_wait_vmcnt<3 * G::LW>();
   split_barrier();
   rd(S0, 0, af, bf);
-  int in_chunk = 0;
-  (void)in_chunk;
   (void)p;
   (void)invp;
-  for (int64_t ks = 0; ks < nst; ks += 4) {
-    MOMMS_PANEL_STAGE(ks, S0, S1)
-    if (ks + 1 < nst) MOMMS_PANEL_STAGE(ks + 1, S1, S2)
-    if (ks + 2 < nst) MOMMS_PANEL_STAGE(ks + 2, S2, S3)
-    if (ks + 3 < nst) MOMMS_PANEL_STAGE(ks + 3, S3, S0)
+  for (int64_t ks = 0; ks < nst; ks += CHUNK_STAGES) {
+#define MOMMS_PANEL_STAGE(J, Sc, Sn)                      \
+  if constexpr (J < CHUNK_STAGES) {                       \
+    if (ks + J < nst) stage(ks + J, Sc, Sn, std::integral_constant<int, J>()); \
+  }
+    MOMMS_PANEL_STAGE(0, S0, S1)
+    MOMMS_PANEL_STAGE(1, S1, S2)
+    MOMMS_PANEL_STAGE(2, S2, S3)
+    MOMMS_PANEL_STAGE(3, S3, S0)
+    MOMMS_PANEL_STAGE(4, S0, S1)
+    MOMMS_PANEL_STAGE(5, S1, S2)
+    MOMMS_PANEL_STAGE(6, S2, S3)
+    MOMMS_PANEL_STAGE(7, S3, S0)
+    MOMMS_PANEL_STAGE(8, S0, S1)
+    MOMMS_PANEL_STAGE(9, S1, S2)
+    MOMMS_PANEL_STAGE(10, S2, S3)
+    MOMMS_PANEL_STAGE(11, S3, S0)
+    MOMMS_PANEL_STAGE(12, S0, S1)
+    MOMMS_PANEL_STAGE(13, S1, S2)
+    MOMMS_PANEL_STAGE(14, S2, S3)
+    MOMMS_PANEL_STAGE(15, S3, S0)
+#undef MOMMS_PANEL_STAGE
   }
   split_wait_vmcnt<0>();  // the ring's last DMAs (reloads) have landed
 }
 
 // Packs count vectors of length len (vector v, element p at X[v*sv + p*sp],
-// bf16 bit patterns) into the panel layout, padding with zeros to vpad
-// vectors and lpad k-values: one 64 x 64 block of (v, p) per 256 threads,
+// bf16 bit patterns) into the panel layout (vpad / PANEL panels of
+// panel_stride(lpad / 16) slabs), padding with zeros to vpad vectors and
+// lpad k-values (the unused slab between panels is not written): one 64 x 64 block of (v, p) per 256 threads,
 // through LDS so the global reads (along whichever index is unit-stride) and
 // the 16-byte panel writes are both coalesced.
 template <int PANEL>
@@ -247,8 +299,8 @@ pack_bf16_panels(const uint16_t* __restrict__ X, int64_t count, int64_t len, int
     w.w = (uint32_t)tile[vv][pc * 8 + 6] | ((uint32_t)tile[vv][pc * 8 + 7] << 16);
     const int64_t pv = gv / PANEL, ks = gp / kSlab;
     const int kh = (int)((gp % kSlab) / 8);
-    *reinterpret_cast<uint4*>(Q + (pv * nslabs + ks) * (PANEL * kSlab) + (int64_t)kh * PANEL * 8 +
-                              (gv % PANEL) * 8) = w;
+    *reinterpret_cast<uint4*>(Q + (pv * panel_stride(nslabs) + ks) * (PANEL * kSlab) +
+                              (int64_t)kh * PANEL * 8 + (gv % PANEL) * 8) = w;
   }
 }
 

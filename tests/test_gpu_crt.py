@@ -57,6 +57,32 @@ def test_bitwise_against_restatement(m, n, k, N):
     assert same_bits(got, want)
 
 
+@pytest.mark.parametrize("sign", [1.0, -1.0])
+@pytest.mark.parametrize("m,n,k", [(260, 70, 2100), (64, 300, 4096)])
+def test_same_sign_accumulation_at_the_fp32_limit(sign, m, n, k):
+    """Constant A and B whose integer images are (p-1)/2 = 255 modulo the
+    largest modulus p = 511: every residue product of that GEMM is the same
+    +-255^2, so each FP32 sum climbs by 256 * 255^2 = 16,646,400 per 256
+    k-values, 0.8 % below 2^24. The sums stay exact only if every accumulator
+    is reduced modulo p at least once per 256 k-values (259 products already
+    exceed 2^24 with an odd value), so equality with the exact restatement pins
+    the reduction schedule of the residue GEMM."""
+    N = 13
+    p1 = O.crt_moduli(N)[0]
+    assert p1 == 511
+    beta = O.crt_beta(k, N)
+    base = 1 << (beta - 1)
+    ap = base + ((((p1 - 1) // 2) - base) % p1)  # smallest A' >= 2^(beta-1), A' = 255 mod 511
+    assert ap < 2 * base and ap % p1 == (p1 - 1) // 2
+    a = float(ap) * 2.0 ** (1 - beta)  # in [1, 2): row exponent 1, image exactly ap
+    rng = np.random.default_rng(SEED + k)
+    A = torch.full((k, m), a, dtype=torch.float64, device="cuda").T
+    B = torch.full((n, k), sign * a, dtype=torch.float64, device="cuda").T
+    C = make(rng, m, n)
+    got, want = run_both(A, B, C, N, 1.0, 1.0)
+    assert same_bits(got, want)
+
+
 @pytest.mark.parametrize("layouts", [("row", "row", "row"), ("gen", "col", "gen"),
                                      ("col", "gen", "row")])
 @pytest.mark.parametrize("e", [0, 16])
