@@ -275,9 +275,14 @@ crt_multiply(int64_t tiles_m, int64_t tiles_n, int64_t nslabs, int64_t mpad, int
 // 4. reconstruction.
 
 // One thread: kCrtRows consecutive rows i0.. of one column j (the residue
-// matrices are padded to whole tiles, so the 16-byte residue loads stay in
-// bounds); the constants of each modulus are loaded once for all of them.
-constexpr int kCrtRows = 8;
+// matrices are padded to whole tiles, so the residue loads of kCrtRows int16
+// stay in bounds); the constants of each modulus are loaded once for all of
+// them. Two rows per thread keep the sums and the loads in flight at 77
+// registers, six wavefronts per SIMD, so the arithmetic of some overlaps the
+// memory traffic of others (eight rows: 150 registers, three wavefronts per
+// SIMD, 0.88 ms at 8192^2 and 13 moduli; four rows 0.81 ms; two rows 0.79 ms).
+constexpr int kCrtRows = 2;
+static_assert(kCrtRows == 2 || kCrtRows == 4, "residue load width");
 
 template <int N>
 __global__ void __launch_bounds__(256)
@@ -287,14 +292,14 @@ crt_reconstruct(int64_t m, int64_t n, int64_t k, int beta_bits, double alpha, do
                 const int* __restrict__ EB, const double* __restrict__ A, int64_t rsA,
                 int64_t csA, const double* __restrict__ B, int64_t rsB, int64_t csB) {
   constexpr CrtRecon K = crt_recon_constants(N);
-  // a thread reconstructs rows i0 .. i0+7 of column j (16-byte residue loads);
+  // a thread reconstructs rows i0 .. i0+kCrtRows-1 of column j;
   // the results go through LDS so that the C update of the block runs along
   // columns, 64 consecutive rows per wavefront access
   __shared__ double vt[256 * (kCrtRows + 1)];
   const int tid = threadIdx.x;
   const int64_t rb = (int64_t)blockIdx.x * (256 * kCrtRows);
   const int64_t i0 = rb + (int64_t)tid * kCrtRows;
-  const bool active = i0 < m;  // (rows up to i0 + 7 < mpad then)
+  const bool active = i0 < m;  // (rows up to i0 + kCrtRows - 1 < mpad then)
   for (int64_t j = blockIdx.y; j < n; j += gridDim.y) {
     double v[kCrtRows];
 #pragma unroll
@@ -305,8 +310,17 @@ crt_reconstruct(int64_t m, int64_t n, int64_t k, int beta_bits, double alpha, do
       for (int e = 0; e < kCrtRows; ++e) crt_sums_init(a[e]);
 #pragma unroll
       for (int l = 0; l < N; ++l) {
-        const uint4 w = *reinterpret_cast<const uint4*>(RC + (int64_t)l * mpad * npad + j * mpad + i0);
-        const uint32_t h[4] = {w.x, w.y, w.z, w.w};
+        uint32_t h[kCrtRows / 2];  // kCrtRows residues, two per 32-bit word
+        {
+          const int16_t* src = RC + (int64_t)l * mpad * npad + j * mpad + i0;
+          if constexpr (kCrtRows == 4) {
+            const uint2 w = *reinterpret_cast<const uint2*>(src);
+            h[0] = w.x;
+            h[kCrtRows / 2 - 1] = w.y;
+          } else {
+            h[0] = *reinterpret_cast<const uint32_t*>(src);
+          }
+        }
 #pragma unroll
         for (int e = 0; e < kCrtRows; ++e) {
           const int16_t r = (int16_t)(h[e / 2] >> (16 * (e % 2)));
@@ -327,8 +341,8 @@ crt_reconstruct(int64_t m, int64_t n, int64_t k, int beta_bits, double alpha, do
         }
       }
     }
-    // row rb + q sits at vt[(q / 8) * 9 + q % 8] (the pad keeps the stores of
-    // a wavefront on distinct banks)
+    // row rb + q sits at vt[(q / kCrtRows) * (kCrtRows + 1) + q % kCrtRows] (the
+    // pad keeps the stores of a wavefront on distinct banks)
 #pragma unroll
     for (int e = 0; e < kCrtRows; ++e) vt[tid * (kCrtRows + 1) + e] = v[e];
     __syncthreads();

@@ -54,6 +54,7 @@ struct CrtRecon {
   float pf[kCrtMaxModuli], invpf[kCrtMaxModuli];      // p_l, fp32(1/p_l)
   double invpd[kCrtMaxModuli];                         // fp64(1/p_l)
   int fix;                                             // N <= 14: the +-P fix
+  int limbs;                                           // 24-bit limbs that P (and so every M_l) needs
 };
 
 // Small unsigned big integers (32-bit limbs) for the constants.
@@ -114,6 +115,8 @@ constexpr CrtRecon crt_recon_constants(int N) {
   K.h_lo = crt_big_lo(H);
   K.h_hi = crt_big_hi(H);
   K.fix = N <= 14 ? 1 : 0;
+  K.limbs = kCrtLimbs;  // the higher limbs of P are 0: fewer suffice, and M_l < P
+  while (K.limbs > 1 && K.pi[K.limbs - 1] == 0.0) --K.limbs;
   return K;
 }
 
@@ -141,7 +144,8 @@ MOMMS_HD inline void crt_sums_add(CrtSums& a, float r, int l, const CrtRecon& K)
   const double td = (double)t;
   a.f = __builtin_fma(td, K.invpd[l], a.f);
 #pragma unroll
-  for (int k = 0; k < kCrtLimbs; ++k) a.S[k] = __builtin_fma(td, K.mu[l][k], a.S[k]);
+  for (int k = 0; k < kCrtLimbs; ++k)
+    if (k < K.limbs) a.S[k] = __builtin_fma(td, K.mu[l][k], a.S[k]);
 }
 
 // C' from the sums over all N residues.
@@ -150,6 +154,7 @@ MOMMS_HD inline __int128 crt_sums_value(const CrtSums& a, const CrtRecon& K) {
   unsigned __int128 x = 0;
 #pragma unroll
   for (int k = 0; k < kCrtLimbs; ++k) {
+    if (k >= K.limbs) break;  // limbs of P and of every M_l above are 0
     const double T = __builtin_fma(-q, K.pi[k], a.S[k]);  // exact integer, |T| < 2^38
     x += (unsigned __int128)(__int128)(int64_t)T << (24 * k);
   }
