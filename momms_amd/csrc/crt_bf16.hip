@@ -24,13 +24,15 @@
 //   2. crt_pack: the balanced residues of A' and B' modulo each p_l
 //      (|r| <= 255, integers bf16 holds exactly) in packed panels: for
 //      modulus l, panel pv and 16-deep k-slab ks, one contiguous block
-//      [kh][PANEL][8] (kh = which half of the slab);
-//   3. crt_multiply: one BF16 GEMM per modulus with FP32 accumulation. A
-//      chunk of 256 k-values adds at most 256 * 255^2 to an accumulator that
-//      starts below 0.51 p in magnitude, so every partial sum is an integer
-//      below 2^24 and exact in any order; after each chunk the accumulator is
-//      reduced modulo p_l (x - p rint(x / p), exact by fma). The tile ends as
-//      the balanced residue of C' modulo p_l (int16);
+//      [kh][PANEL][8] (kh = which half of the slab), panels one unused slab
+//      apart (panel_stride);
+//   3. crt_multiply: one BF16 GEMM per modulus with FP32 accumulation. Any
+//      256 consecutive k-values add at most 256 * 255^2 to an accumulator
+//      that starts below 0.51 p in magnitude, so every partial sum is an
+//      integer below 2^24 and exact in any order; each accumulator is reduced
+//      modulo p_l (x - p rint(x / p), exact by fma) once per 256 k-values (at
+//      a point staggered over the accumulators, panel_gemm.hpp) and at the
+//      end. The tile ends as the balanced residue of C' modulo p_l (int16);
 //   4. crt_reconstruct: per entry, the Chinese remainder sum of the N
 //      residues in 128-bit integers (the quotient by P from an fp64
 //      estimate) gives C', then one correctly rounded conversion, the scaling
