@@ -8,8 +8,11 @@
 //   SpawnThreads<HipGemm> on 4 threads       only thread 0 launches
 //   HipGemm on device-resident matrices      no staging
 //   HipSplitGemm<7>                          FP64-accurate from BF16 slices
-// Tolerance: |C - C_ref| <= 2*gamma_{k+1}*(|A||B| + |C0|) componentwise, plus
-// the randomized residual of src/util.rs:55-85.
+//   HipCrtGemm<13>                           FP64-accurate from BF16 residues
+//   HipGemm<float>                           FP32 (the KNM sgemm counterpart)
+// Tolerance: |C - C_ref| <= 2*gamma_{k+1}*(|A||B| + |C0|) componentwise, with
+// the unit roundoff of T (TripleLoop in T is the reference), plus the
+// randomized residual of src/util.rs:55-85.
 #include <cmath>
 #include <cstdio>
 
@@ -19,11 +22,11 @@ using namespace momms;
 
 static int fails = 0;
 
-template <class Algo>
+template <class Algo, class T = double>
 static void check_tree(const char* name, size_t m, size_t n, size_t k, size_t threads,
                        double tol_scale, bool device) {
   std::mt19937_64 rng(0x6D6F6D6D73 + m + n + k);
-  Matrix<double> a(m, k), b(k, n), c0(m, n), cref(m, n), mag(m, n);
+  Matrix<T> a(m, k), b(k, n), c0(m, n), cref(m, n), mag(m, n);
   a.fill_rand(rng);
   b.fill_rand(rng);
   c0.fill_rand(rng);
@@ -32,7 +35,7 @@ static void check_tree(const char* name, size_t m, size_t n, size_t k, size_t th
   TripleLoop ref;
   ref.run(a, b, cref, one);
   // |A||B| + |C0|
-  Matrix<double> aa(m, k), bb(k, n);
+  Matrix<T> aa(m, k), bb(k, n);
   for (size_t x = 0; x < k; ++x)
     for (size_t y = 0; y < m; ++y) aa.set(y, x, std::fabs(a.get(y, x)));
   for (size_t x = 0; x < n; ++x)
@@ -41,34 +44,39 @@ static void check_tree(const char* name, size_t m, size_t n, size_t k, size_t th
     for (size_t y = 0; y < m; ++y) mag.set(y, x, std::fabs(c0.get(y, x)));
   ref.run(aa, bb, mag, one);
 
-  Matrix<double> c(m, n);
+  Matrix<T> c(m, n);
   c.copy_from(c0);
   Algo algo;
   if constexpr (std::is_same_v<Algo, SpawnThreads<HipGemm<double>>>) algo.set_n_threads(threads);
   auto t0 = std::chrono::steady_clock::now();
   if (device) {
-    auto da = Matrix<double>::device(m, k), db = Matrix<double>::device(k, n),
-         dc = Matrix<double>::device(m, n);
-    hip_check("copy", hipMemcpy(da.get_buffer(), a.get_buffer(), m * k * 8, hipMemcpyHostToDevice));
-    hip_check("copy", hipMemcpy(db.get_buffer(), b.get_buffer(), k * n * 8, hipMemcpyHostToDevice));
-    hip_check("copy", hipMemcpy(dc.get_buffer(), c.get_buffer(), m * n * 8, hipMemcpyHostToDevice));
+    auto da = Matrix<T>::device(m, k), db = Matrix<T>::device(k, n), dc = Matrix<T>::device(m, n);
+    const size_t e = sizeof(T);
+    hip_check("copy", hipMemcpy(da.get_buffer(), a.get_buffer(), m * k * e, hipMemcpyHostToDevice));
+    hip_check("copy", hipMemcpy(db.get_buffer(), b.get_buffer(), k * n * e, hipMemcpyHostToDevice));
+    hip_check("copy", hipMemcpy(dc.get_buffer(), c.get_buffer(), m * n * e, hipMemcpyHostToDevice));
     algo.run(da, db, dc, one);
-    hip_check("copy", hipMemcpy(c.get_buffer(), dc.get_buffer(), m * n * 8, hipMemcpyDeviceToHost));
+    hip_check("copy", hipMemcpy(c.get_buffer(), dc.get_buffer(), m * n * e, hipMemcpyDeviceToHost));
   } else {
     algo.run(a, b, c, one);
   }
   const double secs = seconds_since(t0);
-  const double u = std::ldexp(1.0, -53);
+  const double u = std::ldexp(1.0, std::is_same<T, float>::value ? -24 : -53);
   const double gam = (k + 1) * u / (1 - (k + 1) * u);
   double worst = 0;
   size_t bad = 0;
   for (size_t x = 0; x < n; ++x)
     for (size_t y = 0; y < m; ++y) {
-      const double e = std::fabs(c.get(y, x) - cref.get(y, x)) / mag.get(y, x);
+      const double e = std::fabs((double)c.get(y, x) - (double)cref.get(y, x)) / mag.get(y, x);
       worst = std::max(worst, e);
-      if (!(e <= tol_scale * gam)) ++bad;
+      if (!(e <= tol_scale * gam)) {
+        if (!bad)
+          std::printf("  first bad entry (%zu, %zu): C %.17g  reference %.17g  magnitude %.17g\n", y,
+                      x, (double)c.get(y, x), (double)cref.get(y, x), (double)mag.get(y, x));
+        ++bad;
+      }
     }
-  const double res = std::sqrt(test_c_eq_a_b(a, b, c, &c0, 1.0, 1.0, rng));
+  const double res = std::sqrt((double)test_c_eq_a_b(a, b, c, &c0, T(1), T(1), rng));
   std::printf("%-34s m=%zu n=%zu k=%zu  max|err|/mag=%.3e (bound %.3e)  residual=%.3e  %.2f GFLOP/s  %s\n",
               name, m, n, k, worst, tol_scale * gam, res, gflops(m, n, k, secs),
               bad ? "FAIL" : "PASS");
@@ -83,6 +91,10 @@ int main() {
                                             false);
   check_tree<HipGemm<double>>("HipGemm (device matrices)", 512, 384, 640, 1, 2.0, true);
   check_tree<HipSplitGemm<7>>("HipSplitGemm<7>", 300, 200, 700, 1, 2.0, false);
+  check_tree<HipCrtGemm<13>>("HipCrtGemm<13>", 300, 200, 700, 1, 2.0, false);
+  check_tree<PartN<3000, PartK<192, HipCrtGemm<13>>>>("PartN<3000,PartK<192,HipCrtGemm<13>>>",
+                                                      257, 3100, 400, 1, 2.0, false);
+  check_tree<HipGemm<float>, float>("HipGemm<float> (host views)", 300, 211, 517, 1, 2.0, false);
   std::printf("%s (%d failures)\n", fails ? "FAIL" : "PASS", fails);
   return fails ? 1 : 0;
 }

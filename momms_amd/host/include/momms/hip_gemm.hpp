@@ -105,10 +105,13 @@ void stage_out(M& x, size_t h, size_t w, const T* d, hipStream_t st) {
 
 }  // namespace detail
 
-// Plain GEMM on matrix cores: FP64 (momms_hip_dgemm).
+// Plain GEMM on matrix cores: FP64 (momms_hip_dgemm) or FP32
+// (momms_hip_sgemm, the counterpart of the KNM sgemm leaf,
+// src/kern/knm_kernel.rs:8-81).
 template <class T>
 struct HipGemm {
-  static_assert(std::is_same<T, double>::value, "HipGemm<T>: T = double");
+  static_assert(std::is_same<T, double>::value || std::is_same<T, float>::value,
+                "HipGemm<T>: T = double or float");
   hipStream_t stream = nullptr;
   detail::DevBuf da, db, dc;
 
@@ -121,15 +124,20 @@ struct HipGemm {
   void launch(At& a, Bt& b, Ct& c) {
     const size_t m = c.height(), n = c.width(), k = a.width();
     if (m == 0 || n == 0) return;
-    const double alpha = a.get_scalar() * b.get_scalar();
-    const double beta = c.get_scalar();
-    auto A = detail::stage_in<double>(a, m, k, da, true, stream);
-    auto B = detail::stage_in<double>(b, k, n, db, true, stream);
-    auto C = detail::stage_in<double>(c, m, n, dc, beta != 0.0, stream);
-    abi_check("momms_hip_dgemm",
-              momms_hip_dgemm((int64_t)m, (int64_t)n, (int64_t)k, &alpha, A.ptr, A.rs, A.cs,
-                              B.ptr, B.rs, B.cs, &beta, C.ptr, C.rs, C.cs, stream));
-    detail::stage_out<double>(c, m, n, C.ptr, stream);
+    const T alpha = a.get_scalar() * b.get_scalar();
+    const T beta = c.get_scalar();
+    auto A = detail::stage_in<T>(a, m, k, da, true, stream);
+    auto B = detail::stage_in<T>(b, k, n, db, true, stream);
+    auto C = detail::stage_in<T>(c, m, n, dc, beta != T(0), stream);
+    if constexpr (std::is_same<T, double>::value)
+      abi_check("momms_hip_dgemm",
+                momms_hip_dgemm((int64_t)m, (int64_t)n, (int64_t)k, &alpha, A.ptr, A.rs, A.cs,
+                                B.ptr, B.rs, B.cs, &beta, C.ptr, C.rs, C.cs, stream));
+    else
+      abi_check("momms_hip_sgemm",
+                momms_hip_sgemm((int64_t)m, (int64_t)n, (int64_t)k, &alpha, A.ptr, A.rs, A.cs,
+                                B.ptr, B.rs, B.cs, &beta, C.ptr, C.rs, C.cs, stream));
+    detail::stage_out<T>(c, m, n, C.ptr, stream);
     hip_check("hipStreamSynchronize", hipStreamSynchronize(stream));
   }
   static std::vector<AlgorithmStep> hierarchy_description() { return {}; }
@@ -162,6 +170,44 @@ struct HipSplitGemm {
               momms_hip_dgemm_split_bf16((int64_t)m, (int64_t)n, (int64_t)k, &alpha, A.ptr, A.rs,
                                          A.cs, B.ptr, B.rs, B.cs, &beta, C.ptr, C.rs, C.cs,
                                          Slices, w, wsb, stream));
+    detail::stage_out<double>(c, m, n, C.ptr, stream);
+    hip_check("hipStreamSynchronize", hipStreamSynchronize(stream));
+  }
+  static std::vector<AlgorithmStep> hierarchy_description() { return {}; }
+};
+
+// FP64-accurate GEMM from BF16 residues modulo Moduli coprime moduli
+// (momms_hip_dgemm_crt_bf16; 13 moduli give the FP64-level accuracy at k up
+// to 8192).
+template <int Moduli>
+struct HipCrtGemm {
+  static_assert(Moduli >= 1 && Moduli <= 16, "1..16 moduli");
+  hipStream_t stream = nullptr;
+  detail::DevBuf da, db, dc, ws;
+
+  template <class At, class Bt, class Ct>
+  void run(At& a, Bt& b, Ct& c, const ThreadInfo& thr) {
+    if (thr.thread_id() == 0) launch(a, b, c);
+    if (thr.num_threads() > 1) thr.barrier();
+  }
+  template <class At, class Bt, class Ct>
+  void launch(At& a, Bt& b, Ct& c) {
+    const size_t m = c.height(), n = c.width(), k = a.width();
+    if (m == 0 || n == 0) return;
+    const double alpha = a.get_scalar() * b.get_scalar();
+    const double beta = c.get_scalar();
+    auto A = detail::stage_in<double>(a, m, k, da, true, stream);
+    auto B = detail::stage_in<double>(b, k, n, db, true, stream);
+    auto C = detail::stage_in<double>(c, m, n, dc, beta != 0.0, stream);
+    const size_t wsb = momms_hip_crt_bf16_workspace_size((int64_t)m, (int64_t)n, (int64_t)k, Moduli);
+    if (wsb == 0)
+      throw std::invalid_argument("HipCrtGemm: " + std::to_string(Moduli) +
+                                  " moduli are too few for k = " + std::to_string(k));
+    void* w = ws.get(wsb);
+    abi_check("momms_hip_dgemm_crt_bf16",
+              momms_hip_dgemm_crt_bf16((int64_t)m, (int64_t)n, (int64_t)k, &alpha, A.ptr, A.rs,
+                                       A.cs, B.ptr, B.rs, B.cs, &beta, C.ptr, C.rs, C.cs,
+                                       Moduli, w, wsb, stream));
     detail::stage_out<double>(c, m, n, C.ptr, stream);
     hip_check("hipStreamSynchronize", hipStreamSynchronize(stream));
   }

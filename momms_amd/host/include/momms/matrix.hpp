@@ -184,11 +184,15 @@ class Matrix {
       for (size_t y = 0; y < height(); ++y) s += get(y, x) * get(y, x);
     return s;
   }
+  // this := alpha*o + beta*this (matrix.rs:165-191). With beta == 0 this is
+  // not read (BLAS semantics): the reference reads it, so copy_from into a
+  // fresh, uninitialized matrix can carry NaN garbage through 0 * NaN.
   void axpby(T alpha, const Matrix& o, T beta) {
     if (width() != o.width() || height() != o.height())
       throw std::invalid_argument("Cannot operate on nonconformal matrices!");
     for (size_t x = 0; x < width(); ++x)
-      for (size_t y = 0; y < height(); ++y) set(y, x, alpha * o.get(y, x) + beta * get(y, x));
+      for (size_t y = 0; y < height(); ++y)
+        set(y, x, beta == T(0) ? alpha * o.get(y, x) : alpha * o.get(y, x) + beta * get(y, x));
   }
   void axpy(T alpha, const Matrix& o) { axpby(alpha, o, T(1)); }
   void copy_from(const Matrix& o) { axpby(T(1), o, T(0)); }
