@@ -146,11 +146,17 @@ static int layout_of(int64_t rs, int64_t cs, int64_t h, int64_t w) {
 }
 
 // Number of k-parts for the 128 x 128 DMA tile: enough workgroups for two per
-// CU, each part at least 256 deep; 1 = no split.
+// CU, each part at least 256 deep; then, for small outputs, parts down to 64
+// deep while the partial sums stay within 32 MiB (their traffic is what
+// limits more parts: 512^3 42 -> 28 us, 1024 x 1024 x 512 47 -> 36 us, while
+// eight parts at 1024^3 would be slower than four); 1 = no split.
 int dgemm_splitk_parts(int64_t m, int64_t n, int64_t k) {
   const int64_t tiles = ceil_div(m, 128) * ceil_div(n, 128);
   int64_t parts = 1;
   while (tiles * parts < 512 && k / (parts * 2) >= 256 && parts < 16) parts *= 2;
+  while (tiles * parts < 512 && k / (parts * 2) >= 64 && parts < 16 &&
+         (double)(parts * 2) * (double)m * (double)n * 8.0 <= 32.0 * 1024 * 1024)
+    parts *= 2;
   return (int)parts;
 }
 
