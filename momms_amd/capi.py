@@ -131,13 +131,31 @@ def stream_handle(stream=None) -> int | None:
     return s.cuda_stream
 
 
-def _mat(t, h: int, w: int, name: str):
+def _mat(t, h: int, w: int, name: str, output: bool = False):
+    """(pointer, row stride, column stride) of a 2-D GPU tensor. A stride is
+    replaced by 1 only along a dimension of extent <= 1, where it never
+    scales an index. A stride of 0 (a broadcast view, from expand) along a
+    longer dimension is rejected, and so is an output whose elements
+    overlap: the kernels would read past the storage or race on one place."""
     if t.dim() != 2 or tuple(t.shape) != (h, w):
         raise ValueError(f"{name} has shape {tuple(t.shape)}, expected ({h}, {w})")
+    rs, cs = t.stride()
+    if h <= 1:
+        rs = 1
+    if w <= 1:
+        cs = 1
+    if rs < 1 or cs < 1:
+        raise ValueError(f"{name} has stride {tuple(t.stride())}: a broadcast (stride 0) or "
+                         "negative stride along a dimension longer than 1")
+    if output and h > 1 and w > 1:
+        # the shorter stride's run must end before the longer stride's step
+        (s_small, ext_small), s_big = sorted(((rs, h), (cs, w)))[0], max(rs, cs)
+        if s_small * ext_small > s_big:
+            raise ValueError(f"{name} has overlapping elements (shape {(h, w)}, strides "
+                             f"{tuple(t.stride())})")
     if not t.is_cuda:
         raise ValueError(f"{name} must be a GPU tensor")
-    rs, cs = t.stride()
-    return t.data_ptr(), max(rs, 1), max(cs, 1)
+    return t.data_ptr(), rs, cs
 
 
 def dgemm(A, B, C, alpha: float = 1.0, beta: float = 1.0, stream=None, tile_cfg: int | None = None):
@@ -151,7 +169,7 @@ def dgemm(A, B, C, alpha: float = 1.0, beta: float = 1.0, stream=None, tile_cfg:
     n = B.shape[1]
     a, rsa, csa = _mat(A, m, k, "A")
     b, rsb, csb = _mat(B, k, n, "B")
-    c, rsc, csc = _mat(C, m, n, "C")
+    c, rsc, csc = _mat(C, m, n, "C", output=True)
     al = ctypes.c_double(alpha)
     be = ctypes.c_double(beta)
     s = stream_handle(stream)
@@ -178,7 +196,7 @@ def _gemm_f32acc(fn: str, A, B, C, alpha, beta, in_dtype, stream):
     n = B.shape[1]
     a, rsa, csa = _mat(A, m, k, "A")
     b, rsb, csb = _mat(B, k, n, "B")
-    c, rsc, csc = _mat(C, m, n, "C")
+    c, rsc, csc = _mat(C, m, n, "C", output=True)
     al, be = ctypes.c_float(alpha), ctypes.c_float(beta)
     st = getattr(lib(), fn)(m, n, k, ctypes.byref(al), a, rsa, csa, b, rsb, csb, ctypes.byref(be),
                             c, rsc, csc, stream_handle(stream))
@@ -301,6 +319,17 @@ def _f64(*ts):
             raise TypeError(f"{nm} must be float64")
 
 
+def _owned(ws, stream):
+    """A workspace made for one call and dropped when it returns: its memory
+    goes back to torch's caching allocator, which may hand it out again at
+    once on the stream it was allocated on (the current one). When the
+    kernels run on another stream, the allocator is told to wait for that
+    stream's work before reusing the block."""
+    if stream is not None:
+        ws.buf.record_stream(stream)
+    return ws
+
+
 class SplitWorkspace:
     """Device workspace of the split-BF16 GEMM for one (m, n, k, slices)."""
 
@@ -328,7 +357,7 @@ class SplitWorkspace:
 
     def multiply(self, C, alpha=1.0, beta=1.0, stream=None):
         _f64((C, "C"))
-        c, rsc, csc = _mat(C, self.m, self.n, "C")
+        c, rsc, csc = _mat(C, self.m, self.n, "C", output=True)
         al, be = ctypes.c_double(alpha), ctypes.c_double(beta)
         st = lib().momms_hip_split_bf16_multiply(self.m, self.n, self.k, ctypes.byref(al),
                                                 ctypes.byref(be), c, rsc, csc, self.slices,
@@ -343,12 +372,12 @@ def dgemm_split_bf16(A, B, C, slices: int, alpha: float = 1.0, beta: float = 1.0
     _f64((A, "A"), (B, "B"), (C, "C"))
     m, k = A.shape
     n = B.shape[1]
-    ws = workspace or SplitWorkspace(m, n, k, slices, device=A.device)
+    ws = workspace or _owned(SplitWorkspace(m, n, k, slices, device=A.device), stream)
     if (ws.m, ws.n, ws.k, ws.slices) != (m, n, k, slices):
         raise ValueError("workspace was made for another problem")
     a, rsa, csa = _mat(A, m, k, "A")
     b, rsb, csb = _mat(B, k, n, "B")
-    c, rsc, csc = _mat(C, m, n, "C")
+    c, rsc, csc = _mat(C, m, n, "C", output=True)
     al, be = ctypes.c_double(alpha), ctypes.c_double(beta)
     st = lib().momms_hip_dgemm_split_bf16(m, n, k, ctypes.byref(al), a, rsa, csa, b, rsb, csb,
                                           ctypes.byref(be), c, rsc, csc, slices, ws.ptr,
@@ -391,7 +420,7 @@ class CrtWorkspace:
         _f64((A, "A"), (B, "B"), (C, "C"))
         a, rsa, csa = _mat(A, self.m, self.k, "A")
         b, rsb, csb = _mat(B, self.k, self.n, "B")
-        c, rsc, csc = _mat(C, self.m, self.n, "C")
+        c, rsc, csc = _mat(C, self.m, self.n, "C", output=True)
         al, be = ctypes.c_double(alpha), ctypes.c_double(beta)
         st = lib().momms_hip_crt_bf16_reconstruct(self.m, self.n, self.k, ctypes.byref(al), a,
                                                  rsa, csa, b, rsb, csb, ctypes.byref(be), c, rsc,
@@ -416,12 +445,12 @@ def dgemm_crt_bf16(A, B, C, moduli: int, alpha: float = 1.0, beta: float = 1.0,
     _f64((A, "A"), (B, "B"), (C, "C"))
     m, k = A.shape
     n = B.shape[1]
-    ws = workspace or CrtWorkspace(m, n, k, moduli, device=A.device)
+    ws = workspace or _owned(CrtWorkspace(m, n, k, moduli, device=A.device), stream)
     if (ws.m, ws.n, ws.k, ws.moduli) != (m, n, k, moduli):
         raise ValueError("workspace was made for another problem")
     a, rsa, csa = _mat(A, m, k, "A")
     b, rsb, csb = _mat(B, k, n, "B")
-    c, rsc, csc = _mat(C, m, n, "C")
+    c, rsc, csc = _mat(C, m, n, "C", output=True)
     al, be = ctypes.c_double(alpha), ctypes.c_double(beta)
     st = lib().momms_hip_dgemm_crt_bf16(m, n, k, ctypes.byref(al), a, rsa, csa, b, rsb, csb,
                                         ctypes.byref(be), c, rsc, csc, moduli, ws.ptr,
@@ -518,7 +547,7 @@ def hier_unpack_add(L: HierLayout, P, C, stream=None):
     """Unpacker<Matrix, Hierarch>::add (src/composables/unpack.rs:174-213):
     C[y][x] += P[offset(y, x)] over the h x w matrix C (in place)."""
     _f64((P, "P"), (C, "C"))
-    c, rc, cc = _mat(C, L.h, L.w, "C")
+    c, rc, cc = _mat(C, L.h, L.w, "C", output=True)
     if P.numel() < L.capacity or not P.is_cuda or (P.numel() > 1 and not P.is_contiguous()):
         raise ValueError(f"P must be a contiguous GPU tensor of >= {L.capacity} elements")
     check("momms_hip_hier_unpack_add",

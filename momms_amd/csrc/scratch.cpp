@@ -1,6 +1,7 @@
 #include "scratch.hpp"
 
 #include <mutex>
+#include <thread>
 #include <vector>
 
 namespace momms {
@@ -9,6 +10,7 @@ hipError_t stream_scratch(hipStream_t st, size_t bytes, void** out, int slot) {
   struct Entry {
     int dev;
     hipStream_t st;
+    std::thread::id thread;
     int slot;
     void* p;
     size_t bytes;
@@ -18,9 +20,10 @@ hipError_t stream_scratch(hipStream_t st, size_t bytes, void** out, int slot) {
   int dev = 0;
   hipError_t e = hipGetDevice(&dev);
   if (e != hipSuccess) return e;
+  const std::thread::id self = std::this_thread::get_id();
   std::lock_guard<std::mutex> lock(mu);
   for (Entry& x : pool) {
-    if (x.dev != dev || x.st != st || x.slot != slot) continue;
+    if (x.dev != dev || x.st != st || x.thread != self || x.slot != slot) continue;
     if (x.bytes < bytes) {
       if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
       if ((e = hipFree(x.p)) != hipSuccess) return e;
@@ -34,7 +37,7 @@ hipError_t stream_scratch(hipStream_t st, size_t bytes, void** out, int slot) {
   }
   void* p = nullptr;
   if ((e = hipMalloc(&p, bytes)) != hipSuccess) return e;
-  pool.push_back({dev, st, slot, p, bytes});
+  pool.push_back({dev, st, self, slot, p, bytes});
   *out = p;
   return hipSuccess;
 }

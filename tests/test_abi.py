@@ -155,3 +155,25 @@ def test_crt_staged_entry_points_validate_arguments():
     assert L.momms_hip_crt_bf16_reconstruct(100, 50, 30, ctypes.byref(al), 8, 1, 100, 8, 1, 30,
                                             ctypes.byref(al), 8, 1, 100, 14, 8, 10, None) == -17
     assert "momms_hip_crt_bf16_reconstruct" in capi.last_error()
+
+
+def test_wrapper_rejects_broadcast_and_overlapping_views():
+    """A stride of 0 along a dimension longer than 1 (torch.expand) would make
+    the kernels read past the tensor's storage, and an output whose elements
+    overlap would have two places written at once: both are refused before
+    any GPU work. A stride along a dimension of extent 1 is free."""
+    import torch
+
+    row = torch.zeros(4, dtype=torch.float64)
+    with pytest.raises(ValueError, match="broadcast"):
+        capi._mat(row.expand(3, 4), 3, 4, "A")
+    with pytest.raises(ValueError, match="broadcast"):
+        capi._mat(row[:3].unsqueeze(1).expand(3, 4), 3, 4, "B")
+    base = torch.zeros(16, dtype=torch.float64)
+    with pytest.raises(ValueError, match="overlapping"):
+        capi._mat(base.as_strided((4, 4), (1, 2)), 4, 4, "C", output=True)
+    # the same view as an input (read only) passes the stride check
+    with pytest.raises(ValueError, match="GPU tensor"):
+        capi._mat(base.as_strided((4, 4), (1, 2)), 4, 4, "A")
+    with pytest.raises(ValueError, match="GPU tensor"):
+        capi._mat(torch.zeros(1, 5, dtype=torch.float64).expand(1, 5), 1, 5, "C", output=True)

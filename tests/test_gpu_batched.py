@@ -252,6 +252,32 @@ def test_config5_full_batch_against_loop():
     plan.close()
 
 
+def test_config5_split_graph_bitwise_on_sampled_problems():
+    """The 4096-problem uneven batch through the split-BF16 plan at 7 slices,
+    captured and launched as one graph (the form the batched benchmark times):
+    64 problems drawn with a fixed seed equal the oracle's restatement bit for
+    bit, and the whole output is finite."""
+    rows, al, bl, cl = capi.uneven_batch(4096)
+    A, B, C0 = arenas(rows, al, bl, cl, seed=SEED + 5)
+    plan = capi.BatchPlan(rows, mode=1, slices=7)
+    assert plan.describe()["slices"] == 7
+    C = C0.clone()
+    plan.capture(A, B, C, 1.0, 1.0)
+    plan.launch()
+    torch.cuda.synchronize()
+    assert torch.isfinite(C).all()
+    pick = np.random.default_rng(SEED + 4096).choice(len(rows), 64, replace=False)
+    for i in sorted(pick):
+        m, n, k, ao, bo, co, lda, ldb, ldc = rows[i]
+        a = view(A, ao, m, k, lda).cpu().numpy()
+        b = view(B, bo, k, n, ldb).cpu().numpy()
+        want = np.asfortranarray(view(C0, co, m, n, ldc).cpu().numpy())
+        O.ozaki_gemm(np.asfortranarray(a), np.asfortranarray(b), want, 7, 1.0, 1.0)
+        got = np.asfortranarray(view(C, co, m, n, ldc).cpu().numpy())
+        assert np.array_equal(got.view(np.int64), want.view(np.int64)), (i, m, n, k)
+    plan.close()
+
+
 def test_plan_rejects_bad_tables():
     with pytest.raises(capi.MommsError):
         capi.BatchPlan([(4, 4, 4, 0, 0, 0, 2, 4, 4)])   # lda < m

@@ -146,11 +146,14 @@ def test_submatrix_view_leaves_rest_untouched():
     assert torch.equal(big[mask], before[mask])
 
 
-@pytest.mark.parametrize("m,n,k", [(4096, 4096, 4096), (8192, 64, 8192), (2048, 2048, 256),
-                                   (1024, 1024, 1024), (1000, 300, 5000)])
+@pytest.mark.parametrize("m,n,k", [(512, 512, 512), (1024, 1024, 1024), (2048, 2048, 2048),
+                                   (4096, 4096, 4096), (8192, 8192, 8192), (8192, 8192, 256),
+                                   (8192, 64, 8192), (2048, 2048, 256), (1000, 300, 5000)])
 def test_large_sampled_against_dd(m, n, k):
-    """Full-size shapes: sampled entries against the double-double oracle, and
-    the randomized residual of src/util.rs:55-85 (a size-independent check)."""
+    """Full-size shapes (every shape of the plain FP64 sweep: squares 512 to
+    8192, the rank-256 update and the 64-column skinny product): sampled
+    entries against the double-double oracle, and the randomized residual of
+    src/util.rs:55-85 (a size-independent check)."""
     rng = np.random.default_rng(SEED + m + n + k)
     a, b, c0 = uniform_colmajor(rng, m, k), uniform_colmajor(rng, k, n), uniform_colmajor(rng, m, n)
     A, B, C = to_dev(a), to_dev(b), to_dev(c0)
@@ -167,7 +170,8 @@ def test_large_sampled_against_dd(m, n, k):
     assert np.sqrt(res) <= 4 * gamma(k + n + 2) * scale
 
 
-@pytest.mark.parametrize("m,n,k", [(1024, 1024, 1024), (300, 250, 4099), (8192, 64, 2048)])
+@pytest.mark.parametrize("m,n,k", [(1024, 1024, 1024), (300, 250, 4099), (8192, 64, 2048),
+                                   (512, 512, 512), (1024, 1024, 512), (256, 256, 2048)])
 def test_split_k_is_deterministic_and_accurate(m, n, k):
     """The split-k form (k parts reduced in fixed order): repeat calls give the
     same bits, and the result is within gamma_{k+1} of double-double."""
