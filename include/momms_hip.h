@@ -52,10 +52,14 @@ int momms_hip_dgemm(int64_t m, int64_t n, int64_t k, const double* alpha, const 
  * leading dimensions and 16-byte aligned A, B), 1 = 64x64, 2 = 128x64,
  * 3 = 128x128 register-staged, 4 = 128x128 buffer->LDS with a 4-slab ring,
  * 5 = 128x128 buffer->LDS with k split into up to 16 parts whose partial
- * products a second kernel adds in fixed order (the built-in choice when the
- * 128x128 tiles alone cannot fill the device and k is deep; it uses a device
- * workspace the library allocates once per stream and keeps). Argument 15 is
- * tile_cfg. Same semantics as momms_hip_dgemm. */
+ * products a second kernel adds in fixed order, 6-9 = persistent stream-K:
+ * as many workgroups as stay resident share the (tile, 16-deep k-slab)
+ * iterations evenly, and the parts of tiles split between workgroups are
+ * added in k order by a second kernel (6: 64x64 tiles, three workgroups per
+ * CU; 7: 128x64, two per CU; 8: 64x64, two per CU with a deeper slab ring;
+ * 9: 64x64, one per CU with a six-slab ring). 5-9 use a device workspace the
+ * library allocates once per stream and keeps. Argument 15 is tile_cfg.
+ * Same semantics as momms_hip_dgemm. */
 int momms_hip_dgemm_tiled(int64_t m, int64_t n, int64_t k, const double* alpha,
                           const double* A, int64_t rsA, int64_t csA, const double* B,
                           int64_t rsB, int64_t csB, const double* beta, double* C,

@@ -18,10 +18,12 @@
 // consecutive rows of one column: stores to column-major C are 128-byte
 // coalesced.
 #include "dgemm.hpp"
+#include "dgemm_streamk.hpp"
 #include "dgemm_tile.hpp"
 #include "mfma.hpp"
 #include "scratch.hpp"
 #include "status.hpp"
+
 
 namespace momms {
 
@@ -160,31 +162,88 @@ int dgemm_splitk_parts(int64_t m, int64_t n, int64_t k) {
   return (int)parts;
 }
 
-hipError_t dgemm_launch(const DgemmArgs& g, int tile_cfg) {
+// k per part and the number of parts that hold any k, for the split-k form.
+static void splitk_plan(const DgemmArgs& g, int64_t& kper, int64_t& used) {
+  const int64_t parts = dgemm_splitk_parts(g.m, g.n, g.k);
+  kper = ceil_div(ceil_div(g.k, parts), 16) * 16;
+  used = ceil_div(g.k, kper);
+}
+
+// The configuration a launch of tile_cfg (-1: built-in) actually runs.
+static int resolve_cfg(const DgemmArgs& g, int tile_cfg, uint32_t& a_bytes, uint32_t& b_bytes) {
+  if (tile_cfg < 0) tile_cfg = dgemm_pick_tile(g.m, g.n, g.k);
+  const bool dma = dma_ok(g, a_bytes, b_bytes);
+  if (tile_cfg >= 5 && !dma) {
+    // operands the buffer->LDS path cannot take: the register-staged tiles
+    const int64_t big = ceil_div(g.m, 128) * ceil_div(g.n, 128);
+    const int64_t mid = ceil_div(g.m, 128) * ceil_div(g.n, 64);
+    tile_cfg = big >= 256 ? 0 : mid >= 256 ? 2 : 1;
+  }
+  return tile_cfg;
+}
+
+hipError_t dgemm_workspace_bytes(const DgemmArgs& g, int tile_cfg, size_t& parts_bytes) {
+  parts_bytes = 0;
+  if (g.m <= 0 || g.n <= 0) return hipSuccess;
+  uint32_t a_bytes = 0, b_bytes = 0;
+  tile_cfg = resolve_cfg(g, tile_cfg, a_bytes, b_bytes);
+  if (tile_cfg == 5) {
+    int64_t kper, used;
+    splitk_plan(g, kper, used);
+    parts_bytes = (size_t)used * g.m * g.n * 8;
+  } else if (tile_cfg >= 6 && tile_cfg <= 9) {
+    hipError_t e = hipSuccess;
+    const int64_t grid = streamk_grid(g, tile_cfg - 6, kStreamkMinSlabs, e);
+    if (e != hipSuccess) return e;
+    parts_bytes = streamk_workspace_bytes(grid, tile_cfg - 6);
+  }
+  return hipSuccess;
+}
+
+hipError_t dgemm_launch(const DgemmArgs& g, int tile_cfg, const DgemmWorkspace* ws) {
   // A single column (or row) operand is unit-stride along its only index, so
   // it is treated as column (row) major whatever its other stride.
   const int la = layout_of(g.rsA, g.csA, g.m, g.k);
   const int lb = layout_of(g.rsB, g.csB, g.k, g.n);
-  if (tile_cfg < 0) tile_cfg = dgemm_pick_tile(g.m, g.n, g.k);
   uint32_t a_bytes = 0, b_bytes = 0;
-  if (tile_cfg == 5 && dma_ok(g, a_bytes, b_bytes)) {
-    const int64_t parts = dgemm_splitk_parts(g.m, g.n, g.k);
-    const int64_t kper = ceil_div(ceil_div(g.k, parts), 16) * 16;
-    const int64_t used = ceil_div(g.k, kper);  // parts that hold any k
-    double* W = nullptr;
-    hipError_t e = stream_scratch(g.stream, (size_t)used * g.m * g.n * 8, (void**)&W);
+  tile_cfg = resolve_cfg(g, tile_cfg, a_bytes, b_bytes);
+  if (tile_cfg >= 6 && tile_cfg <= 9) {
+    const int cfg = tile_cfg - 6;
+    hipError_t e = hipSuccess;
+    const int64_t grid = streamk_grid(g, cfg, kStreamkMinSlabs, e);
     if (e != hipSuccess) return e;
+    const size_t pb = streamk_workspace_bytes(grid, cfg);
+    double* parts = nullptr;
+    if (ws == nullptr) {
+      if ((e = stream_scratch(g.stream, pb, (void**)&parts)) != hipSuccess) return e;
+    } else {
+      if (ws->parts_bytes < pb) return hipErrorInvalidValue;
+      parts = ws->parts;
+    }
+    return streamk_launch(g, cfg, grid, a_bytes, b_bytes, parts);
+  }
+  if (tile_cfg == 5) {
+    int64_t kper, used;
+    splitk_plan(g, kper, used);
+    const size_t need = (size_t)used * g.m * g.n * 8;
+    double* W = nullptr;
+    hipError_t e = hipSuccess;
+    if (ws == nullptr) {
+      if ((e = stream_scratch(g.stream, need, (void**)&W)) != hipSuccess) return e;
+    } else {
+      if (ws->parts_bytes < need) return hipErrorInvalidValue;
+      W = ws->parts;
+    }
     const int64_t tiles = ceil_div(g.m, 128) * ceil_div(g.n, 128);
     hipLaunchKernelGGL((dgemm_dma_splitk_kernel<2, 2>), dim3((unsigned)(tiles * used)), dim3(256),
                        0, g.stream, g.m, g.n, g.k, kper, g.A, g.csA, (uint64_t)a_bytes, g.B,
                        g.csB, (uint64_t)b_bytes, W, (int64_t)8);
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    dim3 grid((unsigned)ceil_div(g.m, 256), (unsigned)(g.n < 65535 ? g.n : 65535));
+  dim3 grid((unsigned)ceil_div(g.m, 256), (unsigned)(g.n < 65535 ? g.n : 65535));
     hipLaunchKernelGGL(dgemm_splitk_reduce, grid, dim3(256), 0, g.stream, g.m, g.n, used, W,
                        g.alpha, g.beta, g.C, g.rsC, g.csC);
     return hipGetLastError();
   }
-  if (tile_cfg == 5) tile_cfg = 0;  // operands the DMA path cannot take
   if (tile_cfg == 0 && dma_ok(g, a_bytes, b_bytes)) {
     const int64_t tiles = ceil_div(g.m, 128) * ceil_div(g.n, 128);
     hipLaunchKernelGGL((dgemm_dma_kernel<2, 2>), dim3((unsigned)tiles), dim3(256), 0, g.stream,
@@ -207,15 +266,20 @@ hipError_t dgemm_launch(const DgemmArgs& g, int tile_cfg) {
 }
 
 int dgemm_pick_tile(int64_t m, int64_t n, int64_t k) {
-  // Two big-tile workgroups per CU (the DMA kernel runs two per CU); with
-  // fewer tiles, split k when it is deep enough, else use smaller tiles.
+  // The data-parallel grid of 128 x 128 tiles when they come in nearly whole
+  // rounds of one per CU (one such workgroup keeps a CU's four SIMDs
+  // multiplying); otherwise the persistent stream-K form, which shares the
+  // (tile, k-slab) iterations out evenly whatever the tile count: 64 x 64
+  // tiles at three workgroups per CU when there are fewer big tiles than
+  // CUs, 128 x 64 tiles at two per CU when the last round of big tiles
+  // would be less than 85 % full (3000^3: 47 -> 55 TF).
+  (void)k;
+  constexpr int64_t kCus = 256;
   const int64_t big = ceil_div(m, 128) * ceil_div(n, 128);
-  if (big >= 512) return 0;
-  if (dgemm_splitk_parts(m, n, k) > 1) return 5;
-  if (big >= 256) return 0;
-  const int64_t mid = ceil_div(m, 128) * ceil_div(n, 64);
-  if (mid >= 256) return 2;
-  return 1;
+  if (big < kCus) return 6;
+  const int64_t rounds = ceil_div(big, kCus);
+  if (big * 100 >= rounds * kCus * 85) return 0;
+  return 7;
 }
 
 }  // namespace momms

@@ -24,12 +24,33 @@ struct DgemmArgs {
 // 1 = 64x64, 2 = 128x64, 3 = 128x128 register-staged form only,
 // 4 = 128x128 buffer->LDS with a 4-slab ring, one workgroup per CU,
 // 5 = 128x128 buffer->LDS with k split into parts (dgemm_splitk_parts) and a
-// fixed-order reduction (falls back to 0 when the DMA path does not apply).
-constexpr int kDgemmTileConfigs = 6;
+// fixed-order reduction, 6-9 = persistent stream-K with the tile forms of
+// dgemm_streamk.hpp (6: 64x64 three per CU, 7: 128x64, 8: 64x64 two per CU,
+// 9: 64x64 one per CU) (5-9 fall back to the built-in register-staged
+// choice when the buffer->LDS path does not apply).
+constexpr int kDgemmTileConfigs = 10;
 
 int dgemm_splitk_parts(int64_t m, int64_t n, int64_t k);
 
 int dgemm_pick_tile(int64_t m, int64_t n, int64_t k);
-hipError_t dgemm_launch(const DgemmArgs& g, int tile_cfg);
+
+// Device workspace of one launch: the split-k form's partial products, or
+// the stream-K form's parts of split tiles.
+struct DgemmWorkspace {
+  double* parts;
+  size_t parts_bytes;
+};
+
+// Bytes of workspace the launch of `tile_cfg` (-1: the built-in choice)
+// needs for g (0 for the forms that need none).
+hipError_t dgemm_workspace_bytes(const DgemmArgs& g, int tile_cfg, size_t& parts_bytes);
+
+// Enqueues the GEMM. The split-k and stream-K forms take their workspace
+// from `ws` when given (sized by dgemm_workspace_bytes), else from the
+// library's per-stream scratch.
+hipError_t dgemm_launch(const DgemmArgs& g, int tile_cfg, const DgemmWorkspace* ws = nullptr);
+
+// Minimum k-slabs (16 deep) per stream-K workgroup.
+constexpr int kStreamkMinSlabs = 8;
 
 }  // namespace momms

@@ -78,7 +78,7 @@ def test_ragged_shapes_and_layouts(m, n, k, la, lb, lc):
     check_against_oracle(A, B, C0, C, 1.0, 1.0, k)
 
 
-@pytest.mark.parametrize("tile", [0, 1, 2, 3, 4, 5])
+@pytest.mark.parametrize("tile", list(range(10)))
 def test_every_tile_config(tile):
     rng = np.random.default_rng(SEED + tile)
     m, n, k = 300, 250, 99
@@ -188,3 +188,33 @@ def test_split_k_is_deterministic_and_accurate(m, n, k):
     ref, mag = O.dd_entries(a, b, c0, rows, cols, 0.5, -1.0)
     c = host(C1)
     assert (np.abs(c[rows, cols] - ref) <= gamma(k + 1) * mag).all()
+
+
+@pytest.mark.parametrize("tile", [6, 7, 8, 9])
+@pytest.mark.parametrize("m,n,k", [(1024, 1024, 1024), (512, 512, 512), (8192, 64, 8192),
+                                   (300, 250, 4099), (129, 65, 1000), (2000, 1500, 777),
+                                   (64, 64, 16), (70, 33, 5)])
+def test_stream_k_is_deterministic_and_accurate(tile, m, n, k):
+    """The persistent stream-K form (tiles split between workgroups, their
+    parts added in k order by whichever workgroup arrives last): repeat calls
+    give the same bits, and every sampled entry is within gamma_{k+1} of
+    double-double. A third call on a C holding NaN with beta = 0 shows that
+    C is not read and that every entry is written."""
+    rng = np.random.default_rng(SEED + m + 3 * n + 7 * k + tile)
+    a, b, c0 = (uniform_colmajor(rng, *s) for s in ((m, k), (k, n), (m, n)))
+    A, B = to_dev(a), to_dev(b)
+    C1, C2 = to_dev(c0), to_dev(c0)
+    capi.dgemm(A, B, C1, 0.5, -1.0, tile_cfg=tile)
+    capi.dgemm(A, B, C2, 0.5, -1.0, tile_cfg=tile)
+    C3 = torch.full((n, m), float("nan"), dtype=torch.float64, device="cuda").T
+    capi.dgemm(A, B, C3, 0.5, 0.0, tile_cfg=tile)
+    torch.cuda.synchronize()
+    assert torch.equal(C1, C2)
+    assert torch.isfinite(C3).all()
+    rows = rng.integers(0, m, 2048)
+    cols = rng.integers(0, n, 2048)
+    ref, mag = O.dd_entries(a, b, c0, rows, cols, 0.5, -1.0)
+    c = host(C1)
+    assert (np.abs(c[rows, cols] - ref) <= gamma(k + 1) * mag).all()
+    ref0, mag0 = O.dd_entries(a, b, c0, rows, cols, 0.5, 0.0)
+    assert (np.abs(host(C3)[rows, cols] - ref0) <= gamma(k + 1) * mag0).all()
