@@ -65,6 +65,23 @@ int momms_hip_dgemm_tiled(int64_t m, int64_t n, int64_t k, const double* alpha,
                           int64_t rsB, int64_t csB, const double* beta, double* C,
                           int64_t rsC, int64_t csC, int tile_cfg, void* stream);
 
+/* momms_hip_dgemm for callers that cannot allocate device memory: code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * No counterpart in the reference. Same arguments, kernels and bits as
+ * momms_hip_dgemm; the stream-K form's workspace (the parts of output tiles
+ * split between workgroups, the only device memory beyond A, B and C) lives
+ * in a fixed 80 MiB array that each GPU the library runs on holds until the
+ * process ends, instead of library scratch. That covers every shape on a
+ * 256-CU MI355X; a shape that would need more returns -1. No call allocates
+ * or frees memory. Nothing in the array needs to start at zero: every part
+ * a call reads was written earlier in the same call. Every call on one GPU
+ * shares that GPU's array: calls that overlap on one GPU (other streams,
+ * other host threads) race and must be ordered by the caller. */
+int momms_hip_dgemm_noalloc(int64_t m, int64_t n, int64_t k, const double* alpha,
+                            const double* A, int64_t rsA, int64_t csA, const double* B,
+                            int64_t rsB, int64_t csB, const double* beta, double* C,
+                            int64_t rsC, int64_t csC, void* stream);
+
 /* ---- FP32-accumulating GEMMs ------------------------------------------------
  * C := beta*C + alpha*A*B with float C, alpha, beta; arguments as
  * momms_hip_dgemm. momms_hip_sgemm takes float A, B (FP32 matrix cores,

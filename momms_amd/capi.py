@@ -43,6 +43,8 @@ _SIGS: dict[str, list] = {
                         _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_vp],
     "momms_hip_dgemm_tiled": [_c_i64, _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_vp,
                               _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_int, _c_vp],
+    "momms_hip_dgemm_noalloc": [_c_i64, _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_vp,
+                                _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_vp],
     "momms_last_error_string": [],
     "momms_hip_split_bf16_workspace_size": [_c_i64, _c_i64, _c_i64, _c_int],
     "momms_hip_dgemm_split_bf16": [_c_i64, _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_vp,

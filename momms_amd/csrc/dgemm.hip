@@ -24,6 +24,7 @@
 #include "scratch.hpp"
 #include "status.hpp"
 
+#include <mutex>
 
 namespace momms {
 
@@ -326,4 +327,75 @@ extern "C" int momms_hip_dgemm_tiled(int64_t m, int64_t n, int64_t k, const doub
   DgemmArgs g{m, n, k, *alpha, A, rsA, csA, B, rsB, csB, *beta, C, rsC, csC,
               (hipStream_t)stream};
   return hip_status(fn, dgemm_launch(g, tile_cfg));
+}
+
+// ---------------------------------------------------------------------------
+// Allocation-free entry point. The workspace of the stream-K form (parts of
+// split tiles) lives in a fixed device array instead of library scratch, so
+// no call allocates. The built-in choice takes stream-K tiles of at most
+// 128 x 64 doubles, two workspace slots per resident workgroup: 512
+// workgroups x 2 x 64 KiB = 64 MiB on a 256-CU MI355X (768 x 2 x 32 KiB =
+// 48 MiB for 64 x 64 tiles), so 80 MiB covers it with room to spare; a
+// device that would need more is refused with an error.
+constexpr size_t kNoAllocScratchBytes = size_t(80) << 20;
+static __device__ __align__(256) double scratch[kNoAllocScratchBytes / sizeof(double)];
+
+namespace momms {
+namespace {
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+double* g_scratch_dev[kMaxDevices];  // device address of `scratch` on each device
+
+// The device address of the current device's copy of `scratch`, looked up
+// once per device with that device current.
+hipError_t noalloc_scratch(double** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return e;
+  if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_dev[dev] == nullptr) {
+    void* p = nullptr;
+    if ((e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch))) != hipSuccess) return e;
+    if (p == nullptr) return hipErrorInvalidSymbol;
+    g_scratch_dev[dev] = static_cast<double*>(p);
+  }
+  *out = g_scratch_dev[dev];
+  return hipSuccess;
+}
+
+}  // namespace
+}  // namespace momms
+
+extern "C" int momms_hip_dgemm_noalloc(int64_t m, int64_t n, int64_t k, const double* alpha,
+                                       const double* A, int64_t rsA, int64_t csA,
+                                       const double* B, int64_t rsB, int64_t csB,
+                                       const double* beta, double* C, int64_t rsC, int64_t csC,
+                                       void* stream) {
+  using namespace momms;
+  static const char* fn = "momms_hip_dgemm_noalloc";
+  if (m < 0) return bad_arg(fn, 1, "m < 0");
+  if (n < 0) return bad_arg(fn, 2, "n < 0");
+  if (k < 0) return bad_arg(fn, 3, "k < 0");
+  if (alpha == nullptr) return bad_arg(fn, 4, "alpha is null");
+  int st;
+  if ((st = check_operand(fn, 5, A, m, k, rsA, csA))) return st;
+  if ((st = check_operand(fn, 8, B, k, n, rsB, csB))) return st;
+  if (beta == nullptr) return bad_arg(fn, 11, "beta is null");
+  if ((st = check_operand(fn, 12, C, m, n, rsC, csC))) return st;
+  if (m == 0 || n == 0) return 0;
+  DgemmArgs g{m, n, k, *alpha, A, rsA, csA, B, rsB, csB, *beta, C, rsC, csC,
+              (hipStream_t)stream};
+  size_t parts_bytes = 0;
+  hipError_t e = dgemm_workspace_bytes(g, -1, parts_bytes);
+  if (e != hipSuccess) return hip_status(fn, e);
+  if (parts_bytes > kNoAllocScratchBytes)
+    return bad_arg(fn, 1, "the workspace this shape needs exceeds the 80 MiB static scratch");
+  DgemmWorkspace ws{nullptr, 0};
+  if (parts_bytes > 0) {
+    if ((e = noalloc_scratch(&ws.parts)) != hipSuccess) return hip_status(fn, e);
+    ws.parts_bytes = kNoAllocScratchBytes;
+  }
+  return hip_status(fn, dgemm_launch(g, -1, &ws));
 }
