@@ -40,6 +40,7 @@ METRIC = "FP64-accurate TFLOP/s @8192^3: split-BF16 MFMA vs plain FP64 MFMA, % p
 PEAK_FP64 = 78.6      # TFLOP/s, MI355X dense FP64 matrix
 PEAK_FP32 = 157.3     # TFLOP/s, MI355X dense FP32 matrix
 PEAK_BF16 = 2516.6    # TFLOP/s, MI355X dense BF16 matrix
+PEAK_INT8 = 5033.2    # TOP/s, MI355X dense INT8 matrix
 PEAK_HBM = 8000.0     # GB/s
 SEED = 0x6D6F6D6D73
 # sampled entries of every accuracy statement; the slice-count selection and the
@@ -51,23 +52,47 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def time_loop(fn, steps: int, warmup: int) -> float:
-    """Average milliseconds per call over `steps` calls after `warmup`, from
-    HIP events on torch's current stream (the stream every call launches on),
-    bracketed by device synchronizations."""
+LAST_TIMING: dict = {}
+
+
+def time_loop(fn, steps: int, warmup: int, group: int = 0) -> float:
+    """Median milliseconds per call, from HIP events on torch's current stream
+    (the stream every call launches on). After `warmup` calls and a device
+    synchronization, the timed calls are queued behind a short spin kernel,
+    so the GPU runs them back to back whatever the host's launch rate, and
+    an event pair brackets each group of `group` calls (0: one call per pair
+    when a call takes over 0.5 ms, else enough calls for about 0.5 ms, which
+    keeps the events' own cost, a few microseconds, out of the figure). The
+    median is over `steps` groups; LAST_TIMING also holds the mean over the
+    whole bracket and the group size."""
     import torch
 
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
-    ev0 = torch.cuda.Event(enable_timing=True)
-    ev1 = torch.cuda.Event(enable_timing=True)
-    ev0.record()
-    for _ in range(steps):
+    if group <= 0:
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
         fn()
-    ev1.record()
+        e1.record()
+        torch.cuda.synchronize()
+        one = e0.elapsed_time(e1)
+        group = 1 if one >= 0.5 else min(50, max(1, int(0.5 / max(one, 1e-3))))
+    ev = [torch.cuda.Event(enable_timing=True) for _ in range(steps + 1)]
+    # about 40 us of host time per call to enqueue (spin counted in ~2.5 GHz cycles)
+    torch.cuda._sleep(int(steps * group * 40e-6 * 2.5e9))
+    ev[0].record()
+    for i in range(steps):
+        for _ in range(group):
+            fn()
+        ev[i + 1].record()
     torch.cuda.synchronize()
-    return ev0.elapsed_time(ev1) / steps
+    per = sorted(ev[i].elapsed_time(ev[i + 1]) / group for i in range(steps))
+    LAST_TIMING.clear()
+    LAST_TIMING.update({"median_ms": per[len(per) // 2],
+                        "mean_ms": ev[0].elapsed_time(ev[steps]) / (steps * group),
+                        "group": group, "groups": steps})
+    return per[len(per) // 2]
 
 
 def rand_colmajor(n_rows, n_cols, gen, scale_e=0):
@@ -81,32 +106,107 @@ def rand_colmajor(n_rows, n_cols, gen, scale_e=0):
     return x.T
 
 
-def cpu_baseline(threads: int) -> dict:
+def host_cpus() -> dict:
+    """This host's CPUs as the process may use them: the model, the logical
+    CPUs in its affinity mask, one logical CPU per physical core (SMT
+    siblings share a (package, core) id in /sys/devices/system/cpu/*/topology),
+    the cgroup CPU share (cpu.max quota / period, if one is set), and the
+    maximum clock."""
+    aff = sorted(os.sched_getaffinity(0))
+    first = {}
+    for c in aff:
+        try:
+            top = Path(f"/sys/devices/system/cpu/cpu{c}/topology")
+            key = ((top / "physical_package_id").read_text().strip(),
+                   (top / "core_id").read_text().strip())
+        except OSError:
+            key = ("?", str(c))
+        first.setdefault(key, c)
+    share = None
+    try:
+        quota, period = Path("/sys/fs/cgroup/cpu.max").read_text().split()
+        if quota != "max":
+            share = max(1, int(quota) // int(period))
+    except (OSError, ValueError):
+        pass
+    model, mhz, avx512 = "unknown", None, False
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.startswith("model name") and model == "unknown":
+                model = line.split(":", 1)[1].strip()
+            elif line.startswith("cpu MHz") and mhz is None:
+                mhz = float(line.split(":", 1)[1])
+            elif line.startswith("flags") and " avx512f" in line:
+                avx512 = True
+    except OSError:
+        pass
+    try:
+        mhz = int(Path("/sys/devices/system/cpu/cpu0/cpufreq/cpuinfo_max_freq").read_text()) / 1000
+    except (OSError, ValueError):
+        pass
+    return {"model": model, "logical_cpus": len(aff), "physical_cores": len(first),
+            "cpu_share": share, "cores": sorted(first.values()), "max_mhz": mhz,
+            "avx512": avx512}
+
+
+def _best_of(run, c0, reps: int, flush) -> float:
+    """Best wall-clock seconds of `reps` runs on fresh copies of c0, each after
+    touching a 256 MiB buffer (l3_packing.rs:24-46: fresh inputs, flush, time
+    run only, best of the repetitions)."""
+    best = float("inf")
+    for _ in range(reps):
+        c = c0.copy(order="F")
+        flush += 1.0
+        t0 = time.perf_counter()
+        run(c)
+        best = min(best, time.perf_counter() - t0)
+    return best
+
+
+def cpu_baseline(a, b, c0, a1, b1, c1, threads: int, info: dict) -> dict:
     """The restated reference path (Goto tree, Nc/Kc/Mc = 3000/192/120, 6x8
-    FMA micro-kernel) on a bounded sample of the workload: m=8192, n=1024,
-    k=8192 (one eighth of the columns of C), best of 3, fresh C each time."""
+    FMA micro-kernel, one pinned thread per physical core) on the same inputs
+    as the GPU legs: the 8192^3 workload's A with the first n columns of B and
+    C (a bounded sample), and the m=n=k=1024 configuration; all threads and
+    one thread; best of 5 (1024^3 at one thread: best of 3)."""
     import numpy as np
 
     sys.path.insert(0, str(ROOT / "oracle"))
     import pyoracle as O
 
-    m, n, k = 8192, 1024, 8192
-    rng = np.random.default_rng(SEED)
-    a = np.asfortranarray(rng.uniform(-1, 1, (m, k)))
-    b = np.asfortranarray(rng.uniform(-1, 1, (k, n)))
-    c0 = np.asfortranarray(rng.uniform(-1, 1, (m, n)))
+    cpus = info["cores"][:threads]
     flush = np.zeros(256 * 1024 * 1024 // 8)
-    best = float("inf")
-    for _ in range(3):
-        c = c0.copy(order="F")
-        flush += 1.0
-        t0 = time.perf_counter()
-        O.goto_dgemm(a, b, c, 1.0, 1.0, nthreads=threads)
-        best = min(best, time.perf_counter() - t0)
-    tf = 2.0 * m * n * k / best / 1e12
+    m, k = a.shape
+    n = b.shape[1]
+
+    def goto(A, B, nt, pins):
+        return lambda c: O.goto_dgemm(A, B, c, 1.0, 1.0, nthreads=nt, cpus=pins)
+
+    t_all = _best_of(goto(a, b, threads, cpus), c0, 5, flush)
+    t_c1 = _best_of(goto(a1, b1, threads, cpus), c1, 5, flush)
+    t_c1_one = _best_of(goto(a1, b1, 1, cpus[:1]), c1, 3, flush)
+    tf = 2.0 * m * n * k / t_all / 1e12
+    gf_core = tf * 1e3 / threads
+    c1_flops = 2.0 * a1.shape[0] * b1.shape[1] * a1.shape[1]
+    # 2 FMA pipes x 4 doubles x 2 flops per cycle at the clock the host reports
+    # (its boost clock can be higher: the one-thread rate is the better yardstick)
+    peak_core = 16.0 * info["max_mhz"] / 1e3 if info["max_mhz"] else None
     return {"value": round(tf, 4), "unit": "TFLOP/s", "cores": threads, "kind": "port",
-            "sample": f"plain FP64 Goto+6x8 GEMM (the reference path restated in C) "
-                      f"m={m} n={n} k={k} = 1/8 of the 8192^3 columns, best of 3, {best:.2f} s"}
+            "sample": f"plain FP64 Goto+6x8 GEMM (the reference path restated in C) on the "
+                      f"8192^3 workload's A and its first {n} of 8192 columns of B and C "
+                      f"(m={m} n={n} k={k}), {threads} threads pinned one per physical core, "
+                      f"best of 5 with fresh C and a 256 MiB flush, {t_all:.2f} s",
+            "cpu_model": info["model"], "physical_cores": info["physical_cores"],
+            "logical_cpus": info["logical_cpus"], "cgroup_cpu_share": info["cpu_share"],
+            "gflops_per_core": round(gf_core, 2),
+            "reported_mhz": info["max_mhz"],
+            "avx2_fma_peak_per_core_gflops_at_reported_mhz": round(peak_core, 1) if peak_core else None,
+            "frac_of_core_peak_at_reported_mhz": round(gf_core / peak_core, 3) if peak_core else None,
+            "frac_of_one_thread_rate": round(gf_core / (c1_flops / t_c1_one / 1e9), 3),
+            "config1_1024cube": {"all_threads_gflops": round(c1_flops / t_c1 / 1e9, 2),
+                                 "one_thread_gflops": round(c1_flops / t_c1_one / 1e9, 2),
+                                 "seconds_all": round(t_c1, 4),
+                                 "seconds_one": round(t_c1_one, 4)}}
 
 
 def sweep_plain(capi, torch, gen, steps):
@@ -114,10 +214,12 @@ def sweep_plain(capi, torch, gen, steps):
     shapes = [(s, s, s) for s in (512, 1024, 2048, 4096, 8192)] + [(8192, 8192, 256), (8192, 64, 8192)]
     for m, n, k in shapes:
         A, B, C = rand_colmajor(m, k, gen), rand_colmajor(k, n, gen), rand_colmajor(m, n, gen)
-        ms = time_loop(lambda: capi.dgemm(A, B, C, 1.0, 1.0), steps, 2)
+        ms = time_loop(lambda: capi.dgemm(A, B, C, 1.0, 1.0), steps, 3)
         tf = 2.0 * m * n * k / ms / 1e9
         out.append({"m": m, "n": n, "k": k, "ms": round(ms, 4), "tflops": round(tf, 3),
-                    "frac_fp64_peak": round(tf / PEAK_FP64, 4)})
+                    "frac_fp64_peak": round(tf / PEAK_FP64, 4),
+                    "timing": f"median of {steps} event-timed groups of {LAST_TIMING['group']} "
+                              "queued calls"})
         del A, B, C
     return out
 
@@ -148,21 +250,23 @@ def split_grid(capi, torch, gen, steps, sizes=(4096, 8192), spreads=(0, 8, 16), 
 
 
 def crt_grid(capi, torch, gen, steps, sizes=(4096, 8192), spreads=(0, 8, 16),
-             moduli=(10, 12, 13, 14, 16)):
-    """Config 3 inputs through the residue form: error and rate per moduli count."""
+             moduli=(10, 12, 13, 14, 16), form="bf16"):
+    """Config 3 inputs through a residue form: error and rate per moduli count."""
     out = []
     for n in sizes:
         for e in spreads:
             A, B, C0 = rand_colmajor(n, n, gen, e), rand_colmajor(n, n, gen, e), rand_colmajor(n, n, gen)
             C = C0.clone()
             for nm in moduli:
-                ws = capi.CrtWorkspace(n, n, n, nm)
-                ms = time_loop(lambda: capi.dgemm_crt_bf16(A, B, C, nm, 1.0, 1.0, workspace=ws), steps, 1)
+                ws = capi.CrtWorkspace(n, n, n, nm, form=form)
+                fn = capi.dgemm_crt_bf16 if form == "bf16" else capi.dgemm_crt_i8
+                ms = time_loop(lambda: fn(A, B, C, nm, 1.0, 1.0, workspace=ws), steps, 1)
                 C.copy_(C0)
-                capi.dgemm_crt_bf16(A, B, C, nm, 1.0, 1.0, workspace=ws)
+                fn(A, B, C, nm, 1.0, 1.0, workspace=ws)
                 err = capi.sampled_error(A, B, C0, C, samples=8192, seed=SEED)
                 tf = 2.0 * n ** 3 / ms / 1e9
-                out.append({"n": n, "e": e, "moduli": nm, "beta_bits": capi.crt_beta(n, nm),
+                out.append({"n": n, "e": e, "form": form, "moduli": nm,
+                            "beta_bits": capi.crt_beta(n, nm, form),
                             "ms": round(ms, 4), "tflops_fp64_equiv": round(tf, 3),
                             "frac_fp64_peak": round(tf / PEAK_FP64, 4),
                             "max_scaled_err": err["max_scaled"], "max_rel_err": err["max_rel"]})
@@ -279,7 +383,7 @@ def main() -> int:
     ap.add_argument("--quick", action="store_true", help="headline only, no other configs")
     ap.add_argument("--tile", type=int, default=None, help="FP64 tile configuration (--only plain)")
     ap.add_argument("--moduli", type=int, default=14, help="moduli of the residue form (--only crt)")
-    ap.add_argument("--only", choices=["plain", "split", "multiply", "crt", "batched",
+    ap.add_argument("--only", choices=["plain", "split", "multiply", "crt", "crt_i8", "batched",
                                        "batched_plain", "copy", "hier", "fp32"])
     args = ap.parse_args()
     if args.gpus != 1:
@@ -324,12 +428,16 @@ def main() -> int:
             ms = time_loop(lambda: ws.multiply(C, 1.0, 1.0), args.steps, args.warmup)
         print(json.dumps({"only": args.only, "slices": s, "ms": ms, "tflops": flops / ms / 1e9}))
         return 0
-    if args.only == "crt":
-        ws = capi.CrtWorkspace(n, n, n, args.moduli)
-        ms = time_loop(lambda: capi.dgemm_crt_bf16(A, B, C, args.moduli, 1.0, 1.0, workspace=ws),
+    if args.only in ("crt", "crt_i8"):
+        form = "bf16" if args.only == "crt" else "i8"
+        fn = capi.dgemm_crt_bf16 if form == "bf16" else capi.dgemm_crt_i8
+        ws = capi.CrtWorkspace(n, n, n, args.moduli, form=form)
+        ms = time_loop(lambda: fn(A, B, C, args.moduli, 1.0, 1.0, workspace=ws),
                        args.steps, args.warmup)
-        print(json.dumps({"only": "crt", "moduli": args.moduli, "ms": ms,
-                          "tflops": flops / ms / 1e9}))
+        ws.prepare(A, B)
+        pms = time_loop(ws.products, args.steps, args.warmup)
+        print(json.dumps({"only": args.only, "moduli": args.moduli, "ms": ms,
+                          "tflops": flops / ms / 1e9, "products_ms": pms}))
         return 0
     if args.only == "fp32":
         A32, B32 = A.float(), B.float()
@@ -424,8 +532,34 @@ def main() -> int:
     Cr.copy_(C0)
     capi.dgemm_crt_bf16(A, B, Cr, nm, 1.0, 1.0, workspace=wr)
     err_r = capi.sampled_error(A, B, C0, Cr, 1.0, 1.0, samples=CHECK_SAMPLES, seed=SEED)
-    del wr, Cr, C
+    del wr, Cr
+
+    # the INT8 residue form (not the headline: the metric names BF16), at the
+    # smallest moduli count that meets the same accuracy rule
+    n8 = 16
+    for cand in range(13, 17):
+        wsc = capi.CrtWorkspace(n, n, n, cand, form="i8")
+        Cc = C0.clone()
+        capi.dgemm_crt_i8(A, B, Cc, cand, 1.0, 1.0, workspace=wsc)
+        e = capi.sampled_error(A, B, C0, Cc, 1.0, 1.0, samples=CHECK_SAMPLES, seed=SEED)
+        del wsc, Cc
+        log(f"bench: int8 moduli={cand}: scaled {e['max_scaled']:.3e} rel {e['max_rel']:.3e}")
+        if accurate(e):
+            n8 = cand
+            break
+    log(f"bench: INT8 residues, N={n8}")
+    w8 = capi.CrtWorkspace(n, n, n, n8, form="i8")
+    C8 = C0.clone()
+    i8_ms = time_loop(lambda: capi.dgemm_crt_i8(A, B, C8, n8, 1.0, 1.0, workspace=w8),
+                      args.steps, args.warmup)
+    w8.prepare(A, B)
+    i8_prod_ms = time_loop(w8.products, args.steps, args.warmup)
+    C8.copy_(C0)
+    capi.dgemm_crt_i8(A, B, C8, n8, 1.0, 1.0, workspace=w8)
+    err_8 = capi.sampled_error(A, B, C0, C8, 1.0, 1.0, samples=CHECK_SAMPLES, seed=SEED)
+    del w8, C8, C
     crt_tf = flops / crt_ms / 1e9
+    i8_tf = flops / i8_ms / 1e9
 
     # HBM bytes per launch of the dominant kernel, measured with rocprofv3
     # FETCH_SIZE / WRITE_SIZE (calibrated on a plain copy) for this exact
@@ -480,6 +614,16 @@ def main() -> int:
                 "frac_fp64_peak": round(crt_tf / PEAK_FP64, 4),
                 "speedup_vs_plain": round(crt_tf / plain_tf, 4),
                 "max_scaled_err": err_r["max_scaled"], "max_rel_err": err_r["max_rel"]},
+        "crt_i8": {"moduli": n8, "beta_bits": capi.crt_beta(n, n8, "i8"), "gemms": n8,
+                   "tflops_fp64_equiv": round(i8_tf, 3), "ms": round(i8_ms, 4),
+                   "products_ms": round(i8_prod_ms, 4),
+                   "products_tops": round(n8 * flops / i8_prod_ms / 1e9, 2),
+                   "products_frac_int8_peak": round(n8 * flops / i8_prod_ms / 1e9 / PEAK_INT8, 4),
+                   "frac_fp64_peak": round(i8_tf / PEAK_FP64, 4),
+                   "speedup_vs_plain": round(i8_tf / plain_tf, 4),
+                   "max_scaled_err": err_8["max_scaled"], "max_rel_err": err_8["max_rel"],
+                   "note": "INT8 residues (v_mfma_i32_32x32x32_i8); reported beside the "
+                           "headline, which the metric fixes to the split-BF16 forms"},
         "split": {"slices": s, "products": P, "tflops_fp64_equiv": round(split_tf, 3),
                   "ms": round(split_ms, 4), "multiply_ms": round(mult_ms, 4),
                   "frac_fp64_peak": round(split_tf / PEAK_FP64, 4),
@@ -491,30 +635,57 @@ def main() -> int:
         "accuracy_rule": f"max scaled and max relative error over {CHECK_SAMPLES} sampled "
                          "entries (double-double reference) within 2x plain FP64's",
     }
+    cpu_inputs, c1_gpu_tf = None, None
+    if not args.no_cpu:
+        # host copies for the CPU leg: the workload's A, and B and C0 cut to
+        # their first 1024 columns (A, B, C0 are transposed views of
+        # row-major tensors, so .T is contiguous)
+        ncols = 1024
+        cpu_inputs = [A.T.cpu().numpy().T, B.T[:ncols].cpu().numpy().T,
+                      C0.T[:ncols].cpu().numpy().T]
+        # configuration 1 (m=n=k=1024): drawn on the GPU, timed there too
+        g1 = torch.Generator(device="cuda").manual_seed(SEED & 0xFFFFFFFF)
+        A1, B1, C1 = rand_colmajor(1024, 1024, g1), rand_colmajor(1024, 1024, g1), \
+            rand_colmajor(1024, 1024, g1)
+        cpu_inputs += [A1.T.cpu().numpy().T, B1.T.cpu().numpy().T, C1.T.cpu().numpy().T]
+        C1w = C1.clone()
+        c1_ms = time_loop(lambda: capi.dgemm(A1, B1, C1w, 1.0, 1.0), 20, 3)
+        c1_gpu_tf = 2.0 * 1024 ** 3 / c1_ms / 1e9
+        del A1, B1, C1, C1w
     del A, B, C0
     torch.cuda.empty_cache()
 
     if not args.quick:
         cfg = {}
         log("bench: config 2 sweep (plain FP64)")
-        cfg["plain_sweep"] = sweep_plain(capi, torch, gen, 3)
+        cfg["plain_sweep"] = sweep_plain(capi, torch, gen, 20)
         log("bench: config 3 grid (split-BF16: size x spread x slices)")
-        cfg["split_grid"] = split_grid(capi, torch, gen, 2)
+        cfg["split_grid"] = split_grid(capi, torch, gen, 10)
         log("bench: config 3 grid (split-BF16 residues: size x spread x moduli)")
-        cfg["crt_grid"] = crt_grid(capi, torch, gen, 2)
+        cfg["crt_grid"] = crt_grid(capi, torch, gen, 10)
+        log("bench: config 3 grid (INT8 residues: size x spread x moduli)")
+        cfg["crt_i8_grid"] = crt_grid(capi, torch, gen, 10, moduli=(12, 14, 15, 16), form="i8")
         log("bench: config 4 (FP32, BF16->FP32)")
-        cfg["reduced_precision"] = reduced_precision(capi, torch, gen, 3)
+        cfg["reduced_precision"] = reduced_precision(capi, torch, gen, 20)
         log("bench: config 5 (4096 uneven GEMMs)")
-        cfg["batched"] = batched(capi, torch, gen, 5)
+        cfg["batched"] = batched(capi, torch, gen, 20)
         log("bench: hierarchical layout pack / unpack-add at 8192^2")
-        cfg["hier_layout"] = hier(capi, torch, gen, 10)
+        cfg["hier_layout"] = hier(capi, torch, gen, 20)
         result["configs"] = cfg
         torch.cuda.empty_cache()
 
     if not args.no_cpu:
-        threads = args.cpu_threads or min(16, len(os.sched_getaffinity(0)))
-        log(f"bench: CPU baseline on {threads} threads")
-        result["cpu_baseline"] = cpu_baseline(threads)
+        info = host_cpus()
+        # one thread per physical core the process may use, within its cgroup share
+        threads = info["physical_cores"]
+        if info["cpu_share"]:
+            threads = min(threads, info["cpu_share"])
+        threads = args.cpu_threads or max(1, threads)
+        log(f"bench: CPU baseline on {threads} threads ({info['model']}, "
+            f"{info['physical_cores']} physical cores, share {info['cpu_share']})")
+        cb = cpu_baseline(*cpu_inputs, threads, info)
+        cb["config1_1024cube"]["gpu_tflops"] = round(c1_gpu_tf, 3)
+        result["cpu_baseline"] = cb
 
     print(json.dumps(result), flush=True)
     return 0

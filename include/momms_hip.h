@@ -183,6 +183,38 @@ int momms_hip_crt_bf16_reconstruct(int64_t m, int64_t n, int64_t k, const double
                                    int64_t rsC, int64_t csC, int moduli, const void* workspace,
                                    size_t workspace_bytes, void* stream);
 
+/* ---- FP64-accurate GEMM from INT8 residues (Ozaki scheme II on INT8) ----------
+ * No reference counterpart: the residue form above on INT8 matrix cores
+ * (v_mfma_i32_32x32x32_i8, twice the BF16 rate). The moduli are the N
+ * (1..16) largest pairwise-coprime odd integers <= 255
+ * (momms_hip_crt_i8_modulus), whose balanced residues are int8 values; the
+ * integer width beta follows the same rule over their product
+ * (momms_hip_crt_i8_beta: 15 moduli give the 51 bits of 13 BF16 moduli at
+ * k = 8192), and the INT32 sums are exact without reduction for
+ * k <= 131072. The result is the same function of beta as the BF16 form's,
+ * and equals, bit for bit, oracle/crt.c's INT8 restatement. Arguments,
+ * workspace rules and return codes as the BF16 residue form; k > 131072
+ * returns -3. */
+int momms_hip_crt_i8_beta(int64_t k, int moduli);     /* -1 if invalid */
+int momms_hip_crt_i8_modulus(int index);              /* 0 outside 0..15 */
+size_t momms_hip_crt_i8_workspace_size(int64_t m, int64_t n, int64_t k, int moduli);
+int momms_hip_dgemm_crt_i8(int64_t m, int64_t n, int64_t k, const double* alpha,
+                           const double* A, int64_t rsA, int64_t csA, const double* B,
+                           int64_t rsB, int64_t csB, const double* beta, double* C,
+                           int64_t rsC, int64_t csC, int moduli, void* workspace,
+                           size_t workspace_bytes, void* stream);
+/* Staged form, as momms_hip_crt_bf16_prepare / _products / _reconstruct. */
+int momms_hip_crt_i8_prepare(int64_t m, int64_t n, int64_t k, const double* A, int64_t rsA,
+                             int64_t csA, const double* B, int64_t rsB, int64_t csB,
+                             int moduli, void* workspace, size_t workspace_bytes, void* stream);
+int momms_hip_crt_i8_products(int64_t m, int64_t n, int64_t k, int moduli, void* workspace,
+                              size_t workspace_bytes, void* stream);
+int momms_hip_crt_i8_reconstruct(int64_t m, int64_t n, int64_t k, const double* alpha,
+                                 const double* A, int64_t rsA, int64_t csA, const double* B,
+                                 int64_t rsB, int64_t csB, const double* beta, double* C,
+                                 int64_t rsC, int64_t csC, int moduli, const void* workspace,
+                                 size_t workspace_bytes, void* stream);
+
 /* ---- batched uneven GEMMs -----------------------------------------------------
  * No reference counterpart: the batching addition. Problem i of a batch is
  * C_i := beta*C_i + alpha*A_i*B_i with column-major operands at element

@@ -80,6 +80,19 @@ _SIGS: dict[str, list] = {
                                    _c_i64, _c_int, _c_vp, ctypes.c_size_t, _c_vp],
     "momms_hip_crt_bf16_products": [_c_i64, _c_i64, _c_i64, _c_int, _c_vp, ctypes.c_size_t,
                                     _c_vp],
+    "momms_hip_crt_i8_beta": [_c_i64, _c_int],
+    "momms_hip_crt_i8_modulus": [_c_int],
+    "momms_hip_crt_i8_workspace_size": [_c_i64, _c_i64, _c_i64, _c_int],
+    "momms_hip_dgemm_crt_i8": [_c_i64, _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_vp,
+                               _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_int, _c_vp,
+                               ctypes.c_size_t, _c_vp],
+    "momms_hip_crt_i8_prepare": [_c_i64, _c_i64, _c_i64, _c_vp, _c_i64, _c_i64, _c_vp, _c_i64,
+                                 _c_i64, _c_int, _c_vp, ctypes.c_size_t, _c_vp],
+    "momms_hip_crt_i8_products": [_c_i64, _c_i64, _c_i64, _c_int, _c_vp, ctypes.c_size_t,
+                                  _c_vp],
+    "momms_hip_crt_i8_reconstruct": [_c_i64, _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64,
+                                     _c_vp, _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64,
+                                     _c_int, _c_vp, ctypes.c_size_t, _c_vp],
     "momms_hier_layout_init": [_c_vp, _c_i64, _c_i64, _c_vp, _c_i64, ctypes.c_int32,
                                ctypes.c_int32, _c_i64, _c_i64, _c_i64, _c_i64],
     "momms_hip_hier_pack": [_c_vp, _c_vp, _c_i64, _c_i64, _c_vp, _c_vp],
@@ -92,6 +105,7 @@ _SIGS: dict[str, list] = {
 _RESTYPES = {"momms_last_error_string": ctypes.c_char_p,
              "momms_hip_split_bf16_workspace_size": ctypes.c_size_t,
              "momms_hip_crt_bf16_workspace_size": ctypes.c_size_t,
+             "momms_hip_crt_i8_workspace_size": ctypes.c_size_t,
              "momms_sample_index": ctypes.c_uint64,
              "momms_hip_batch_plan_describe": ctypes.c_char_p}
 
@@ -389,13 +403,17 @@ def dgemm_split_bf16(A, B, C, slices: int, alpha: float = 1.0, beta: float = 1.0
 
 
 class CrtWorkspace:
-    """Device workspace of the Chinese-remainder split GEMM for one (m, n, k, moduli)."""
+    """Device workspace of a residue split GEMM for one (m, n, k, moduli):
+    form "bf16" (BF16 residues, moduli <= 511) or "i8" (INT8 residues,
+    moduli <= 255)."""
 
-    def __init__(self, m: int, n: int, k: int, moduli: int, device="cuda"):
+    def __init__(self, m: int, n: int, k: int, moduli: int, device="cuda", form: str = "bf16"):
         import torch
 
-        self.m, self.n, self.k, self.moduli = m, n, k, moduli
-        self.nbytes = int(lib().momms_hip_crt_bf16_workspace_size(m, n, k, moduli))
+        if form not in ("bf16", "i8"):
+            raise ValueError(f"unknown residue form {form!r}")
+        self.m, self.n, self.k, self.moduli, self.form = m, n, k, moduli, form
+        self.nbytes = int(getattr(lib(), f"momms_hip_crt_{form}_workspace_size")(m, n, k, moduli))
         if self.nbytes == 0:
             raise ValueError(f"invalid residue problem m={m} n={n} k={k} moduli={moduli}")
         self.buf = torch.empty(self.nbytes, dtype=torch.uint8, device=device)
@@ -404,19 +422,22 @@ class CrtWorkspace:
     def ptr(self) -> int:
         return self.buf.data_ptr()
 
+    def _fn(self, stage: str):
+        name = f"momms_hip_crt_{self.form}_{stage}"
+        return name, getattr(lib(), name)
+
     def prepare(self, A, B, stream=None):
         _f64((A, "A"), (B, "B"))
         a, rsa, csa = _mat(A, self.m, self.k, "A")
         b, rsb, csb = _mat(B, self.k, self.n, "B")
-        st = lib().momms_hip_crt_bf16_prepare(self.m, self.n, self.k, a, rsa, csa, b, rsb, csb,
-                                             self.moduli, self.ptr, self.nbytes,
-                                             stream_handle(stream))
-        check("momms_hip_crt_bf16_prepare", st)
+        name, f = self._fn("prepare")
+        check(name, f(self.m, self.n, self.k, a, rsa, csa, b, rsb, csb, self.moduli, self.ptr,
+                      self.nbytes, stream_handle(stream)))
 
     def products(self, stream=None):
-        st = lib().momms_hip_crt_bf16_products(self.m, self.n, self.k, self.moduli, self.ptr,
-                                              self.nbytes, stream_handle(stream))
-        check("momms_hip_crt_bf16_products", st)
+        name, f = self._fn("products")
+        check(name, f(self.m, self.n, self.k, self.moduli, self.ptr, self.nbytes,
+                      stream_handle(stream)))
 
     def reconstruct(self, A, B, C, alpha=1.0, beta=1.0, stream=None):
         _f64((A, "A"), (B, "B"), (C, "C"))
@@ -424,41 +445,52 @@ class CrtWorkspace:
         b, rsb, csb = _mat(B, self.k, self.n, "B")
         c, rsc, csc = _mat(C, self.m, self.n, "C", output=True)
         al, be = ctypes.c_double(alpha), ctypes.c_double(beta)
-        st = lib().momms_hip_crt_bf16_reconstruct(self.m, self.n, self.k, ctypes.byref(al), a,
-                                                 rsa, csa, b, rsb, csb, ctypes.byref(be), c, rsc,
-                                                 csc, self.moduli, self.ptr, self.nbytes,
-                                                 stream_handle(stream))
-        check("momms_hip_crt_bf16_reconstruct", st)
+        name, f = self._fn("reconstruct")
+        check(name, f(self.m, self.n, self.k, ctypes.byref(al), a, rsa, csa, b, rsb, csb,
+                      ctypes.byref(be), c, rsc, csc, self.moduli, self.ptr, self.nbytes,
+                      stream_handle(stream)))
         return C
 
 
-def crt_beta(k: int, moduli: int) -> int:
+def crt_beta(k: int, moduli: int, form: str = "bf16") -> int:
     """Bits of the exponent-aligned integers for k and `moduli` (-1: too few)."""
-    return int(lib().momms_hip_crt_bf16_beta(k, moduli))
+    return int(getattr(lib(), f"momms_hip_crt_{form}_beta")(k, moduli))
 
 
-def crt_moduli(count: int) -> list[int]:
-    return [int(lib().momms_hip_crt_bf16_modulus(i)) for i in range(count)]
+def crt_moduli(count: int, form: str = "bf16") -> list[int]:
+    f = getattr(lib(), f"momms_hip_crt_{form}_modulus")
+    return [int(f(i)) for i in range(count)]
 
 
-def dgemm_crt_bf16(A, B, C, moduli: int, alpha: float = 1.0, beta: float = 1.0,
-                   workspace: CrtWorkspace | None = None, stream=None):
-    """FP64-accurate C := beta*C + alpha*A*B from BF16 residues (in place on C)."""
+def _dgemm_crt(form, A, B, C, moduli, alpha, beta, workspace, stream):
     _f64((A, "A"), (B, "B"), (C, "C"))
     m, k = A.shape
     n = B.shape[1]
-    ws = workspace or _owned(CrtWorkspace(m, n, k, moduli, device=A.device), stream)
-    if (ws.m, ws.n, ws.k, ws.moduli) != (m, n, k, moduli):
+    ws = workspace or _owned(CrtWorkspace(m, n, k, moduli, device=A.device, form=form), stream)
+    if (ws.m, ws.n, ws.k, ws.moduli, ws.form) != (m, n, k, moduli, form):
         raise ValueError("workspace was made for another problem")
     a, rsa, csa = _mat(A, m, k, "A")
     b, rsb, csb = _mat(B, k, n, "B")
     c, rsc, csc = _mat(C, m, n, "C", output=True)
     al, be = ctypes.c_double(alpha), ctypes.c_double(beta)
-    st = lib().momms_hip_dgemm_crt_bf16(m, n, k, ctypes.byref(al), a, rsa, csa, b, rsb, csb,
-                                        ctypes.byref(be), c, rsc, csc, moduli, ws.ptr,
-                                        ws.nbytes, stream_handle(stream))
-    check("momms_hip_dgemm_crt_bf16", st)
+    name = f"momms_hip_dgemm_crt_{form}"
+    st = getattr(lib(), name)(m, n, k, ctypes.byref(al), a, rsa, csa, b, rsb, csb,
+                              ctypes.byref(be), c, rsc, csc, moduli, ws.ptr, ws.nbytes,
+                              stream_handle(stream))
+    check(name, st)
     return C
+
+
+def dgemm_crt_bf16(A, B, C, moduli: int, alpha: float = 1.0, beta: float = 1.0,
+                   workspace: CrtWorkspace | None = None, stream=None):
+    """FP64-accurate C := beta*C + alpha*A*B from BF16 residues (in place on C)."""
+    return _dgemm_crt("bf16", A, B, C, moduli, alpha, beta, workspace, stream)
+
+
+def dgemm_crt_i8(A, B, C, moduli: int, alpha: float = 1.0, beta: float = 1.0,
+                 workspace: CrtWorkspace | None = None, stream=None):
+    """FP64-accurate C := beta*C + alpha*A*B from INT8 residues (in place on C)."""
+    return _dgemm_crt("i8", A, B, C, moduli, alpha, beta, workspace, stream)
 
 
 def sampled_error(A, B, C0, C, alpha=1.0, beta=1.0, samples=65536, seed=0, ref_out=None,

@@ -41,6 +41,7 @@
 
 #include "crt_bf16.hpp"
 #include "c_update.hpp"
+#include "crt_common.hpp"
 #include "crt_recon.hpp"
 #include "panel_gemm.hpp"
 #include "mfma.hpp"
@@ -53,52 +54,36 @@ namespace momms {
 // ---------------------------------------------------------------------------
 // moduli and the constants derived from them
 
-struct CrtTables {
-  float pf[kCrtMaxModuli], invpf[kCrtMaxModuli];
-  double pd[kCrtMaxModuli], invpd[kCrtMaxModuli];
-};
-
-constexpr CrtTables make_crt_tables() {
-  CrtTables T{};
-  for (int l = 0; l < kCrtMaxModuli; ++l) {
-    T.pf[l] = (float)kCrtModuli[l];
-    T.invpf[l] = 1.0f / (float)kCrtModuli[l];
-    T.pd[l] = (double)kCrtModuli[l];
-    T.invpd[l] = 1.0 / (double)kCrtModuli[l];
-  }
-  return T;
-}
-
-__constant__ CrtTables kCrt = make_crt_tables();
-
 int crt_modulus(int l) { return (l >= 0 && l < kCrtMaxModuli) ? kCrtModuli[l] : 0; }
 
-static int product_bitlen(int N) {  // exact, in 32-bit limbs
-  uint32_t w[8] = {1, 0, 0, 0, 0, 0, 0, 0};
+int crt_beta_for(int64_t k, int N, const int* mods) {
+  if (N < 1 || N > kCrtMaxModuli || k < 0) return -1;
+  uint32_t w[8] = {1, 0, 0, 0, 0, 0, 0, 0};  // P, exactly, in 32-bit limbs
   for (int l = 0; l < N; ++l) {
     uint64_t carry = 0;
     for (int i = 0; i < 8; ++i) {
-      const uint64_t t = (uint64_t)w[i] * (uint64_t)kCrtModuli[l] + carry;
+      const uint64_t t = (uint64_t)w[i] * (uint64_t)mods[l] + carry;
       w[i] = (uint32_t)t;
       carry = t >> 32;
     }
   }
+  int bitlen = 0;
   for (int i = 7; i >= 0; --i)
-    if (w[i]) return 32 * i + (32 - __builtin_clz(w[i]));
-  return 0;
-}
-
-int crt_beta(int64_t k, int N) {
-  if (N < 1 || N > kCrtMaxModuli || k < 0) return -1;
+    if (w[i]) {
+      bitlen = 32 * i + (32 - __builtin_clz(w[i]));
+      break;
+    }
   int kb = 0;
   while (((int64_t)1 << kb) < k) ++kb;
-  const int room = product_bitlen(N) - 2 - kb;
+  const int room = bitlen - 2 - kb;
   int b = room >= 0 ? room / 2 : -1;
   const int cap = (125 - kb) / 2;
   if (b > cap) b = cap;
   if (b > 53) b = 53;
   return b;
 }
+
+int crt_beta(int64_t k, int N) { return crt_beta_for(k, N, kCrtModuli); }
 
 static inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
@@ -130,93 +115,6 @@ CrtLayout crt_layout(int64_t m, int64_t n, int64_t k, int N) {
 }
 
 // ---------------------------------------------------------------------------
-// 2. residues in packed panels.
-//
-// Vector v (a row of A or a column of B), element p at X[v*sv + p*sp]. For
-// modulus l the residue goes to
-//   R[(l*(vpad/PANEL) + pv)*S*(PANEL*16) + ks*(PANEL*16) + kh*PANEL*8 + (v%PANEL)*8 + p%8]
-// with pv = v / PANEL, ks = p / 16, kh = (p % 16) / 8 and S = panel_stride(lpad/16)
-// slabs per panel (one unused slab between panels); padding is 0. A 64 x 64
-// block of (v, p) goes through LDS so the global read (along whichever index
-// is unit-stride) and the 16-byte packed writes are both coalesced.
-
-template <int PANEL>
-__global__ void __launch_bounds__(256)
-crt_pack(const double* __restrict__ X, int64_t count, int64_t len, int64_t sv, int64_t sp,
-         const int* __restrict__ ex, int64_t vpad, int64_t lpad, int nmod, int beta,
-         uint16_t* __restrict__ R) {
-  __shared__ double tile[64][65];
-  const int tid = threadIdx.x;
-  const int64_t v0 = (int64_t)blockIdx.x * 64, p0 = (int64_t)blockIdx.y * 64;
-  const bool v_unit = (sv == 1);
-  if (count > 0 && len > 0) {  // (the same for the whole grid: X has an element 0)
-    // all 16 loads first (X[0] for elements outside X, masked after), then the
-    // LDS stores, so the loads of a thread are in flight together
-    double x[16];
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int e = tid + 256 * r;
-      const int a = e & 63, b = e >> 6;  // a runs fastest
-      const int64_t gv = v0 + (v_unit ? a : b), gp = p0 + (v_unit ? b : a);
-      x[r] = X[(gv < count && gp < len) ? gv * sv + gp * sp : 0];
-    }
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int e = tid + 256 * r;
-      const int a = e & 63, b = e >> 6;
-      const int vv = v_unit ? a : b, pp = v_unit ? b : a;
-      tile[vv][pp] = (v0 + vv < count && p0 + pp < len) ? x[r] : 0.0;
-    }
-  } else {
-#pragma unroll
-    for (int r = 0; r < 16; ++r) {
-      const int e = tid + 256 * r;
-      const int a = e & 63, b = e >> 6;
-      tile[v_unit ? a : b][v_unit ? b : a] = 0.0;
-    }
-  }
-  __syncthreads();
-  const int64_t nslabs = lpad / kSlab;
-  const int64_t mod_stride = vpad * panel_stride(nslabs) * kSlab;
-#pragma unroll
-  for (int r = 0; r < 2; ++r) {
-    const int c = tid + 256 * r;
-    const int vv = c & 63;
-    const int pc = c >> 6;  // chunk of 8 along p
-    const int64_t gv = v0 + vv;
-    const int64_t gp = p0 + pc * 8;
-    if (gv >= vpad || gp >= lpad) continue;
-    const int E = (gv < count) ? ex[gv] : 0;
-    const bool finite = (E != kNonFinite);
-    double q[8];
-#pragma unroll
-    for (int e = 0; e < 8; ++e) q[e] = finite ? rint(ldexp(tile[vv][pc * 8 + e], beta - E)) : 0.0;
-    const int64_t pv = gv / PANEL, ks = gp / kSlab;
-    const int kh = (int)((gp % kSlab) / 8);
-    uint16_t* dst = R + (pv * panel_stride(nslabs) + ks) * (PANEL * kSlab) +
-                    (int64_t)kh * PANEL * 8 + (gv % PANEL) * 8;
-    for (int l = 0; l < nmod; ++l) {
-      const double p = kCrt.pd[l], invp = kCrt.invpd[l], h = 0.5 * (p - 1.0);
-      uint32_t bits[8];
-#pragma unroll
-      for (int e = 0; e < 8; ++e) {
-        // q - p*rint(q/p) is an integer below 0.51 p in magnitude: exact by fma
-        double rr = __builtin_fma(-rint(q[e] * invp), p, q[e]);
-        rr = rr > h ? rr - p : rr;
-        rr = rr < -h ? rr + p : rr;
-        bits[e] = __float_as_uint((float)rr) >> 16;  // |rr| <= 255: exact in bf16
-      }
-      uint4 w;
-      w.x = bits[0] | (bits[1] << 16);
-      w.y = bits[2] | (bits[3] << 16);
-      w.z = bits[4] | (bits[5] << 16);
-      w.w = bits[6] | (bits[7] << 16);
-      *reinterpret_cast<uint4*>(dst + (int64_t)l * mod_stride) = w;
-    }
-  }
-}
-
-// ---------------------------------------------------------------------------
 // 3. one residue GEMM per (modulus, tile): the panel GEMM (panel_gemm.hpp)
 // with the modular reduction every 256 k-values, ending as the balanced
 // residue of C' modulo p_l (int16, column-major, leading dimension mpad).
@@ -239,7 +137,7 @@ crt_multiply(int64_t tiles_m, int64_t tiles_n, int64_t nslabs, int64_t mpad, int
   const int l = (int)(t / tiles);
   int64_t tm, tn;
   grouped_tile(t - (int64_t)l * tiles, tiles_m, tiles_n, 8, tm, tn);
-  const float p = kCrt.pf[l], invp = kCrt.invpf[l];
+  const float p = kCrtBf16Tab.pf[l], invp = kCrtBf16Tab.invpf[l];
 
   v16f32 acc[FM][FN];
 #pragma unroll
@@ -274,97 +172,6 @@ crt_multiply(int64_t tiles_m, int64_t tiles_n, int64_t nslabs, int64_t mpad, int
 }
 
 // ---------------------------------------------------------------------------
-// 4. reconstruction.
-
-// One thread: kCrtRows consecutive rows i0.. of one column j (the residue
-// matrices are padded to whole tiles, so the residue loads of kCrtRows int16
-// stay in bounds); the constants of each modulus are loaded once for all of
-// them. Two rows per thread keep the sums and the loads in flight at 77
-// registers, six wavefronts per SIMD, so the arithmetic of some overlaps the
-// memory traffic of others (eight rows: 150 registers, three wavefronts per
-// SIMD, 0.88 ms at 8192^2 and 13 moduli; four rows 0.81 ms; two rows 0.79 ms).
-constexpr int kCrtRows = 2;
-static_assert(kCrtRows == 2 || kCrtRows == 4, "residue load width");
-
-template <int N>
-__global__ void __launch_bounds__(256)
-crt_reconstruct(int64_t m, int64_t n, int64_t k, int beta_bits, double alpha, double beta,
-                double* __restrict__ C, int64_t rsC, int64_t csC, const int16_t* __restrict__ RC,
-                int64_t mpad, int64_t npad, const int* __restrict__ EA,
-                const int* __restrict__ EB, const double* __restrict__ A, int64_t rsA,
-                int64_t csA, const double* __restrict__ B, int64_t rsB, int64_t csB) {
-  constexpr CrtRecon K = crt_recon_constants(N);
-  // a thread reconstructs rows i0 .. i0+kCrtRows-1 of column j;
-  // the results go through LDS so that the C update of the block runs along
-  // columns, 64 consecutive rows per wavefront access
-  __shared__ double vt[256 * (kCrtRows + 1)];
-  const int tid = threadIdx.x;
-  const int64_t rb = (int64_t)blockIdx.x * (256 * kCrtRows);
-  const int64_t i0 = rb + (int64_t)tid * kCrtRows;
-  const bool active = i0 < m;  // (rows up to i0 + kCrtRows - 1 < mpad then)
-  for (int64_t j = blockIdx.y; j < n; j += gridDim.y) {
-    double v[kCrtRows];
-#pragma unroll
-    for (int e = 0; e < kCrtRows; ++e) v[e] = 0.0;
-    if (active) {
-      CrtSums a[kCrtRows];
-#pragma unroll
-      for (int e = 0; e < kCrtRows; ++e) crt_sums_init(a[e]);
-#pragma unroll
-      for (int l = 0; l < N; ++l) {
-        uint32_t h[kCrtRows / 2];  // kCrtRows residues, two per 32-bit word
-        {
-          const int16_t* src = RC + (int64_t)l * mpad * npad + j * mpad + i0;
-          if constexpr (kCrtRows == 4) {
-            const uint2 w = *reinterpret_cast<const uint2*>(src);
-            h[0] = w.x;
-            h[kCrtRows / 2 - 1] = w.y;
-          } else {
-            h[0] = *reinterpret_cast<const uint32_t*>(src);
-          }
-        }
-#pragma unroll
-        for (int e = 0; e < kCrtRows; ++e) {
-          const int16_t r = (int16_t)(h[e / 2] >> (16 * (e % 2)));
-          crt_sums_add(a[e], (float)r, l, K);
-        }
-      }
-      const int ej = EB[j];
-#pragma unroll
-      for (int e = 0; e < kCrtRows; ++e) {
-        const int64_t i = i0 + e;
-        const int ei = EA[i];  // EA holds mpad entries
-        if (i < m && (ei == kNonFinite || ej == kNonFinite)) {
-          double s = 0.0;
-          for (int64_t p = 0; p < k; ++p) s = __builtin_fma(A[i * rsA + p * csA], B[p * rsB + j * csB], s);
-          v[e] = alpha * s;
-        } else {
-          v[e] = alpha * ldexp(crt_i128_to_f64(crt_sums_value(a[e], K)), ei + ej - 2 * beta_bits);
-        }
-      }
-    }
-    // row rb + q sits at vt[(q / kCrtRows) * (kCrtRows + 1) + q % kCrtRows] (the
-    // pad keeps the stores of a wavefront on distinct banks)
-#pragma unroll
-    for (int e = 0; e < kCrtRows; ++e) vt[tid * (kCrtRows + 1) + e] = v[e];
-    __syncthreads();
-    int64_t idx[kCrtRows];
-    bool ok[kCrtRows];
-    double w[kCrtRows];
-#pragma unroll
-    for (int e = 0; e < kCrtRows; ++e) {
-      const int q = tid + 256 * e;
-      const int64_t i = rb + q;
-      ok[e] = i < m;
-      idx[e] = i * rsC + j * csC;
-      w[e] = vt[(q / kCrtRows) * (kCrtRows + 1) + q % kCrtRows];
-    }
-    __syncthreads();  // vt free for the next column
-    update_c(C, idx, ok, w, beta);
-  }
-}
-
-// ---------------------------------------------------------------------------
 // host side
 
 constexpr int kCrtFM = 2, kCrtFN = 4, kCrtWM = 4, kCrtWN = 2, kCrtSPS = 2;
@@ -384,10 +191,10 @@ hipError_t crt_prepare(const CrtLayout& L, int64_t m, int64_t n, int64_t k, cons
   if ((e = vector_exponents(A, m, k, rsA, csA, L.mpad, ma, ea, st)) != hipSuccess) return e;
   if ((e = vector_exponents(B, n, k, csB, rsB, L.npad, mb, eb, st)) != hipSuccess) return e;
   dim3 ga((unsigned)ceil_div(L.mpad, 64), (unsigned)ceil_div(L.kpad, 64)), block(256);
-  hipLaunchKernelGGL((crt_pack<kCrtBM>), ga, block, 0, st, A, m, k, rsA, csA, ea, L.mpad, L.kpad,
+  hipLaunchKernelGGL((crt_pack<kCrtBM, false>), ga, block, 0, st, A, m, k, rsA, csA, ea, L.mpad, L.kpad,
                      L.nmod, L.beta, ra);
   dim3 gb((unsigned)ceil_div(L.npad, 64), (unsigned)ceil_div(L.kpad, 64));
-  hipLaunchKernelGGL((crt_pack<kCrtBN>), gb, block, 0, st, B, n, k, csB, rsB, eb, L.npad, L.kpad,
+  hipLaunchKernelGGL((crt_pack<kCrtBN, false>), gb, block, 0, st, B, n, k, csB, rsB, eb, L.npad, L.kpad,
                      L.nmod, L.beta, rb);
   return hipGetLastError();
 }
@@ -406,11 +213,11 @@ hipError_t crt_reconstruct_launch(const CrtLayout& L, int64_t m, int64_t n, int6
                                   double alpha, double beta, double* C, int64_t rsC, int64_t csC,
                                   const double* A, int64_t rsA, int64_t csA, const double* B,
                                   int64_t rsB, int64_t csB, const char* ws, hipStream_t st) {
-  const int16_t* rc = (const int16_t*)(ws + L.off_rc);
+  const void* rc = ws + L.off_rc;
   const int* ea = (const int*)(ws + L.off_ea);
   const int* eb = (const int*)(ws + L.off_eb);
   dim3 grid((unsigned)ceil_div(m, 256 * kCrtRows), (unsigned)(n < 65535 ? n : 65535)), block(256);
-#define MOMMS_RC(N_) hipLaunchKernelGGL((crt_reconstruct<N_>), grid, block, 0, st, m, n, k, L.beta, alpha, beta, C, rsC, csC, rc, L.mpad, L.npad, ea, eb, A, rsA, csA, B, rsB, csB)
+#define MOMMS_RC(N_) hipLaunchKernelGGL((crt_reconstruct<N_, false>), grid, block, 0, st, m, n, k, L.beta, alpha, beta, C, rsC, csC, rc, L.mpad, L.npad, ea, eb, A, rsA, csA, B, rsB, csB)
   switch (L.nmod) {
     case 1: MOMMS_RC(1); break;
     case 2: MOMMS_RC(2); break;

@@ -1,7 +1,8 @@
-// Chinese remainder step of the residue split GEMM (crt_bf16.hip), shared by
-// the device kernel and a host build of the same code (tests/native), so the
-// integer reconstruction is checked on the CPU against exact integers,
-// including the ends of the balanced range (-P/2, P/2).
+// Chinese remainder step of the residue split GEMMs (crt_bf16.hip with BF16
+// residues, crt_i8.hip with INT8 residues), shared by the device kernels and
+// a host build of the same code (tests/native), so the integer
+// reconstruction is checked on the CPU against exact integers, including the
+// ends of the balanced range (-P/2, P/2).
 //
 // For N moduli p_1..p_N (P = p_1...p_N, M_l = P / p_l, y_l = M_l^-1 mod p_l)
 // and residues r_l of an integer C' with |C'| < P/2 and |C'| < 2^126:
@@ -12,9 +13,10 @@
 // (wrapping), exact because C' fits.
 // sum_l t_l / p_l is within 10^-10 of C'/P + q_true, so q is exact unless
 // |C'| is within 10^-10 P of P/2. That needs P < 2 k 2^(2 beta) (1 + 10^-10),
-// which only the uncapped beta of N <= 14 can give; there P < 2^127, so C'
-// +- P still fits and one compare against floor(P/2) fixes it. For N >= 15
-// beta is capped, |C'| <= 2^125 < 2^-8 P and q is always exact.
+// which only an uncapped beta can give. When P < 2^127 (BF16 moduli: N <= 14;
+// INT8 moduli: every N), C' +- P still fits and one compare against
+// floor(P/2) fixes it. Otherwise (BF16, N >= 15) beta is capped, |C'| <=
+// 2^125 < 2^-8 P and q is always exact.
 #pragma once
 #include <stdint.h>
 
@@ -27,8 +29,14 @@
 namespace momms {
 
 constexpr int kCrtMaxModuli = 16;
+// BF16 residues: the 16 largest pairwise-coprime odd integers <= 511
+// (balanced residues |r| <= 255, integers bf16 holds exactly).
 constexpr int kCrtModuli[kCrtMaxModuli] = {511, 509, 507, 505, 503, 499, 493, 491,
                                            487, 479, 473, 467, 463, 461, 457, 449};
+// INT8 residues: the 16 largest pairwise-coprime odd integers <= 255
+// (balanced residues |r| <= 127, int8 values).
+constexpr int kCrtI8Moduli[kCrtMaxModuli] = {255, 253, 251, 247, 241, 239, 233, 229,
+                                             227, 223, 217, 211, 199, 197, 193, 191};
 
 constexpr int mod_inverse(int a, int m) {  // extended Euclid, gcd(a, m) = 1
   int t = 0, nt = 1, r = m, nr = a % m;
@@ -53,7 +61,7 @@ struct CrtRecon {
   float y[kCrtMaxModuli];                              // M_l^-1 mod p_l
   float pf[kCrtMaxModuli], invpf[kCrtMaxModuli];      // p_l, fp32(1/p_l)
   double invpd[kCrtMaxModuli];                         // fp64(1/p_l)
-  int fix;                                             // N <= 14: the +-P fix
+  int fix;                                             // P < 2^127: the +-P fix
   int limbs;                                           // 24-bit limbs that P (and so every M_l) needs
 };
 
@@ -89,23 +97,34 @@ constexpr double crt_big_limb24(const CrtBig& a, int k) {  // bits 24k .. 24k+23
 constexpr uint64_t crt_big_lo(const CrtBig& a) { return (uint64_t)a.w[0] | ((uint64_t)a.w[1] << 32); }
 constexpr uint64_t crt_big_hi(const CrtBig& a) { return (uint64_t)a.w[2] | ((uint64_t)a.w[3] << 32); }
 
-// The constants for N moduli; constexpr, so the kernel for a given N holds
-// them as instruction literals (a 24-bit limb as a double has a zero low
-// word: one 32-bit literal).
-constexpr CrtRecon crt_recon_constants(int N) {
+constexpr int crt_big_bitlen(const CrtBig& a) {
+  for (int i = 5; i >= 0; --i)
+    if (a.w[i]) {
+      int b = 0;
+      while (b < 32 && (a.w[i] >> b)) ++b;
+      return 32 * i + b;
+    }
+  return 0;
+}
+
+// The constants for the first N moduli of a table (kCrtModuli or
+// kCrtI8Moduli); constexpr, so the kernel for a given N holds them as
+// instruction literals (a 24-bit limb as a double has a zero low word: one
+// 32-bit literal).
+constexpr CrtRecon crt_recon_constants(int N, const int* mods = kCrtModuli) {
   CrtRecon K{};
   CrtBig P;
   P.w[0] = 1;
-  for (int l = 0; l < N; ++l) crt_big_mul(P, (uint32_t)kCrtModuli[l]);
+  for (int l = 0; l < N; ++l) crt_big_mul(P, (uint32_t)mods[l]);
   for (int l = 0; l < N; ++l) {
     CrtBig M, unused;
-    (void)crt_big_div(P, (uint32_t)kCrtModuli[l], M);
-    const uint32_t mr = crt_big_div(M, (uint32_t)kCrtModuli[l], unused);
+    (void)crt_big_div(P, (uint32_t)mods[l], M);
+    const uint32_t mr = crt_big_div(M, (uint32_t)mods[l], unused);
     for (int k = 0; k < kCrtLimbs; ++k) K.mu[l][k] = crt_big_limb24(M, k);
-    K.y[l] = (float)mod_inverse((int)mr, kCrtModuli[l]);
-    K.pf[l] = (float)kCrtModuli[l];
-    K.invpf[l] = 1.0f / (float)kCrtModuli[l];
-    K.invpd[l] = 1.0 / (double)kCrtModuli[l];
+    K.y[l] = (float)mod_inverse((int)mr, mods[l]);
+    K.pf[l] = (float)mods[l];
+    K.invpf[l] = 1.0f / (float)mods[l];
+    K.invpd[l] = 1.0 / (double)mods[l];
   }
   CrtBig H;
   (void)crt_big_div(P, 2, H);
@@ -114,7 +133,10 @@ constexpr CrtRecon crt_recon_constants(int N) {
   K.p_hi = crt_big_hi(P);
   K.h_lo = crt_big_lo(H);
   K.h_hi = crt_big_hi(H);
-  K.fix = N <= 14 ? 1 : 0;
+  // C' +- P must fit in 128 bits for the compare against floor(P/2): P < 2^127
+  // (BF16 moduli: N <= 14; above, beta is capped and q is always exact;
+  // INT8 moduli: every N <= 16)
+  K.fix = crt_big_bitlen(P) <= 127 ? 1 : 0;
   K.limbs = kCrtLimbs;  // the higher limbs of P are 0: fewer suffice, and M_l < P
   while (K.limbs > 1 && K.pi[K.limbs - 1] == 0.0) --K.limbs;
   return K;
@@ -167,7 +189,8 @@ MOMMS_HD inline __int128 crt_sums_value(const CrtSums& a, const CrtRecon& K) {
   return c;
 }
 
-// C' from its balanced residues r[0..N-1] (integers in fp32).
+// C' from its balanced residues r[0..N-1] (integers in fp32) modulo the
+// moduli K was built for.
 inline __int128 crt_combine(int N, const float* r, const CrtRecon& K) {
   CrtSums a;
   crt_sums_init(a);

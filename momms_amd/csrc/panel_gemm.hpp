@@ -1,6 +1,9 @@
-// BF16 matrix-core GEMM over packed panels, shared by the residue split GEMM
-// (crt_bf16.hip, one GEMM per modulus, FP32 sums reduced modulo p) and the
-// BF16 -> FP32 GEMM (gemm_f32acc.hip, alpha/beta epilogue).
+// BF16 (or INT8) matrix-core GEMM over packed panels, shared by the residue
+// split GEMMs (crt_bf16.hip, one BF16 GEMM per modulus, FP32 sums reduced
+// modulo p; crt_i8.hip, one INT8 GEMM per modulus, exact INT32 sums) and the
+// BF16 -> FP32 GEMM (gemm_f32acc.hip, alpha/beta epilogue). An INT8 panel has
+// the byte layout of a BF16 one, each 16-byte group holding 16 int8 k-values
+// instead of 8 bf16 ones: a slab is 32 k-values deep.
 //
 // Panel layout (written by the pack kernels): for a panel of PANEL rows of A
 // (columns of B) and 16-deep k-slab ks, one contiguous block [kh][PANEL][8]
@@ -78,6 +81,27 @@ __device__ __forceinline__ void lds_dma16_untracked(const void* src, const void*
 }
 #pragma clang diagnostic pop
 
+// The matrix-core operation on 16-byte fragments of a slab: BF16 (a 16-deep
+// slab, FP32 sums) or INT8 (the same bytes hold a 32-deep slab, exact INT32
+// sums). Both put lane l's fragment at row l%32 and k-half l/32 of a 32-row
+// block, and both return a 32 x 32 block in the same lane layout.
+template <bool I8>
+struct PanelOps {
+  typedef v8bf16 frag;
+  typedef v16f32 acc;
+  static __device__ __forceinline__ acc mma(frag a, frag b, acc c) {
+    return mfma_bf16_32x32x16(a, b, c);
+  }
+};
+template <>
+struct PanelOps<true> {
+  typedef v4i32 frag;
+  typedef v16i32 acc;
+  static __device__ __forceinline__ acc mma(frag a, frag b, acc c) {
+    return mfma_i8_32x32x32(a, b, c);
+  }
+};
+
 template <int FM, int FN>
 __device__ __forceinline__ void panel_reduce_mod(v16f32 (&acc)[FM][FN], float p, float invp) {
 #pragma unroll
@@ -118,8 +142,8 @@ __device__ __forceinline__ void panel_slab_order() {
 // chunk end. Between two reductions of a block lie exactly 256 k-values
 // (|x| < 0.51 p + 256 * 255^2 < 2^24), and the caller reduces once more after
 // the loop, so the residues equal those of a reduction at every chunk end.
-template <int FM, int FN, int WM, int WN, int SPS, bool MOD>
-__device__ __forceinline__ void panel_mainloop(v16f32 (&acc)[FM][FN],
+template <int FM, int FN, int WM, int WN, int SPS, bool MOD, bool I8 = false>
+__device__ __forceinline__ void panel_mainloop(typename PanelOps<I8>::acc (&acc)[FM][FN],
                                                const uint16_t* __restrict__ Abase,
                                                const uint16_t* __restrict__ Bbase,
                                                int64_t nslabs, float p, float invp, uint16_t* S0,
@@ -130,6 +154,8 @@ __device__ __forceinline__ void panel_mainloop(v16f32 (&acc)[FM][FN],
   static_assert((256 / kSlab) % SPS == 0 && CHUNK_STAGES % 4 == 0 && CHUNK_STAGES <= 16,
                 "a chunk is whole stages and whole turns of the four-slot ring");
   static_assert(!MOD || NQ >= CHUNK_STAGES, "every stage of a chunk reduces some block");
+  static_assert(!(MOD && I8), "INT8 sums are exact without reductions");
+  typedef typename PanelOps<I8>::frag Frag;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
   const int64_t nst = nslabs / SPS;
@@ -150,21 +176,21 @@ __device__ __forceinline__ void panel_mainloop(v16f32 (&acc)[FM][FN],
   const int kh = lane >> 5;
   const int a_off = kh * G::BM * 8 + (wm * 32 * FM + (lane & 31)) * 8;
   const int b_off = G::BOFF + kh * G::BN * 8 + (wn * 32 * FN + (lane & 31)) * 8;
-  v8bf16 af[FM], bf[FN], an[FM], bn[FN];
-  auto rd = [&](const uint16_t* S, int ss, v8bf16 (&xa)[FM], v8bf16 (&xb)[FN]) {
+  Frag af[FM], bf[FN], an[FM], bn[FN];
+  auto rd = [&](const uint16_t* S, int ss, Frag (&xa)[FM], Frag (&xb)[FN]) {
 #pragma unroll
     for (int a = 0; a < FM; ++a)
-      xa[a] = *reinterpret_cast<const v8bf16*>(&S[ss * G::ABLK + a_off + a * 32 * 8]);
+      xa[a] = *reinterpret_cast<const Frag*>(&S[ss * G::ABLK + a_off + a * 32 * 8]);
 #pragma unroll
     for (int b = 0; b < FN; ++b)
-      xb[b] = *reinterpret_cast<const v8bf16*>(&S[ss * G::BBLK + b_off + b * 32 * 8]);
+      xb[b] = *reinterpret_cast<const Frag*>(&S[ss * G::BBLK + b_off + b * 32 * 8]);
   };
   // one slab's MFMAs, block q = a FN + b, in the order first, first + 1, ... (mod NQ)
   auto mm = [&](int first) {
 #pragma unroll
     for (int t = 0; t < NQ; ++t) {
       const int q = (first + t) % NQ, a = q / FN, b = q % FN;
-      acc[a][b] = mfma_bf16_32x32x16(bf[b], af[a], acc[a][b]);
+      acc[a][b] = PanelOps<I8>::mma(bf[b], af[a], acc[a][b]);
     }
   };
   // stage KS (J: its index in the chunk) from slot Sc; slot Sn holds stage KS + 1

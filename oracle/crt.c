@@ -29,7 +29,10 @@
  * The device computes each residue C' mod p_l with one BF16 matrix-core GEMM
  * of the balanced residues of A' and B' and recovers C' by the Chinese
  * remainder theorem; here C' comes straight from 128-bit integer sums, so
- * equality with the device checks the whole residue path.
+ * equality with the device checks the whole residue path. The INT8 form
+ * (oracle_crt_i8_*) takes the largest pairwise-coprime odd moduli <= 255 and
+ * the same beta rule over their product; its result is the same function of
+ * beta.
  */
 #include <math.h>
 #include <stdlib.h>
@@ -38,16 +41,23 @@
 
 static const int kModuli[ORACLE_CRT_MAX_MODULI] = {511, 509, 507, 505, 503, 499, 493, 491,
                                                    487, 479, 473, 467, 463, 461, 457, 449};
+/* The INT8 residue form: the largest pairwise-coprime odd integers <= 255,
+ * whose balanced residues (|r| <= 127) are int8 values. */
+static const int kModuliI8[ORACLE_CRT_MAX_MODULI] = {255, 253, 251, 247, 241, 239, 233, 229,
+                                                     227, 223, 217, 211, 199, 197, 193, 191};
 
 int oracle_crt_modulus(int l) { return (l >= 0 && l < ORACLE_CRT_MAX_MODULI) ? kModuli[l] : 0; }
+int oracle_crt_i8_modulus(int l) {
+  return (l >= 0 && l < ORACLE_CRT_MAX_MODULI) ? kModuliI8[l] : 0;
+}
 
 /* Bit length of prod_{l<N} p_l, exactly, in 32-bit limbs (16 moduli: 143 bits). */
-static int product_bitlen(int N) {
+static int product_bitlen(int N, const int* mods) {
   uint32_t w[8] = {1, 0, 0, 0, 0, 0, 0, 0};
   for (int l = 0; l < N; ++l) {
     uint64_t carry = 0;
     for (int i = 0; i < 8; ++i) {
-      const uint64_t t = (uint64_t)w[i] * (uint64_t)kModuli[l] + carry;
+      const uint64_t t = (uint64_t)w[i] * (uint64_t)mods[l] + carry;
       w[i] = (uint32_t)t;
       carry = t >> 32;
     }
@@ -57,16 +67,23 @@ static int product_bitlen(int N) {
   return 0;
 }
 
-int oracle_crt_beta(int64_t k, int N) {
+static int beta_for(int64_t k, int N, const int* mods) {
   if (N < 1 || N > ORACLE_CRT_MAX_MODULI) return -1;
   int kb = 0;
   while (((int64_t)1 << kb) < k) ++kb;
-  const int room = product_bitlen(N) - 2 - kb;
+  const int room = product_bitlen(N, mods) - 2 - kb;
   int b = room >= 0 ? room / 2 : -1;
   const int cap = (125 - kb) / 2;
   if (b > cap) b = cap;
   if (b > 53) b = 53;
   return b;
+}
+
+int oracle_crt_beta(int64_t k, int N) { return beta_for(k, N, kModuli); }
+
+/* The INT8 form's INT32 sums are exact while k 127^2 < 2^31. */
+int oracle_crt_i8_beta(int64_t k, int N) {
+  return k > 131072 ? -1 : beta_for(k, N, kModuliI8);
 }
 
 /* Integer images of count vectors of length k: vector v starts at X + v*sv,
@@ -102,11 +119,12 @@ int oracle_crt_balanced(int64_t x, int p) {
 
 static double int128_to_double(__int128 x) { return (double)x; }
 
-int oracle_crt_gemm(int64_t m, int64_t n, int64_t k, const double* alpha, const double* A,
-                    int64_t rsA, int64_t csA, const double* B, int64_t rsB, int64_t csB,
-                    const double* betaC, double* C, int64_t rsC, int64_t csC, int N) {
+/* The result of either residue form is fixed by beta alone (C' is the exact
+ * product of the beta-bit integer images): both entry points below end here. */
+static int crt_gemm_bits(int64_t m, int64_t n, int64_t k, const double* alpha, const double* A,
+                         int64_t rsA, int64_t csA, const double* B, int64_t rsB, int64_t csB,
+                         const double* betaC, double* C, int64_t rsC, int64_t csC, int beta) {
   if (m <= 0 || n <= 0) return 0;
-  const int beta = oracle_crt_beta(k, N);
   if (beta < 1) return -1;
   const int64_t kk = k > 0 ? k : 1;
   int64_t* ia = (int64_t*)malloc(sizeof(int64_t) * (size_t)(m * kk));
@@ -145,4 +163,19 @@ int oracle_crt_gemm(int64_t m, int64_t n, int64_t k, const double* alpha, const 
   free(ea);
   free(eb);
   return 0;
+}
+
+int oracle_crt_gemm(int64_t m, int64_t n, int64_t k, const double* alpha, const double* A,
+                    int64_t rsA, int64_t csA, const double* B, int64_t rsB, int64_t csB,
+                    const double* betaC, double* C, int64_t rsC, int64_t csC, int N) {
+  return crt_gemm_bits(m, n, k, alpha, A, rsA, csA, B, rsB, csB, betaC, C, rsC, csC,
+                       oracle_crt_beta(k, N));
+}
+
+/* The INT8 residue form (N moduli of kModuliI8). */
+int oracle_crt_i8_gemm(int64_t m, int64_t n, int64_t k, const double* alpha, const double* A,
+                       int64_t rsA, int64_t csA, const double* B, int64_t rsB, int64_t csB,
+                       const double* betaC, double* C, int64_t rsC, int64_t csC, int N) {
+  return crt_gemm_bits(m, n, k, alpha, A, rsA, csA, B, rsB, csB, betaC, C, rsC, csC,
+                       oracle_crt_i8_beta(k, N));
 }

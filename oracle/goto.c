@@ -19,10 +19,13 @@
  *     src/composables/part.rs:96-110) bitwise, and the double-double oracle
  *     within gamma_{k+1}.
  */
+#define _GNU_SOURCE /* pthread_setaffinity_np, CPU_SET */
 #include "oracle.h"
 
 #include <immintrin.h>
 #include <pthread.h>
+#include <sched.h>
+#include <stdatomic.h>
 #include <stdlib.h>
 #include <string.h>
 
@@ -38,17 +41,49 @@
  * C[6x8] := beta*C + alpha*(sum_p a_p b_p^T), element-wise fma chain. */
 static void ukernel_6x8(int64_t k, double alpha, const double* a, const double* b,
                         double beta, double* c, int64_t rs_c, int64_t cs_c) {
-  __m256d acc[MR][2];
-  for (int i = 0; i < MR; ++i) acc[i][0] = acc[i][1] = _mm256_setzero_pd();
-  for (int64_t p = 0; p < k; ++p) {
-    const __m256d b0 = _mm256_loadu_pd(b + p * NR);
-    const __m256d b1 = _mm256_loadu_pd(b + p * NR + 4);
-    for (int i = 0; i < MR; ++i) {
-      const __m256d ai = _mm256_broadcast_sd(a + p * MR + i);
-      acc[i][0] = _mm256_fmadd_pd(ai, b0, acc[i][0]);
-      acc[i][1] = _mm256_fmadd_pd(ai, b1, acc[i][1]);
-    }
+  /* twelve accumulators held in registers (the loops over the six rows are
+   * written out: at -O2 gcc keeps a looped array of them on the stack), the
+   * k loop unrolled by four with the panels prefetched a few steps ahead, as
+   * the reference kernel does (d6x8.c:725-843); each element is still one
+   * fma chain in p order */
+  __m256d c00 = _mm256_setzero_pd(), c01 = c00, c10 = c00, c11 = c00, c20 = c00, c21 = c00;
+  __m256d c30 = c00, c31 = c00, c40 = c00, c41 = c00, c50 = c00, c51 = c00;
+#define MOMMS_STEP(p)                                              \
+  do {                                                             \
+    const __m256d b0 = _mm256_loadu_pd(b + (p) * NR);              \
+    const __m256d b1 = _mm256_loadu_pd(b + (p) * NR + 4);          \
+    __m256d ai = _mm256_broadcast_sd(a + (p) * MR + 0);            \
+    c00 = _mm256_fmadd_pd(ai, b0, c00);                            \
+    c01 = _mm256_fmadd_pd(ai, b1, c01);                            \
+    ai = _mm256_broadcast_sd(a + (p) * MR + 1);                    \
+    c10 = _mm256_fmadd_pd(ai, b0, c10);                            \
+    c11 = _mm256_fmadd_pd(ai, b1, c11);                            \
+    ai = _mm256_broadcast_sd(a + (p) * MR + 2);                    \
+    c20 = _mm256_fmadd_pd(ai, b0, c20);                            \
+    c21 = _mm256_fmadd_pd(ai, b1, c21);                            \
+    ai = _mm256_broadcast_sd(a + (p) * MR + 3);                    \
+    c30 = _mm256_fmadd_pd(ai, b0, c30);                            \
+    c31 = _mm256_fmadd_pd(ai, b1, c31);                            \
+    ai = _mm256_broadcast_sd(a + (p) * MR + 4);                    \
+    c40 = _mm256_fmadd_pd(ai, b0, c40);                            \
+    c41 = _mm256_fmadd_pd(ai, b1, c41);                            \
+    ai = _mm256_broadcast_sd(a + (p) * MR + 5);                    \
+    c50 = _mm256_fmadd_pd(ai, b0, c50);                            \
+    c51 = _mm256_fmadd_pd(ai, b1, c51);                            \
+  } while (0)
+  int64_t p = 0;
+  for (; p + 4 <= k; p += 4) {
+    _mm_prefetch((const char*)(a + (p + 8) * MR), _MM_HINT_T0);
+    _mm_prefetch((const char*)(b + (p + 8) * NR), _MM_HINT_T0);
+    _mm_prefetch((const char*)(b + (p + 10) * NR), _MM_HINT_T0);
+    MOMMS_STEP(p);
+    MOMMS_STEP(p + 1);
+    MOMMS_STEP(p + 2);
+    MOMMS_STEP(p + 3);
   }
+  for (; p < k; ++p) MOMMS_STEP(p);
+#undef MOMMS_STEP
+  __m256d acc[MR][2] = {{c00, c01}, {c10, c11}, {c20, c21}, {c30, c31}, {c40, c41}, {c50, c51}};
   const __m256d va = _mm256_set1_pd(alpha);
   double t[MR][NR];
   for (int i = 0; i < MR; ++i) {
@@ -73,10 +108,14 @@ void oracle_ukernel_6x8(int64_t k, const double* alpha, const double* a, const d
 }
 
 /* ---- thread team: the reference's ThreadComm barrier (src/thread_comm.rs:50-66),
- * with acquire/release ordering instead of Relaxed. */
+ * a sense-reversing spin barrier, with acquire/release ordering instead of
+ * Relaxed. (A futex-based pthread barrier costs tens of microseconds per
+ * episode at 16 threads, and the tree passes one about 18,000 times at
+ * m = k = 8192, n = 1024.) */
 typedef struct {
   int n;
-  pthread_barrier_t bar;
+  _Atomic int arrived;
+  _Atomic int sense;
 } team_t;
 
 typedef struct {
@@ -92,10 +131,28 @@ typedef struct {
   int64_t rsC, csC;
   double* bpack; /* shared (ceil(NC/NR)+1)*NR*KC */
   double* apack; /* shared (ceil(MC/MR)+1)*MR*KC */
+  int cpu;       /* core to pin this thread to, or -1 */
 } job_t;
 
+/* One thread per core, as SpawnThreads pins its pool with hwloc
+ * (src/composables/spawn.rs:20-35). */
+static void pin_to(int cpu) {
+  if (cpu < 0) return;
+  cpu_set_t set;
+  CPU_ZERO(&set);
+  CPU_SET(cpu, &set);
+  pthread_setaffinity_np(pthread_self(), sizeof(set), &set);
+}
+
 static void barrier(team_t* t) {
-  if (t->n > 1) pthread_barrier_wait(&t->bar);
+  if (t->n <= 1) return;
+  const int my = !atomic_load_explicit(&t->sense, memory_order_relaxed);
+  if (atomic_fetch_add_explicit(&t->arrived, 1, memory_order_acq_rel) == t->n - 1) {
+    atomic_store_explicit(&t->arrived, 0, memory_order_relaxed);
+    atomic_store_explicit(&t->sense, my, memory_order_release);
+  } else {
+    while (atomic_load_explicit(&t->sense, memory_order_acquire) != my) _mm_pause();
+  }
 }
 
 /* Packer<Matrix, ColumnPanelMatrix<8>> (src/composables/pack.rs:51-90), with
@@ -167,6 +224,7 @@ static void kernel_nm(const job_t* j, int64_t m0, int64_t mc, int64_t n0, int64_
 
 static void* goto_thread(void* arg) {
   job_t* j = (job_t*)arg;
+  if (j->tid > 0) pin_to(j->cpu);
   for (int64_t n0 = 0; n0 < j->n; n0 += NC) {          /* PartN<3000> */
     const int64_t nc = j->n - n0 < NC ? j->n - n0 : NC;
     double beta = j->beta;
@@ -202,11 +260,20 @@ static void* goto_thread(void* arg) {
 int oracle_goto_dgemm(int64_t m, int64_t n, int64_t k, const double* alpha, const double* A,
                       int64_t rsA, int64_t csA, const double* B, int64_t rsB, int64_t csB,
                       const double* beta, double* C, int64_t rsC, int64_t csC, int nthreads) {
+  return oracle_goto_dgemm_pinned(m, n, k, alpha, A, rsA, csA, B, rsB, csB, beta, C, rsC, csC,
+                                  nthreads, NULL);
+}
+
+int oracle_goto_dgemm_pinned(int64_t m, int64_t n, int64_t k, const double* alpha,
+                             const double* A, int64_t rsA, int64_t csA, const double* B,
+                             int64_t rsB, int64_t csB, const double* beta, double* C,
+                             int64_t rsC, int64_t csC, int nthreads, const int* cpus) {
   if (m <= 0 || n <= 0) return 0;
   if (nthreads < 1) nthreads = 1;
   team_t team;
   team.n = nthreads;
-  if (nthreads > 1) pthread_barrier_init(&team.bar, NULL, (unsigned)nthreads);
+  atomic_init(&team.arrived, 0);
+  atomic_init(&team.sense, 0);
   const size_t bcap = (size_t)((NC + NR - 1) / NR + 1) * NR * KC;
   const size_t acap = (size_t)((MC + MR - 1) / MR + 1) * MR * KC;
   double* bpack = (double*)aligned_alloc(4096, bcap * sizeof(double));
@@ -228,11 +295,15 @@ int oracle_goto_dgemm(int64_t m, int64_t n, int64_t k, const double* alpha, cons
     j->B = B; j->rsB = rsB; j->csB = csB;
     j->C = C; j->rsC = rsC; j->csC = csC;
     j->bpack = bpack; j->apack = apack;
+    j->cpu = cpus ? cpus[t] : -1;
   }
+  cpu_set_t saved;
+  const int restore = cpus && pthread_getaffinity_np(pthread_self(), sizeof(saved), &saved) == 0;
   for (int t = 1; t < nthreads; ++t) pthread_create(&th[t], NULL, goto_thread, &jobs[t]);
+  if (cpus) pin_to(cpus[0]);
   goto_thread(&jobs[0]);
   for (int t = 1; t < nthreads; ++t) pthread_join(th[t], NULL);
-  if (nthreads > 1) pthread_barrier_destroy(&team.bar);
+  if (restore) pthread_setaffinity_np(pthread_self(), sizeof(saved), &saved);
   free(jobs);
   free(th);
   free(bpack);

@@ -20,6 +20,12 @@
 int oracle_goto_dgemm(int64_t m, int64_t n, int64_t k, const double* alpha, const double* A,
                       int64_t rsA, int64_t csA, const double* B, int64_t rsB, int64_t csB,
                       const double* beta, double* C, int64_t rsC, int64_t csC, int nthreads);
+/* oracle_goto_dgemm with thread t pinned to core cpus[t] (NULL: unpinned);
+ * the caller's own affinity is restored before returning. */
+int oracle_goto_dgemm_pinned(int64_t m, int64_t n, int64_t k, const double* alpha,
+                             const double* A, int64_t rsA, int64_t csA, const double* B,
+                             int64_t rsB, int64_t csB, const double* beta, double* C,
+                             int64_t rsC, int64_t csC, int nthreads, const int* cpus);
 void oracle_goto_scalar(int64_t m, int64_t n, int64_t k, const double* alpha, const double* A,
                         int64_t rsA, int64_t csA, const double* B, int64_t rsB, int64_t csB,
                         const double* beta, double* C, int64_t rsC, int64_t csC);
@@ -61,6 +67,11 @@ void oracle_crt_integers(int64_t count, int64_t k, const double* X, int64_t sv, 
 int oracle_crt_gemm(int64_t m, int64_t n, int64_t k, const double* alpha, const double* A,
                     int64_t rsA, int64_t csA, const double* B, int64_t rsB, int64_t csB,
                     const double* betaC, double* C, int64_t rsC, int64_t csC, int N);
+int oracle_crt_i8_modulus(int l);
+int oracle_crt_i8_beta(int64_t k, int N);
+int oracle_crt_i8_gemm(int64_t m, int64_t n, int64_t k, const double* alpha, const double* A,
+                       int64_t rsA, int64_t csA, const double* B, int64_t rsB, int64_t csB,
+                       const double* betaC, double* C, int64_t rsC, int64_t csC, int N);
 
 /* hierarchical layout: a layout is int64[9 + 4 * ORACLE_HIER_LEVELS] =
    [h, w, h_pad, w_pad, ny, nx, capacity, lrs, lcs, y_blksz[], y_stride[],

@@ -51,16 +51,22 @@ def colmajor(h: int, w: int) -> np.ndarray:
     return np.zeros((w, h), dtype=np.float64).T
 
 
-def goto_dgemm(A, B, C, alpha=1.0, beta=1.0, nthreads=1):
-    """Restated Goto tree, in place on C (any strides)."""
+def goto_dgemm(A, B, C, alpha=1.0, beta=1.0, nthreads=1, cpus=None):
+    """Restated Goto tree, in place on C (any strides); thread t pinned to
+    core cpus[t] when a list of nthreads cores is given."""
     m, k = A.shape
     n = B.shape[1]
     ra, ca = strides(A)
     rb, cb = strides(B)
     rc, cc = strides(C)
-    st = lib().oracle_goto_dgemm(_i64(m), _i64(n), _i64(k), _d(alpha), _p(A), _i64(ra), _i64(ca),
-                                 _p(B), _i64(rb), _i64(cb), _d(beta), _p(C), _i64(rc), _i64(cc),
-                                 ctypes.c_int(nthreads))
+    pins = None
+    if cpus is not None:
+        if len(cpus) < nthreads:
+            raise ValueError("need one core per thread")
+        pins = (ctypes.c_int * nthreads)(*cpus[:nthreads])
+    st = lib().oracle_goto_dgemm_pinned(_i64(m), _i64(n), _i64(k), _d(alpha), _p(A), _i64(ra),
+                                        _i64(ca), _p(B), _i64(rb), _i64(cb), _d(beta), _p(C),
+                                        _i64(rc), _i64(cc), ctypes.c_int(nthreads), pins)
     if st != 0:
         raise MemoryError("oracle_goto_dgemm could not allocate its pack buffers")
     return C
@@ -210,6 +216,29 @@ def crt_gemm(A, B, C, N, alpha=1.0, beta=1.0):
                                ctypes.c_int(N))
     if st != 0:
         raise ValueError(f"oracle_crt_gemm: status {st} (too few moduli for k={k}?)")
+    return C
+
+
+def crt_i8_moduli(N: int) -> list[int]:
+    return [lib().oracle_crt_i8_modulus(ctypes.c_int(l)) for l in range(N)]
+
+
+def crt_i8_beta(k: int, N: int) -> int:
+    return lib().oracle_crt_i8_beta(_i64(k), ctypes.c_int(N))
+
+
+def crt_i8_gemm(A, B, C, N, alpha=1.0, beta=1.0):
+    """The INT8 residue form with N moduli (<= 255), in place on C."""
+    m, k = A.shape
+    n = B.shape[1]
+    ra, ca = strides(A)
+    rb, cb = strides(B)
+    rc, cc = strides(C)
+    st = lib().oracle_crt_i8_gemm(_i64(m), _i64(n), _i64(k), _d(alpha), _p(A), _i64(ra),
+                                  _i64(ca), _p(B), _i64(rb), _i64(cb), _d(beta), _p(C),
+                                  _i64(rc), _i64(cc), ctypes.c_int(N))
+    if st != 0:
+        raise ValueError(f"oracle_crt_i8_gemm: status {st} (too few moduli for k={k}?)")
     return C
 
 

@@ -2,7 +2,9 @@
 // (momms_amd/csrc/crt_recon.hpp, the code the device kernel runs).
 // Reads lines "N r_1 ... r_N" from stdin and prints, per line, the hex of
 // the correctly rounded double of the reconstructed integer and the integer
-// itself in decimal, for comparison with exact Python integers.
+// itself in decimal, for comparison with exact Python integers. The residues
+// are taken modulo the BF16 form's moduli, or the INT8 form's when the first
+// argument is "i8".
 #include <cstdio>
 
 #include "crt_recon.hpp"
@@ -22,9 +24,11 @@ static void print_i128(__int128 v) {
   while (n) putchar(buf[--n]);
 }
 
-int main() {
+int main(int argc, char** argv) {
+  const bool i8 = argc > 1 && argv[1][0] == 'i';
   CrtRecon K[kCrtMaxModuli + 1];
-  for (int N = 1; N <= kCrtMaxModuli; ++N) K[N] = crt_recon_constants(N);
+  for (int N = 1; N <= kCrtMaxModuli; ++N)
+    K[N] = crt_recon_constants(N, i8 ? kCrtI8Moduli : kCrtModuli);
   int N;
   while (scanf("%d", &N) == 1) {
     if (N < 1 || N > kCrtMaxModuli) return 2;

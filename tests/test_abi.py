@@ -177,3 +177,15 @@ def test_wrapper_rejects_broadcast_and_overlapping_views():
         capi._mat(base.as_strided((4, 4), (1, 2)), 4, 4, "A")
     with pytest.raises(ValueError, match="GPU tensor"):
         capi._mat(torch.zeros(1, 5, dtype=torch.float64).expand(1, 5), 1, 5, "C", output=True)
+
+
+def test_crt_i8_rejects_k_beyond_exact_int32_sums():
+    al = D(1.0)
+    lib = capi.lib()
+    assert lib.momms_hip_crt_i8_workspace_size(8, 8, 131073, 16) == 0
+    assert lib.momms_hip_crt_i8_workspace_size(8, 8, 131072, 16) > 0
+    st = lib.momms_hip_dgemm_crt_i8(8, 8, 131073, ctypes.byref(al), 8, 1, 8, 8, 1, 131073,
+                                    ctypes.byref(al), 8, 1, 8, 16, 8, 1 << 30, None)
+    assert st == -3 and "131072" in capi.last_error()
+    assert lib.momms_hip_crt_i8_beta(8192, 15) == 51
+    assert [lib.momms_hip_crt_i8_modulus(i) for i in (0, 15, 16)] == [255, 191, 0]

@@ -1,6 +1,7 @@
-"""CPU check of the residue split GEMM's Chinese remainder step: the host build
-of momms_amd/csrc/crt_recon.hpp (the code the reconstruction kernel runs)
-recovers every integer it is given from its balanced residues, at the ends of
+"""CPU check of the residue split GEMMs' Chinese remainder step: the host build
+of momms_amd/csrc/crt_recon.hpp (the code the reconstruction kernels run),
+for the BF16 form's moduli (<= 511) and the INT8 form's (<= 255), recovers
+every integer it is given from its balanced residues, at the ends of
 the balanced range (-P/2, P/2) (where the quotient estimate can be one off
 and the fix applies), at the 2^125 bound of the capped moduli counts, and at
 random points; and its double is the correctly rounded one (Python's
@@ -22,12 +23,13 @@ def balanced(x, p):
     return r - p if r > (p - 1) // 2 else r
 
 
-@pytest.fixture(scope="module")
-def cases():
+@pytest.fixture(scope="module", params=["bf16", "i8"])
+def cases(request):
     if not PROG.exists():
         pytest.fail(f"{PROG} is not built (run the build)")
     gen = random.Random(20261203)
-    mods = O.crt_moduli(O.CRT_MAX_MODULI)
+    table = request.param
+    mods = (O.crt_moduli if table == "bf16" else O.crt_i8_moduli)(O.CRT_MAX_MODULI)
     rows = []
     for N in range(1, O.CRT_MAX_MODULI + 1):
         P = math.prod(mods[:N])
@@ -43,13 +45,13 @@ def cases():
                 vals += [2 ** e + 2 ** (e - 53), 2 ** e + 3 * 2 ** (e - 53), -(2 ** e + 2 ** (e - 53)),
                          2 ** e + 2 ** (e - 53) + 1]
         rows += [(N, c) for c in vals]
-    return mods, rows
+    return table, mods, rows
 
 
 def test_reconstruction_recovers_every_integer(cases):
-    mods, rows = cases
+    table, mods, rows = cases
     text = "\n".join(f"{N} " + " ".join(str(balanced(c, p)) for p in mods[:N]) for N, c in rows)
-    out = subprocess.run([str(PROG)], input=text + "\n", capture_output=True, text=True,
+    out = subprocess.run([str(PROG), table], input=text + "\n", capture_output=True, text=True,
                          check=True).stdout.split("\n")
     assert len([l for l in out if l]) == len(rows)
     for (N, c), line in zip(rows, out):

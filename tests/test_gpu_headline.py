@@ -69,3 +69,13 @@ def test_residues_against_plain(inputs, moduli, passes):
     err = capi.sampled_error(A, B, C0, C, 1.0, 1.0, samples=SAMPLES, seed=SEED)
     del C
     assert within_rule(err, plain) == passes, (moduli, err, plain)
+
+
+@pytest.mark.parametrize("moduli,passes", [(15, True), (14, False)])
+def test_int8_residues_against_plain(inputs, moduli, passes):
+    A, B, C0, plain = inputs
+    C = C0.clone()
+    capi.dgemm_crt_i8(A, B, C, moduli, 1.0, 1.0)
+    err = capi.sampled_error(A, B, C0, C, 1.0, 1.0, samples=SAMPLES, seed=SEED)
+    del C
+    assert within_rule(err, plain) == passes, (moduli, err, plain)

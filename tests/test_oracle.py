@@ -358,3 +358,70 @@ def test_crt_planted_mistake_is_caught(golden):
     O.crt_gemm(A, B, C, N - 2, float.fromhex(case["alpha"]), float.fromhex(case["beta"]))
     out = unhex(case["out"], m, n)
     assert not np.array_equal(C.view(np.int64), out.view(np.int64))
+
+
+# --- the INT8 residue form -----------------------------------------------------
+
+def test_crt_i8_moduli_are_the_largest_coprime_odd_set_below_256():
+    mods = O.crt_i8_moduli(O.CRT_MAX_MODULI)
+    want, p = [], 255
+    while len(want) < O.CRT_MAX_MODULI:       # greedy from 255 down
+        if p % 2 and all(math.gcd(p, q) == 1 for q in want):
+            want.append(p)
+        p -= 1
+    assert mods == want
+    # balanced residues |r| <= 127 are int8 values, and a k-term sum of their
+    # products stays an exact INT32 for every k the form accepts
+    assert max(mods) <= 255 and 131072 * 127 ** 2 < 2 ** 31
+    assert O.crt_i8_beta(131073, 16) == -1 and O.crt_i8_beta(131072, 16) > 0
+
+
+@pytest.mark.parametrize("k", [0, 1, 2, 3, 300, 4096, 4097, 8192, 1 << 17])
+def test_crt_i8_beta_fits_the_product_of_moduli(k):
+    for N in range(1, O.CRT_MAX_MODULI + 1):
+        b = O.crt_i8_beta(k, N)
+        P = math.prod(O.crt_i8_moduli(N))
+        kb = (k - 1).bit_length() if k > 1 else 0
+        if b < 1:
+            assert 2 * 2 ** kb * 2 ** 2 > 2 ** (P.bit_length() - 2)
+            continue
+        assert 2 * max(k, 1) * 2 ** (2 * b) < P
+        assert max(k, 1) * 2 ** (2 * b) <= 2 ** 125
+        if b < 53:
+            assert 2 * 2 ** kb * 2 ** (2 * (b + 1)) >= 2 ** (P.bit_length() - 1) or \
+                2 ** kb * 2 ** (2 * (b + 1)) > 2 ** 125
+    # 15 INT8 moduli reach the integer width of 13 BF16 moduli at k = 8192
+    assert O.crt_i8_beta(8192, 15) == O.crt_beta(8192, 13) == 51
+
+
+def test_crt_i8_golden_through_equal_beta(golden):
+    """Either residue form's result is fixed by beta alone: for every golden
+    case (exact Python integers, BF16 moduli count N) and every INT8 moduli
+    count with the same beta, the INT8 restatement gives the golden bits."""
+    hits = 0
+    for case in golden["crt"]:
+        m, n, k, N = case["m"], case["n"], case["k"], case["N"]
+        b = O.crt_beta(k, N)
+        for N8 in range(1, O.CRT_MAX_MODULI + 1):
+            if O.crt_i8_beta(k, N8) != b:
+                continue
+            A, B, C = unhex(case["A"], m, k), unhex(case["B"], k, n), unhex(case["C"], m, n)
+            O.crt_i8_gemm(A, B, C, N8, float.fromhex(case["alpha"]), float.fromhex(case["beta"]))
+            out = unhex(case["out"], m, n)
+            assert np.array_equal(C.view(np.int64), out.view(np.int64)), (m, n, k, N, N8)
+            hits += 1
+    assert hits >= 3
+
+
+@pytest.mark.parametrize("N,bound", [(10, 2.0 ** -18), (13, 2.0 ** -30), (16, 2.0 ** -51)])
+def test_crt_i8_error_against_double_double(N, bound):
+    rng = np.random.default_rng(SEED + 100 + N)
+    m, n, k = 24, 20, 300
+    A, B, C = (uniform_colmajor(rng, *sh) for sh in ((m, k), (k, n), (m, n)))
+    ref, mag = O.dd_gemm(A, B, C)
+    c = C.copy(order="F")
+    O.crt_i8_gemm(A, B, c, N)
+    err = np.abs(c - ref) / mag
+    assert err.max() <= bound
+    if N < 16:
+        assert err.max() > bound / 2 ** 24   # the truncation at beta bits is visible
