@@ -224,6 +224,26 @@ def sweep_plain(capi, torch, gen, steps):
     return out
 
 
+def variant_trees(capi, torch, gen, steps, sizes=(4096, 8192)):
+    """The reference's variant control trees (Goto, resident A / B / C, L4C)
+    as tile orders of the FP64 GEMM (momms_hip_dgemm_tree), beside the
+    built-in grouped raster of the same 128 x 128 tile (configuration 0):
+    same bits, different L2 sharing."""
+    out = []
+    for n in sizes:
+        A, B, C = rand_colmajor(n, n, gen), rand_colmajor(n, n, gen), rand_colmajor(n, n, gen)
+        row = {"m": n, "n": n, "k": n}
+        ms = time_loop(lambda: capi.dgemm(A, B, C, 1.0, 1.0, tile_cfg=0), steps, 3)
+        row["grouped_raster"] = round(2.0 * n ** 3 / ms / 1e9, 3)
+        for name, tree in capi.REFERENCE_TREES.items():
+            ms = time_loop(lambda: capi.dgemm_tree(A, B, C, tree, 1.0, 1.0), steps, 3)
+            row[name] = round(2.0 * n ** 3 / ms / 1e9, 3)
+        row["unit"] = "TFLOP/s"
+        out.append(row)
+        del A, B, C
+    return out
+
+
 def split_grid(capi, torch, gen, steps, sizes=(4096, 8192), spreads=(0, 8, 16), slices=range(3, 8)):
     out = []
     for n in sizes:
@@ -659,6 +679,8 @@ def main() -> int:
         cfg = {}
         log("bench: config 2 sweep (plain FP64)")
         cfg["plain_sweep"] = sweep_plain(capi, torch, gen, 20)
+        log("bench: variant control trees as tile orders")
+        cfg["variant_trees"] = variant_trees(capi, torch, gen, 20)
         log("bench: config 3 grid (split-BF16: size x spread x slices)")
         cfg["split_grid"] = split_grid(capi, torch, gen, 10)
         log("bench: config 3 grid (split-BF16 residues: size x spread x moduli)")

@@ -295,6 +295,30 @@ typedef struct momms_hier_layout {
   int64_t x_blksz[MOMMS_HIER_MAX_LEVELS], x_stride[MOMMS_HIER_MAX_LEVELS];
 } momms_hier_layout;
 
+/* momms_hip_dgemm with the output tiles taken in the order of a control
+ * tree's loops instead of the built-in raster: the device form of the
+ * reference's variant control trees (Goto, src/exper/l3_packing.rs:58-66;
+ * resident A / B / C, src/exper/l3_shapes.rs:69-100; L4C,
+ * src/exper/l4.rs:71-78), which differ in the loop order and block sizes
+ * that decide which operand stays in which cache. steps is the tree's
+ * hierarchy description (GemmNode::hierarchy_description,
+ * src/composables/mod.rs:24-36: innermost step first, block sizes in
+ * elements, as for momms_hier_layout_init). Its M and N steps, outermost
+ * first and rounded down to whole 128 x 128 tiles, split the tile grid into
+ * nested blocks taken in order (what the last step leaves is walked with m
+ * fastest), consecutive tiles going to one XCD, whose L2 then holds the
+ * panels they share; K steps leave the order alone (each tile keeps its
+ * whole k loop). Every tile is the 128 x 128 buffer->LDS tile of
+ * momms_hip_dgemm_tiled's configuration 0, so the result is bit-identical
+ * to it whatever the tree; operands that tile cannot take run configuration
+ * 0's register-staged form in its own order. Returns -15 for a step kind
+ * other than M, N, K, a block size below 1 or more than 8 M / N steps, -16
+ * for a bad nsteps. */
+int momms_hip_dgemm_tree(int64_t m, int64_t n, int64_t k, const double* alpha, const double* A,
+                         int64_t rsA, int64_t csA, const double* B, int64_t rsB, int64_t csB,
+                         const double* beta, double* C, int64_t rsC, int64_t csC,
+                         const momms_algo_step* steps, int64_t nsteps, void* stream);
+
 /* Hierarch::new (hierarch.rs:90-140): the layout of an h x w matrix whose
  * rows follow steps of kind ydim and columns steps of kind xdim; lh, lw give
  * the top-level block when no step of that kind exists. Host only. Returns

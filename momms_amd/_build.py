@@ -2,7 +2,7 @@
 
 Produces, in-tree:
   momms_amd/libmomms_hip.so   the HIP kernel library behind include/momms_hip.h
-  momms_amd/host/bin/*        C++ programs of the GemmNode host layer
+  tests/native/bin/*          test programs (GemmNode host layer, native checks)
   oracle/liboracle.so         the CPU oracle (test infrastructure; built by
                               its own Makefile)
 
@@ -22,7 +22,6 @@ ROOT = PKG.parent
 CSRC = PKG / "csrc"
 LIB = PKG / "libmomms_hip.so"
 HOST = PKG / "host"
-HOST_BIN = HOST / "bin"
 ORACLE = ROOT / "oracle"
 
 ROCM = os.environ.get("ROCM_PATH", "/opt/rocm")
@@ -82,23 +81,36 @@ def build_hip_library(jobs: int = 8) -> Path:
     return LIB
 
 
+def _host_flags() -> list[str]:
+    return ["-O2", "-std=c++17", "-Wall", "-pthread", "-mavx2", "-mfma",
+            f"-I{HOST / 'include'}", f"-I{ROOT / 'include'}",
+            "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include"]
+
+
+def _host_libs() -> list[str]:
+    return [f"-L{PKG}", "-lmomms_hip", f"-Wl,-rpath,{PKG}",
+            f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib"]
+
+
 def build_host_programs() -> list[Path]:
-    """C++ programs of the GemmNode host layer (header-only, links the ABI)."""
+    """Test programs of the GemmNode host layer (header-only, links the ABI):
+    tests/native/host/*.cpp -> tests/native/bin/, plus a ThreadSanitizer
+    build of the CPU self-test (host code only, compiled by g++)."""
     progs = []
-    src_dir = HOST / "examples"
+    src_dir = ROOT / "tests" / "native" / "host"
     if not src_dir.is_dir():
         return progs
-    HOST_BIN.mkdir(exist_ok=True)
+    out_dir = ROOT / "tests" / "native" / "bin"
+    out_dir.mkdir(exist_ok=True)
     for src in sorted(src_dir.glob("*.cpp")):
-        out = HOST_BIN / src.stem
-        _run([
-            "g++", "-O2", "-std=c++17", "-Wall", "-pthread", "-mavx2", "-mfma",
-            f"-I{HOST / 'include'}", f"-I{ROOT / 'include'}",
-            "-D__HIP_PLATFORM_AMD__", f"-I{ROCM}/include",
-            str(src), "-o", str(out),
-            f"-L{PKG}", "-lmomms_hip", f"-Wl,-rpath,{PKG}", "-Wl,-rpath,$ORIGIN/../..",
-            f"-L{ROCM}/lib", "-lamdhip64", f"-Wl,-rpath,{ROCM}/lib",
-        ])
+        out = out_dir / src.stem
+        _run(["g++", *_host_flags(), str(src), "-o", str(out), *_host_libs()])
+        progs.append(out)
+    selftest = src_dir / "host_selftest.cpp"
+    if selftest.exists():
+        out = out_dir / "host_selftest_tsan"
+        _run(["g++", *_host_flags(), "-g", "-fsanitize=thread", str(selftest), "-o", str(out),
+              *_host_libs()])
         progs.append(out)
     return progs
 

@@ -43,6 +43,8 @@ _SIGS: dict[str, list] = {
                         _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_vp],
     "momms_hip_dgemm_tiled": [_c_i64, _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_vp,
                               _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_int, _c_vp],
+    "momms_hip_dgemm_tree": [_c_i64, _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_vp,
+                             _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_vp, _c_i64, _c_vp],
     "momms_hip_dgemm_noalloc": [_c_i64, _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_vp,
                                 _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_vp],
     "momms_last_error_string": [],
@@ -197,6 +199,40 @@ def dgemm(A, B, C, alpha: float = 1.0, beta: float = 1.0, stream=None, tile_cfg:
         st = lib().momms_hip_dgemm_tiled(m, n, k, ctypes.byref(al), a, rsa, csa, b, rsb, csb,
                                          ctypes.byref(be), c, rsc, csc, int(tile_cfg), s)
         check("momms_hip_dgemm_tiled", st)
+    return C
+
+
+# Hierarchy descriptions (innermost step first, (kind, block size)) of the
+# reference's control trees with MR = 4, NR = 12: Goto (src/exper/l3_shapes.rs:
+# 58-64), resident A, B, C (l3_shapes.rs:69-100) and L4C (src/exper/l4.rs:
+# 71-78); kinds 0 = M, 1 = N, 2 = K.
+REFERENCE_TREES = {
+    "goto": [(0, 4), (1, 12), (0, 120), (2, 192), (1, 3000)],
+    "l3a": [(1, 12), (0, 4), (2, 192), (1, 120), (2, 768), (0, 768)],
+    "l3b": [(0, 4), (1, 12), (2, 192), (0, 120), (2, 768), (1, 768)],
+    "l3c": [(0, 4), (1, 12), (0, 156), (2, 156), (0, 624), (1, 624)],
+    "l4c": [(0, 4), (1, 12), (0, 120), (2, 192), (1, 3600), (0, 3600)],
+}
+
+
+def dgemm_tree(A, B, C, steps, alpha: float = 1.0, beta: float = 1.0, stream=None):
+    """C := beta*C + alpha*A*B with the tiles in the order of a control tree
+    whose hierarchy description (innermost first) is `steps`, a list of
+    (STEP_M | STEP_N | STEP_K, block size) pairs."""
+    _f64((A, "A"), (B, "B"), (C, "C"))
+    m, k = A.shape
+    n = B.shape[1]
+    a, rsa, csa = _mat(A, m, k, "A")
+    b, rsb, csb = _mat(B, k, n, "B")
+    c, rsc, csc = _mat(C, m, n, "C", output=True)
+    arr = (AlgoStep * max(1, len(steps)))()
+    for i, (kind, bsz) in enumerate(steps):
+        arr[i].kind, arr[i].bsz = int(kind), int(bsz)
+    al, be = ctypes.c_double(alpha), ctypes.c_double(beta)
+    st = lib().momms_hip_dgemm_tree(m, n, k, ctypes.byref(al), a, rsa, csa, b, rsb, csb,
+                                    ctypes.byref(be), c, rsc, csc, ctypes.addressof(arr),
+                                    len(steps), stream_handle(stream))
+    check("momms_hip_dgemm_tree", st)
     return C
 
 

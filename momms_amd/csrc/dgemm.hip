@@ -23,6 +23,8 @@
 #include "mfma.hpp"
 #include "scratch.hpp"
 #include "status.hpp"
+#include "tile_order.hpp"
+#include "momms_hip.h"
 
 #include <mutex>
 
@@ -56,6 +58,25 @@ dgemm_dma_kernel(int64_t m, int64_t n, int64_t k, double alpha, const double* __
   const int64_t t = xcd_remap(blockIdx.x, (int64_t)gridDim.x);
   int64_t tm, tn;
   grouped_tile(t, tiles_m, tiles_n, group_m, tm, tn);
+  dgemm_tile_dma<STAGES>(sm, m, n, k, alpha, make_rsrc(A, a_bytes), lda, make_rsrc(B, b_bytes),
+                         ldb, beta, C, ldc, tm * 128, tn * 128);
+}
+
+// The data-parallel kernel with the tiles taken in the order of a control
+// tree's M / N loops (tile_order.hpp) instead of the grouped raster: same
+// tiles, same k loop, so the same bits as dgemm_dma_kernel.
+template <int STAGES, int MINB>
+__global__ void __launch_bounds__(256, MINB)
+dgemm_dma_tree_kernel(int64_t m, int64_t n, int64_t k, double alpha, const double* __restrict__ A,
+                      int64_t lda, uint32_t a_bytes, const double* __restrict__ B, int64_t ldb,
+                      uint32_t b_bytes, double beta, double* __restrict__ C, int64_t ldc,
+                      TileOrder order) {
+  __shared__ DgemmDmaSmem<STAGES> sm;
+  const int64_t tiles_m = ceil_div(m, 128);
+  const int64_t tiles_n = ceil_div(n, 128);
+  const int64_t t = xcd_remap(blockIdx.x, (int64_t)gridDim.x);  // consecutive tiles on one XCD
+  int64_t tm, tn;
+  tile_order_decode(t, tiles_m, tiles_n, order, tm, tn);
   dgemm_tile_dma<STAGES>(sm, m, n, k, alpha, make_rsrc(A, a_bytes), lda, make_rsrc(B, b_bytes),
                          ldb, beta, C, ldc, tm * 128, tn * 128);
 }
@@ -304,6 +325,46 @@ extern "C" int momms_hip_dgemm(int64_t m, int64_t n, int64_t k, const double* al
   DgemmArgs g{m, n, k, *alpha, A, rsA, csA, B, rsB, csB, *beta, C, rsC, csC,
               (hipStream_t)stream};
   return hip_status(fn, dgemm_launch(g, -1));
+}
+
+extern "C" int momms_hip_dgemm_tree(int64_t m, int64_t n, int64_t k, const double* alpha,
+                                    const double* A, int64_t rsA, int64_t csA, const double* B,
+                                    int64_t rsB, int64_t csB, const double* beta, double* C,
+                                    int64_t rsC, int64_t csC, const momms_algo_step* steps,
+                                    int64_t nsteps, void* stream) {
+  using namespace momms;
+  static const char* fn = "momms_hip_dgemm_tree";
+  if (m < 0) return bad_arg(fn, 1, "m < 0");
+  if (n < 0) return bad_arg(fn, 2, "n < 0");
+  if (k < 0) return bad_arg(fn, 3, "k < 0");
+  if (alpha == nullptr) return bad_arg(fn, 4, "alpha is null");
+  int st;
+  if ((st = check_operand(fn, 5, A, m, k, rsA, csA))) return st;
+  if ((st = check_operand(fn, 8, B, k, n, rsB, csB))) return st;
+  if (beta == nullptr) return bad_arg(fn, 11, "beta is null");
+  if ((st = check_operand(fn, 12, C, m, n, rsC, csC))) return st;
+  if (nsteps < 0 || nsteps > 64 || (nsteps > 0 && steps == nullptr))
+    return bad_arg(fn, 16, "nsteps");
+  int32_t kinds[64];
+  int64_t bszs[64];
+  for (int64_t i = 0; i < nsteps; ++i) {
+    kinds[i] = steps[i].kind;
+    bszs[i] = steps[i].bsz;
+  }
+  TileOrder order;
+  if (!tile_order_from_steps(kinds, bszs, nsteps, 128, 128, order))
+    return bad_arg(fn, 15, "steps: a kind other than M, N, K, a block size below 1, or more "
+                           "than 8 M / N steps");
+  if (m == 0 || n == 0) return 0;
+  DgemmArgs g{m, n, k, *alpha, A, rsA, csA, B, rsB, csB, *beta, C, rsC, csC,
+              (hipStream_t)stream};
+  uint32_t a_bytes = 0, b_bytes = 0;
+  if (!dma_ok(g, a_bytes, b_bytes)) return hip_status(fn, dgemm_launch(g, 0));
+  const int64_t tiles = ceil_div(m, 128) * ceil_div(n, 128);
+  hipLaunchKernelGGL((dgemm_dma_tree_kernel<2, 2>), dim3((unsigned)tiles), dim3(256), 0, g.stream,
+                     g.m, g.n, g.k, g.alpha, g.A, g.csA, a_bytes, g.B, g.csB, b_bytes, g.beta,
+                     g.C, g.csC, order);
+  return hip_status(fn, hipGetLastError());
 }
 
 extern "C" int momms_hip_dgemm_tiled(int64_t m, int64_t n, int64_t k, const double* alpha,

@@ -1,0 +1,229 @@
+// Control trees on the GPU: the reference's variant trees as tile schedules,
+// and its packing nodes over device-resident hierarchical buffers.
+//
+//   Ukernel<Mr, Nr>          the hierarchy-description shape of KernelNM<Nr, Mr>
+//                            (src/kern/kernel_nm.rs:112-117): [M Mr, N Nr]
+//   GpuSchedule<Desc>        leaf: the whole product on FP64 matrix cores, its
+//                            tiles taken in the order of the tree Desc (the
+//                            device form of the loops Desc spells out,
+//                            momms_hip_dgemm_tree); Desc is only described,
+//                            never run
+//   HierMatrix               a device-resident Hierarch<f64> buffer
+//                            (src/matrix/hierarch.rs:18-184)
+//   PackA/PackB<LH,LW,LRS,LCS, Child>   pack the current A (B) view into a
+//                            HierMatrix laid out by Child's hierarchy
+//                            description (src/composables/pack.rs:311-419)
+//   UnpackC<LH,LW,LRS,LCS, Child>  Child computes alpha*A*B into a zeroed
+//                            HierMatrix, which is then added to C
+//                            (src/composables/unpack.rs:214-270)
+//   HipGemmHier<Desc>        leaf on three HierMatrix operands
+//                            (momms_hip_dgemm_hier)
+// As with HipGemm, under a team of threads only thread 0 launches and the
+// team meets at a barrier; host-resident views are staged to HBM.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <vector>
+
+#include "momms/composables.hpp"
+#include "momms/hip_gemm.hpp"
+#include "momms/matrix.hpp"
+#include "momms_hip.h"
+
+namespace momms {
+
+// KernelNM<Nr, Mr>'s hierarchy description; never run.
+template <size_t Mr, size_t Nr>
+struct Ukernel {
+  static std::vector<AlgorithmStep> hierarchy_description() {
+    return {{AlgorithmStep::M, Mr}, {AlgorithmStep::N, Nr}};
+  }
+};
+// KernelMN<Nr, Mr>'s (src/kern/kernel_mn.rs:91-96): [N Nr, M Mr]; never run.
+template <size_t Nr, size_t Mr>
+struct UkernelMN {
+  static std::vector<AlgorithmStep> hierarchy_description() {
+    return {{AlgorithmStep::N, Nr}, {AlgorithmStep::M, Mr}};
+  }
+};
+
+inline std::vector<momms_algo_step> abi_steps(const std::vector<AlgorithmStep>& d) {
+  std::vector<momms_algo_step> s;
+  for (const auto& x : d) {
+    momms_algo_step t{};
+    t.kind = x.kind == AlgorithmStep::M ? MOMMS_STEP_M
+             : x.kind == AlgorithmStep::N ? MOMMS_STEP_N
+                                          : MOMMS_STEP_K;
+    t.bsz = (int64_t)x.bsz;
+    s.push_back(t);
+  }
+  return s;
+}
+
+template <class Desc>
+struct GpuSchedule {
+  hipStream_t stream = nullptr;
+  detail::DevBuf da, db, dc;
+
+  template <class At, class Bt, class Ct>
+  void run(At& a, Bt& b, Ct& c, const ThreadInfo& thr) {
+    if (thr.thread_id() == 0) launch(a, b, c);
+    if (thr.num_threads() > 1) thr.barrier();
+  }
+  template <class At, class Bt, class Ct>
+  void launch(At& a, Bt& b, Ct& c) {
+    const size_t m = c.height(), n = c.width(), k = a.width();
+    if (m == 0 || n == 0) return;
+    const double alpha = a.get_scalar() * b.get_scalar();
+    const double beta = c.get_scalar();
+    auto A = detail::stage_in<double>(a, m, k, da, true, stream);
+    auto B = detail::stage_in<double>(b, k, n, db, true, stream);
+    auto C = detail::stage_in<double>(c, m, n, dc, beta != 0.0, stream);
+    const auto steps = abi_steps(Desc::hierarchy_description());
+    abi_check("momms_hip_dgemm_tree",
+              momms_hip_dgemm_tree((int64_t)m, (int64_t)n, (int64_t)k, &alpha, A.ptr, A.rs, A.cs,
+                                   B.ptr, B.rs, B.cs, &beta, C.ptr, C.rs, C.cs, steps.data(),
+                                   (int64_t)steps.size(), stream));
+    detail::stage_out<double>(c, m, n, C.ptr, stream);
+    hip_check("hipStreamSynchronize", hipStreamSynchronize(stream));
+  }
+  static std::vector<AlgorithmStep> hierarchy_description() {
+    return Desc::hierarchy_description();
+  }
+};
+
+// A device-resident hierarchical buffer (Hierarch<f64, LH, LW, LRS, LCS>):
+// its layout, the device memory (grown, never shrunk: a ResizableBuffer,
+// src/matrix/hierarch.rs:318-356) and the scalar.
+class HierMatrix {
+ public:
+  momms_hier_layout layout{};
+  double* ptr() const { return static_cast<double*>(buf_.p); }
+  size_t height() const { return (size_t)layout.h; }
+  size_t width() const { return (size_t)layout.w; }
+  double get_scalar() const { return scalar_; }
+  void set_scalar(double s) { scalar_ = s; }
+  // Hierarch::resize_to: lays out an h x w matrix along the steps.
+  void resize_to(size_t h, size_t w, const std::vector<AlgorithmStep>& desc, int32_t ydim,
+                 int32_t xdim, int64_t lh, int64_t lw, int64_t lrs, int64_t lcs) {
+    const auto steps = abi_steps(desc);
+    abi_check("momms_hier_layout_init",
+              momms_hier_layout_init(&layout, (int64_t)h, (int64_t)w, steps.data(),
+                                     (int64_t)steps.size(), ydim, xdim, lh, lw, lrs, lcs));
+    buf_.get(std::max<size_t>((size_t)layout.capacity * sizeof(double), 256));
+  }
+
+ private:
+  detail::DevBuf buf_;
+  double scalar_ = 1.0;
+};
+
+namespace detail {
+inline int32_t step_kind(AlgorithmStep::Kind k) {
+  return k == AlgorithmStep::M ? MOMMS_STEP_M : k == AlgorithmStep::N ? MOMMS_STEP_N : MOMMS_STEP_K;
+}
+// Packs an h x w view (host or device) into P laid out by P.layout.
+template <class M>
+void pack_view(const M& x, size_t h, size_t w, HierMatrix& P, DevBuf& stage, hipStream_t st) {
+  auto X = stage_in<double>(x, h, w, stage, true, st);
+  abi_check("momms_hip_hier_pack",
+            momms_hip_hier_pack(&P.layout, X.ptr, X.rs, X.cs, P.ptr(), st));
+}
+}  // namespace detail
+
+template <size_t LH, size_t LW, size_t LRS, size_t LCS, class Child>
+struct PackA {
+  Child child;
+  HierMatrix a_pack;
+  detail::DevBuf stage;
+  hipStream_t stream = nullptr;
+  template <class At, class Bt, class Ct>
+  void run(At& a, Bt& b, Ct& c, const ThreadInfo& thr) {
+    if (thr.thread_id() == 0) {
+      a_pack.resize_to(a.height(), a.width(), Child::hierarchy_description(), MOMMS_STEP_M,
+                       MOMMS_STEP_K, LH, LW, LRS, LCS);
+      detail::pack_view(a, a.height(), a.width(), a_pack, stage, stream);
+      a_pack.set_scalar(a.get_scalar());
+    }
+    if (thr.num_threads() > 1) thr.barrier();
+    child.run(a_pack, b, c, thr);
+  }
+  static std::vector<AlgorithmStep> hierarchy_description() { return Child::hierarchy_description(); }
+};
+
+template <size_t LH, size_t LW, size_t LRS, size_t LCS, class Child>
+struct PackB {
+  Child child;
+  HierMatrix b_pack;
+  detail::DevBuf stage;
+  hipStream_t stream = nullptr;
+  template <class At, class Bt, class Ct>
+  void run(At& a, Bt& b, Ct& c, const ThreadInfo& thr) {
+    if (thr.thread_id() == 0) {
+      b_pack.resize_to(b.height(), b.width(), Child::hierarchy_description(), MOMMS_STEP_K,
+                       MOMMS_STEP_N, LH, LW, LRS, LCS);
+      detail::pack_view(b, b.height(), b.width(), b_pack, stage, stream);
+      b_pack.set_scalar(b.get_scalar());
+    }
+    if (thr.num_threads() > 1) thr.barrier();
+    child.run(a, b_pack, c, thr);
+  }
+  static std::vector<AlgorithmStep> hierarchy_description() { return Child::hierarchy_description(); }
+};
+
+// C := C + alpha*A*B: Child writes alpha*A*B into a hierarchical buffer with
+// scalar 0 (not read), whose elements are then added to C's (Unpacker::add,
+// unpack.rs:61-98 and :174-213; C's own scalar is not applied, as there).
+template <size_t LH, size_t LW, size_t LRS, size_t LCS, class Child>
+struct UnpackC {
+  Child child;
+  HierMatrix c_pack;
+  detail::DevBuf stage;
+  hipStream_t stream = nullptr;
+  template <class At, class Bt, class Ct>
+  void run(At& a, Bt& b, Ct& c, const ThreadInfo& thr) {
+    if (thr.thread_id() == 0) {
+      c_pack.resize_to(c.height(), c.width(), Child::hierarchy_description(), MOMMS_STEP_M,
+                       MOMMS_STEP_N, LH, LW, LRS, LCS);
+      c_pack.set_scalar(0.0);
+    }
+    if (thr.num_threads() > 1) thr.barrier();
+    child.run(a, b, c_pack, thr);
+    if (thr.thread_id() == 0) {
+      const size_t m = c.height(), n = c.width();
+      if (m && n) {
+        auto C = detail::stage_in<double>(c, m, n, stage, true, stream);
+        abi_check("momms_hip_hier_unpack_add",
+                  momms_hip_hier_unpack_add(&c_pack.layout, c_pack.ptr(), C.ptr, C.rs, C.cs,
+                                            stream));
+        detail::stage_out<double>(c, m, n, C.ptr, stream);
+      }
+      hip_check("hipStreamSynchronize", hipStreamSynchronize(stream));
+    }
+    if (thr.num_threads() > 1) thr.barrier();
+  }
+  static std::vector<AlgorithmStep> hierarchy_description() { return Child::hierarchy_description(); }
+};
+
+// GemmNode<f64, Hierarch, Hierarch, Hierarch> on the GPU (the l3
+// experiment's trees on hierarchical operands, src/exper/l3.rs:46-72).
+template <class Desc>
+struct HipGemmHier {
+  hipStream_t stream = nullptr;
+  void run(HierMatrix& a, HierMatrix& b, HierMatrix& c, const ThreadInfo& thr) {
+    if (thr.thread_id() == 0) {
+      const double alpha = a.get_scalar() * b.get_scalar();
+      const double beta = c.get_scalar();
+      abi_check("momms_hip_dgemm_hier",
+                momms_hip_dgemm_hier(&a.layout, a.ptr(), &b.layout, b.ptr(), &alpha, &beta,
+                                     &c.layout, c.ptr(), stream));
+      hip_check("hipStreamSynchronize", hipStreamSynchronize(stream));
+    }
+    if (thr.num_threads() > 1) thr.barrier();
+  }
+  static std::vector<AlgorithmStep> hierarchy_description() {
+    return Desc::hierarchy_description();
+  }
+};
+
+}  // namespace momms

@@ -2,6 +2,7 @@
 #pragma once
 #include "momms/composables.hpp"
 #include "momms/hip_gemm.hpp"
+#include "momms/hip_tree.hpp"
 #include "momms/matrix.hpp"
 #include "momms/thread_comm.hpp"
 #include "momms/util.hpp"

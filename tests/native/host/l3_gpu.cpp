@@ -10,6 +10,10 @@
 //   HipSplitGemm<7>                          FP64-accurate from BF16 slices
 //   HipCrtGemm<13>                           FP64-accurate from BF16 residues
 //   HipGemm<float>                           FP32 (the KNM sgemm counterpart)
+//   GpuSchedule<Goto | L3A | L3B | L3C | L4C> the variant trees' tile orders
+//   SpawnThreads<PartM<PartN<Barrier<GpuSchedule>>>>  out of core: host blocks
+//                                            streamed through the GPU
+//   UnpackC<PackB<PackA<HipGemmHier>>>       device Hierarch operands
 // Tolerance: |C - C_ref| <= 2*gamma_{k+1}*(|A||B| + |C0|) componentwise, with
 // the unit roundoff of T (TripleLoop in T is the reference), plus the
 // randomized residual of src/util.rs:55-85.
@@ -21,6 +25,29 @@
 using namespace momms;
 
 static int fails = 0;
+
+template <class Algo>
+static auto set_threads(Algo& a, size_t n) -> decltype(a.set_n_threads(n), void()) {
+  a.set_n_threads(n);
+}
+static void set_threads(...) {}
+
+// The reference's variant trees (block sizes of src/exper/l3_shapes.rs:50-100,
+// src/exper/l4.rs:44-78 and src/exper/out_of_core.rs:52-87, MR = 4, NR = 12),
+// as descriptions for GpuSchedule: their M / N loops order the device tiles.
+using GotoDesc = PartN<3000, PartK<192, PartM<120, Ukernel<4, 12>>>>;
+using L3ADesc = PartM<768, PartK<768, PartN<120, PartK<192, UkernelMN<12, 4>>>>>;
+using L3BDesc = PartN<768, PartK<768, PartM<120, PartK<192, Ukernel<4, 12>>>>>;
+using L3CDesc = PartN<624, PartM<624, PartK<156, PartM<156, Ukernel<4, 12>>>>>;
+using L4CDesc = PartM<3600, PartN<3600, PartK<192, PartM<120, Ukernel<4, 12>>>>>;
+// out of core (out_of_core.rs:77-87): blocks of C streamed through the GPU,
+// each block's product ordered as the tree's resident-B inner part
+using OutOfCore = SpawnThreads<PartM<700, PartN<700, Barrier<GpuSchedule<
+    PartK<768, PartN<768, PartM<120, PartK<192, Ukernel<4, 12>>>>>>>>>>;
+// the l3 experiment's Goto tree on hierarchical operands (l3.rs:80-108):
+// A, B and C packed into device Hierarch buffers laid out by the tree
+using GotoHier = UnpackC<4, 12, 12, 1, PackB<192, 12, 12, 1, PackA<4, 192, 1, 4,
+                 HipGemmHier<GotoDesc>>>>;
 
 template <class Algo, class T = double>
 static void check_tree(const char* name, size_t m, size_t n, size_t k, size_t threads,
@@ -47,7 +74,7 @@ static void check_tree(const char* name, size_t m, size_t n, size_t k, size_t th
   Matrix<T> c(m, n);
   c.copy_from(c0);
   Algo algo;
-  if constexpr (std::is_same_v<Algo, SpawnThreads<HipGemm<double>>>) algo.set_n_threads(threads);
+  set_threads(algo, threads);
   auto t0 = std::chrono::steady_clock::now();
   if (device) {
     auto da = Matrix<T>::device(m, k), db = Matrix<T>::device(k, n), dc = Matrix<T>::device(m, n);
@@ -95,6 +122,16 @@ int main() {
   check_tree<PartN<3000, PartK<192, HipCrtGemm<13>>>>("PartN<3000,PartK<192,HipCrtGemm<13>>>",
                                                       257, 3100, 400, 1, 2.0, false);
   check_tree<HipGemm<float>, float>("HipGemm<float> (host views)", 300, 211, 517, 1, 2.0, false);
+  check_tree<GpuSchedule<GotoDesc>>("GpuSchedule<Goto>", 1000, 3100, 600, 1, 2.0, true);
+  check_tree<GpuSchedule<L3ADesc>>("GpuSchedule<L3A>", 1000, 900, 600, 1, 2.0, true);
+  check_tree<GpuSchedule<L3BDesc>>("GpuSchedule<L3B>", 900, 1000, 600, 1, 2.0, true);
+  check_tree<GpuSchedule<L3CDesc>>("GpuSchedule<L3C>", 1300, 1250, 400, 1, 2.0, true);
+  check_tree<GpuSchedule<L4CDesc>>("GpuSchedule<L4C> (host views)", 700, 800, 300, 1, 2.0, false);
+  check_tree<OutOfCore>("SpawnThreads<PartM<700,PartN<700,Barrier<GpuSchedule>>>> x2", 1500, 1400,
+                        500, 2, 2.0, false);
+  check_tree<GotoHier>("UnpackC<PackB<PackA<HipGemmHier<Goto>>>>", 700, 500, 450, 1, 2.0, false);
+  check_tree<SpawnThreads<GotoHier>>("SpawnThreads<UnpackC<...HipGemmHier>> x3", 300, 260, 200, 3,
+                                     2.0, false);
   std::printf("%s (%d failures)\n", fails ? "FAIL" : "PASS", fails);
   return fails ? 1 : 0;
 }

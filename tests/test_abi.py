@@ -189,3 +189,20 @@ def test_crt_i8_rejects_k_beyond_exact_int32_sums():
     assert st == -3 and "131072" in capi.last_error()
     assert lib.momms_hip_crt_i8_beta(8192, 15) == 51
     assert [lib.momms_hip_crt_i8_modulus(i) for i in (0, 15, 16)] == [255, 191, 0]
+
+
+def test_dgemm_tree_rejects_bad_steps_before_gpu_work():
+    al = D(1.0)
+    lib = capi.lib()
+    steps = (capi.AlgoStep * 9)()
+    for s in steps:
+        s.kind, s.bsz = capi.STEP_M, 128
+    args = (4, 4, 4, ctypes.byref(al), 8, 1, 4, 8, 1, 4, ctypes.byref(al), 8, 1, 4)
+    assert lib.momms_hip_dgemm_tree(*args, ctypes.addressof(steps), 9, None) == -15
+    steps[0].bsz = 0
+    assert lib.momms_hip_dgemm_tree(*args, ctypes.addressof(steps), 1, None) == -15
+    steps[0].kind, steps[0].bsz = 7, 128
+    assert lib.momms_hip_dgemm_tree(*args, ctypes.addressof(steps), 1, None) == -15
+    assert lib.momms_hip_dgemm_tree(*args, None, 2, None) == -16
+    assert lib.momms_hip_dgemm_tree(*args, None, -1, None) == -16
+    assert "momms_hip_dgemm_tree" in capi.last_error()

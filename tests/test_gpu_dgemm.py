@@ -218,3 +218,43 @@ def test_stream_k_is_deterministic_and_accurate(tile, m, n, k):
     assert (np.abs(c[rows, cols] - ref) <= gamma(k + 1) * mag).all()
     ref0, mag0 = O.dd_entries(a, b, c0, rows, cols, 0.5, 0.0)
     assert (np.abs(host(C3)[rows, cols] - ref0) <= gamma(k + 1) * mag0).all()
+
+
+@pytest.mark.parametrize("tree", sorted(capi.REFERENCE_TREES))
+@pytest.mark.parametrize("m,n,k", [(1000, 3100, 600), (1300, 1250, 400), (257, 130, 77)])
+def test_tree_order_gives_the_bits_of_the_128_tile(tree, m, n, k):
+    """momms_hip_dgemm_tree with the reference's variant trees: the tile order
+    changes which panels share an L2, never a tile's arithmetic, so every
+    tree gives the bits of tile configuration 0; and within gamma_{k+1} of
+    double-double at sampled entries."""
+    rng = np.random.default_rng(SEED + m + n + k + len(tree))
+    a, b, c0 = (uniform_colmajor(rng, *s) for s in ((m, k), (k, n), (m, n)))
+    A, B = to_dev(a), to_dev(b)
+    C1, C2 = to_dev(c0), to_dev(c0)
+    capi.dgemm_tree(A, B, C1, capi.REFERENCE_TREES[tree], 1.5, -0.5)
+    capi.dgemm(A, B, C2, 1.5, -0.5, tile_cfg=0)
+    torch.cuda.synchronize()
+    assert torch.equal(C1, C2)
+    rows, cols = rng.integers(0, m, 2048), rng.integers(0, n, 2048)
+    ref, mag = O.dd_entries(a, b, c0, rows, cols, 1.5, -0.5)
+    assert (np.abs(host(C1)[rows, cols] - ref) <= gamma(k + 1) * mag).all()
+
+
+def test_tree_order_on_row_major_operands_and_an_empty_tree():
+    rng = np.random.default_rng(SEED + 99)
+    m, n, k = 300, 200, 150
+    A, B, C = layout_view(rng, m, k, "row"), layout_view(rng, k, n, "col"), layout_view(rng, m, n, "row")
+    C0 = C.clone()
+    C2 = C.clone()
+    capi.dgemm_tree(A, B, C, capi.REFERENCE_TREES["l3c"], 1.0, 1.0)
+    capi.dgemm(A, B, C2, 1.0, 1.0, tile_cfg=0)
+    torch.cuda.synchronize()
+    assert torch.equal(C, C2)
+    check_against_oracle(A, B, C0, C, 1.0, 1.0, k)
+    A2, B2 = to_dev(uniform_colmajor(rng, m, k)), to_dev(uniform_colmajor(rng, k, n))
+    C3, C4 = to_dev(uniform_colmajor(rng, m, n)), None
+    C4 = C3.clone()
+    capi.dgemm_tree(A2, B2, C3, [], 1.0, 1.0)
+    capi.dgemm(A2, B2, C4, 1.0, 1.0, tile_cfg=0)
+    torch.cuda.synchronize()
+    assert torch.equal(C3, C4)
