@@ -82,6 +82,18 @@ int momms_hip_dgemm_noalloc(int64_t m, int64_t n, int64_t k, const double* alpha
                             int64_t rsB, int64_t csB, const double* beta, double* C,
                             int64_t rsC, int64_t csC, void* stream);
 
+/* Small FP64 GEMM in one launch, without packing or workspace: the device
+ * counterpart of the reference's libxsmm leaves Xsmm and KernelXsmmA2
+ * (src/kern/kernel_xsmm.rs:8-139), which multiply a small block straight
+ * from its operands when all are column-major, or all row-major (then
+ * C^T = B^T A^T). One wavefront per 32 x 32 tile of C, registers only. Other
+ * strides, leading dimensions over 2^21 or operands over 2^28 doubles run
+ * momms_hip_dgemm. Same arguments and conventions as momms_hip_dgemm; the
+ * result is within the same gamma_{k+1} bound (another summation order). */
+int momms_hip_dgemm_small(int64_t m, int64_t n, int64_t k, const double* alpha, const double* A,
+                          int64_t rsA, int64_t csA, const double* B, int64_t rsB, int64_t csB,
+                          const double* beta, double* C, int64_t rsC, int64_t csC, void* stream);
+
 /* ---- FP32-accumulating GEMMs ------------------------------------------------
  * C := beta*C + alpha*A*B with float C, alpha, beta; arguments as
  * momms_hip_dgemm. momms_hip_sgemm takes float A, B (FP32 matrix cores,
@@ -348,6 +360,17 @@ int momms_hip_hier_pack(const momms_hier_layout* L, const double* A, int64_t rsA
  * for every element of the h x w matrix. */
 int momms_hip_hier_unpack_add(const momms_hier_layout* L, const double* P, double* C,
                               int64_t rsC, int64_t csC, void* stream);
+
+/* momms_hip_hier_pack and momms_hip_hier_unpack_add for Hierarch<f32>
+ * buffers (the KNM sgemm tree's packed operands, src/exper/knm.rs:89-97),
+ * and the copy form of the unpack: C := P at the matrix's places (C not
+ * read), which gives a column-major copy of a hierarchical operand. */
+int momms_hip_hier_pack_f32(const momms_hier_layout* L, const float* A, int64_t rsA, int64_t csA,
+                            float* P, void* stream);
+int momms_hip_hier_unpack_add_f32(const momms_hier_layout* L, const float* P, float* C,
+                                  int64_t rsC, int64_t csC, void* stream);
+int momms_hip_hier_unpack_f32(const momms_hier_layout* L, const float* P, float* C, int64_t rsC,
+                              int64_t csC, void* stream);
 
 /* GemmNode::run over Hierarch<f64> operands, as the reference's l3
  * experiment runs its control trees on them (src/exper/l3.rs:46-72): C :=

@@ -45,6 +45,11 @@ _SIGS: dict[str, list] = {
                               _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_int, _c_vp],
     "momms_hip_dgemm_tree": [_c_i64, _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_vp,
                              _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_vp, _c_i64, _c_vp],
+    "momms_hip_dgemm_small": [_c_i64, _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_vp,
+                              _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_vp],
+    "momms_hip_hier_pack_f32": [_c_vp, _c_vp, _c_i64, _c_i64, _c_vp, _c_vp],
+    "momms_hip_hier_unpack_add_f32": [_c_vp, _c_vp, _c_vp, _c_i64, _c_i64, _c_vp],
+    "momms_hip_hier_unpack_f32": [_c_vp, _c_vp, _c_vp, _c_i64, _c_i64, _c_vp],
     "momms_hip_dgemm_noalloc": [_c_i64, _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_vp,
                                 _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_vp],
     "momms_last_error_string": [],
@@ -213,6 +218,21 @@ REFERENCE_TREES = {
     "l3c": [(0, 4), (1, 12), (0, 156), (2, 156), (0, 624), (1, 624)],
     "l4c": [(0, 4), (1, 12), (0, 120), (2, 192), (1, 3600), (0, 3600)],
 }
+
+
+def dgemm_small(A, B, C, alpha: float = 1.0, beta: float = 1.0, stream=None):
+    """C := beta*C + alpha*A*B by the single-launch small-GEMM leaf."""
+    _f64((A, "A"), (B, "B"), (C, "C"))
+    m, k = A.shape
+    n = B.shape[1]
+    a, rsa, csa = _mat(A, m, k, "A")
+    b, rsb, csb = _mat(B, k, n, "B")
+    c, rsc, csc = _mat(C, m, n, "C", output=True)
+    al, be = ctypes.c_double(alpha), ctypes.c_double(beta)
+    check("momms_hip_dgemm_small",
+          lib().momms_hip_dgemm_small(m, n, k, ctypes.byref(al), a, rsa, csa, b, rsb, csb,
+                                      ctypes.byref(be), c, rsc, csc, stream_handle(stream)))
+    return C
 
 
 def dgemm_tree(A, B, C, steps, alpha: float = 1.0, beta: float = 1.0, stream=None):
@@ -601,27 +621,55 @@ def hier_layout(h: int, w: int, steps, ydim: int, xdim: int, lh: int, lw: int, l
     return L
 
 
-def hier_pack(L: HierLayout, A, P, stream=None):
-    """Packer<Matrix, Hierarch>::pack (src/composables/pack.rs:153-309): P (a
-    float64 GPU tensor of L.capacity elements) := A in layout L, padding 0."""
-    _f64((A, "A"), (P, "P"))
-    a, ra, ca = _mat(A, L.h, L.w, "A")
+def _hier_dtype(*ts):
+    """'' for float64 tensors, '_f32' for float32 ones (all alike)."""
+    import torch
+
+    dts = {t.dtype for t, _ in ts}
+    if dts == {torch.float64}:
+        return ""
+    if dts == {torch.float32}:
+        return "_f32"
+    raise TypeError(f"{', '.join(n for _, n in ts)} must be all float64 or all float32")
+
+
+def _hier_buf(L, P):
     if P.numel() < L.capacity or not P.is_cuda or (P.numel() > 1 and not P.is_contiguous()):
         raise ValueError(f"P must be a contiguous GPU tensor of >= {L.capacity} elements")
-    check("momms_hip_hier_pack",
-          lib().momms_hip_hier_pack(ctypes.byref(L), a, ra, ca, P.data_ptr(), stream_handle(stream)))
+
+
+def hier_pack(L: HierLayout, A, P, stream=None):
+    """Packer<Matrix, Hierarch>::pack (src/composables/pack.rs:153-309): P (a
+    float64 or float32 GPU tensor of L.capacity elements, as A) := A in
+    layout L, padding 0."""
+    sfx = _hier_dtype((A, "A"), (P, "P"))
+    a, ra, ca = _mat(A, L.h, L.w, "A")
+    _hier_buf(L, P)
+    name = "momms_hip_hier_pack" + sfx
+    check(name, getattr(lib(), name)(ctypes.byref(L), a, ra, ca, P.data_ptr(),
+                                     stream_handle(stream)))
     return P
 
 
 def hier_unpack_add(L: HierLayout, P, C, stream=None):
     """Unpacker<Matrix, Hierarch>::add (src/composables/unpack.rs:174-213):
     C[y][x] += P[offset(y, x)] over the h x w matrix C (in place)."""
-    _f64((P, "P"), (C, "C"))
+    sfx = _hier_dtype((P, "P"), (C, "C"))
     c, rc, cc = _mat(C, L.h, L.w, "C", output=True)
-    if P.numel() < L.capacity or not P.is_cuda or (P.numel() > 1 and not P.is_contiguous()):
-        raise ValueError(f"P must be a contiguous GPU tensor of >= {L.capacity} elements")
-    check("momms_hip_hier_unpack_add",
-          lib().momms_hip_hier_unpack_add(ctypes.byref(L), P.data_ptr(), c, rc, cc,
+    _hier_buf(L, P)
+    name = "momms_hip_hier_unpack_add" + sfx
+    check(name, getattr(lib(), name)(ctypes.byref(L), P.data_ptr(), c, rc, cc,
+                                     stream_handle(stream)))
+    return C
+
+
+def hier_unpack_f32(L: HierLayout, P, C, stream=None):
+    """C[y][x] := P[offset(y, x)] (float32, C not read)."""
+    _hier_dtype((P, "P"), (C, "C"))
+    c, rc, cc = _mat(C, L.h, L.w, "C", output=True)
+    _hier_buf(L, P)
+    check("momms_hip_hier_unpack_f32",
+          lib().momms_hip_hier_unpack_f32(ctypes.byref(L), P.data_ptr(), c, rc, cc,
                                           stream_handle(stream)))
     return C
 

@@ -52,10 +52,10 @@ hier_tables(momms_hier_layout L, int64_t* __restrict__ t) {
 //    by the G minor indices, so a wave touches 64 consecutive places.
 // Placement is from the offset tables in either order; the order only decides
 // which places one wave instruction touches.
-template <int G>
+template <typename T, int G>
 struct HierTile {
   static constexpr int kStride = 256 + 4;  // conflict-free reads in layout order
-  double v[G * kStride];
+  T v[G * kStride];
   int64_t ou[256];
   int64_t ov[G];
 };
@@ -72,12 +72,12 @@ __device__ __forceinline__ void layout_order(int q, int log2r, int& uu, int& c) 
 // h == 0 or w == 0 (grid-uniform): nothing is read and every place is padding.
 // PADW: padding places are written (zeros); otherwise only the matrix's own
 // places are.
-template <int G, bool LX, bool PADW>
+template <typename T, int G, bool LX, bool PADW>
 __global__ void __launch_bounds__(256)
 hier_pack_kernel(int64_t h, int64_t w, int64_t h_pad, int64_t w_pad, uint64_t cap, uint32_t nbu,
                  int log2r, const int64_t* __restrict__ oy, const int64_t* __restrict__ ox,
-                 const double* __restrict__ A, int64_t rs, int64_t cs, double* __restrict__ P) {
-  __shared__ HierTile<G> T;
+                 const T* __restrict__ A, int64_t rs, int64_t cs, T* __restrict__ P) {
+  __shared__ HierTile<T, G> S;
   const uint32_t bu = blockIdx.x % nbu, bv = blockIdx.x / nbu;
   const int64_t u0 = (int64_t)bu * 256, v0 = (int64_t)bv * G;
   const int64_t u = u0 + threadIdx.x;
@@ -86,7 +86,7 @@ hier_pack_kernel(int64_t h, int64_t w, int64_t h_pad, int64_t w_pad, uint64_t ca
   const int64_t su = LX ? cs : rs, sv = LX ? rs : cs;
   const int64_t* ou = LX ? ox : oy;
   const int64_t* ov = LX ? oy : ox;
-  double v[G];
+  T v[G];
   if (h > 0 && w > 0) {
     // all loads first (clamped into the matrix; padding is replaced by 0)
     const int64_t uc = u < hu ? u : hu - 1;
@@ -97,34 +97,34 @@ hier_pack_kernel(int64_t h, int64_t w, int64_t h_pad, int64_t w_pad, uint64_t ca
     }
 #pragma unroll
     for (int c = 0; c < G; ++c)
-      if (u >= hu || v0 + c >= hv) v[c] = 0.0;
+      if (u >= hu || v0 + c >= hv) v[c] = T(0);
   } else {
 #pragma unroll
-    for (int c = 0; c < G; ++c) v[c] = 0.0;
+    for (int c = 0; c < G; ++c) v[c] = T(0);
   }
-  T.ou[threadIdx.x] = ou[u < U ? u : U - 1];
-  if (threadIdx.x < G) T.ov[threadIdx.x] = ov[v0 + threadIdx.x < V ? v0 + threadIdx.x : V - 1];
+  S.ou[threadIdx.x] = ou[u < U ? u : U - 1];
+  if (threadIdx.x < G) S.ov[threadIdx.x] = ov[v0 + threadIdx.x < V ? v0 + threadIdx.x : V - 1];
 #pragma unroll
-  for (int c = 0; c < G; ++c) T.v[c * HierTile<G>::kStride + threadIdx.x] = v[c];
+  for (int c = 0; c < G; ++c) S.v[c * HierTile<T, G>::kStride + threadIdx.x] = v[c];
   __syncthreads();
 #pragma unroll
   for (int q = 0; q < G; ++q) {
     int uu, c;
     layout_order<G>(q, log2r, uu, c);
-    const int64_t o = T.ou[uu] + T.ov[c];
+    const int64_t o = S.ou[uu] + S.ov[c];
     const bool place = PADW ? (u0 + uu < U && v0 + c < V) : (u0 + uu < hu && v0 + c < hv);
-    if (place && (uint64_t)o < cap) P[o] = T.v[c * HierTile<G>::kStride + uu];
+    if (place && (uint64_t)o < cap) P[o] = S.v[c * HierTile<T, G>::kStride + uu];
   }
 }
 
 // ADD: C += P at the matrix's places; otherwise C := P there (C not read).
-template <int G, bool LX, bool ADD>
+template <typename T, int G, bool LX, bool ADD>
 __global__ void __launch_bounds__(256)
 hier_unpack_add_kernel(int64_t h, int64_t w, uint64_t cap, uint32_t nbu, int log2r,
                        const int64_t* __restrict__ oy, const int64_t* __restrict__ ox,
-                       const double* __restrict__ P, double* __restrict__ C, int64_t rs,
+                       const T* __restrict__ P, T* __restrict__ C, int64_t rs,
                        int64_t cs) {
-  __shared__ HierTile<G> T;
+  __shared__ HierTile<T, G> S;
   const uint32_t bu = blockIdx.x % nbu, bv = blockIdx.x / nbu;
   const int64_t u0 = (int64_t)bu * 256, v0 = (int64_t)bv * G;
   const int64_t u = u0 + threadIdx.x;
@@ -134,30 +134,30 @@ hier_unpack_add_kernel(int64_t h, int64_t w, uint64_t cap, uint32_t nbu, int log
   const int64_t* ov = LX ? oy : ox;
   // C first (its loads overlap the layout side), clamped into the matrix
   const int64_t uc = u < hu ? u : hu - 1;
-  double c0[G];
+  T c0[G];
 #pragma unroll
   for (int c = 0; c < G; ++c) {
     const int64_t j = v0 + c < hv ? v0 + c : hv - 1;
-    c0[c] = ADD ? C[uc * su + j * sv] : 0.0;
+    c0[c] = ADD ? C[uc * su + j * sv] : T(0);
   }
-  T.ou[threadIdx.x] = ou[uc];
-  if (threadIdx.x < G) T.ov[threadIdx.x] = ov[v0 + threadIdx.x < hv ? v0 + threadIdx.x : hv - 1];
+  S.ou[threadIdx.x] = ou[uc];
+  if (threadIdx.x < G) S.ov[threadIdx.x] = ov[v0 + threadIdx.x < hv ? v0 + threadIdx.x : hv - 1];
   __syncthreads();
-  double p[G];
+  T p[G];
 #pragma unroll
   for (int q = 0; q < G; ++q) {
     int uu, c;
     layout_order<G>(q, log2r, uu, c);
-    const int64_t o = T.ou[uu] + T.ov[c];
+    const int64_t o = S.ou[uu] + S.ov[c];
     const bool in = (uint64_t)o < cap;
     p[q] = P[in ? o : 0];
-    p[q] = in ? p[q] : 0.0;
+    p[q] = in ? p[q] : T(0);
   }
 #pragma unroll
   for (int q = 0; q < G; ++q) {
     int uu, c;
     layout_order<G>(q, log2r, uu, c);
-    T.v[c * HierTile<G>::kStride + uu] = p[q];
+    S.v[c * HierTile<T, G>::kStride + uu] = p[q];
   }
   __syncthreads();
   if (u >= hu) return;
@@ -165,8 +165,8 @@ hier_unpack_add_kernel(int64_t h, int64_t w, uint64_t cap, uint32_t nbu, int log
   for (int c = 0; c < G; ++c)
     if (v0 + c < hv)
       C[u * su + (v0 + c) * sv] =
-          ADD ? T.v[c * HierTile<G>::kStride + threadIdx.x] + c0[c]
-              : T.v[c * HierTile<G>::kStride + threadIdx.x];
+          ADD ? S.v[c * HierTile<T, G>::kStride + threadIdx.x] + c0[c]
+              : S.v[c * HierTile<T, G>::kStride + threadIdx.x];
 }
 
 // Lanes along the source's unit stride (rows of a column-major matrix, the
@@ -193,24 +193,24 @@ static HierShape hier_shape(const momms_hier_layout& L, int64_t rs, int64_t cs) 
   return sh;
 }
 
-template <int G, bool LX, bool PADW>
-static void launch_pack(const momms_hier_layout& L, int log2r, const int64_t* t, const double* A,
-                        int64_t rs, int64_t cs, double* P, hipStream_t s) {
+template <typename T, int G, bool LX, bool PADW>
+static void launch_pack(const momms_hier_layout& L, int log2r, const int64_t* t, const T* A,
+                        int64_t rs, int64_t cs, T* P, hipStream_t s) {
   const int64_t U = LX ? L.w_pad : L.h_pad, V = LX ? L.h_pad : L.w_pad;
   const uint32_t nbu = (uint32_t)ceil_div(U, 256);
   const uint32_t blocks = nbu * (uint32_t)ceil_div(V, G);
-  hipLaunchKernelGGL((hier_pack_kernel<G, LX, PADW>), dim3(blocks), dim3(256), 0, s, L.h, L.w,
+  hipLaunchKernelGGL((hier_pack_kernel<T, G, LX, PADW>), dim3(blocks), dim3(256), 0, s, L.h, L.w,
                      L.h_pad, L.w_pad, (uint64_t)L.capacity, nbu, log2r, t, t + L.h_pad, A, rs, cs,
                      P);
 }
 
-template <int G, bool LX, bool ADD>
+template <typename T, int G, bool LX, bool ADD>
 static void launch_unpack(const momms_hier_layout& L, int log2r, const int64_t* t,
-                          const double* P, double* C, int64_t rs, int64_t cs, hipStream_t s) {
+                          const T* P, T* C, int64_t rs, int64_t cs, hipStream_t s) {
   const int64_t U = LX ? L.w : L.h, V = LX ? L.h : L.w;
   const uint32_t nbu = (uint32_t)ceil_div(U, 256);
   const uint32_t blocks = nbu * (uint32_t)ceil_div(V, G);
-  hipLaunchKernelGGL((hier_unpack_add_kernel<G, LX, ADD>), dim3(blocks), dim3(256), 0, s, L.h,
+  hipLaunchKernelGGL((hier_unpack_add_kernel<T, G, LX, ADD>), dim3(blocks), dim3(256), 0, s, L.h,
                      L.w, (uint64_t)L.capacity, nbu, log2r, t, t + L.h_pad, P, C, rs, cs);
 }
 
@@ -240,29 +240,29 @@ static int check_layout(const char* fn, int arg, const momms_hier_layout* L) {
 }
 
 // strided matrix -> layout (all padding places written as 0 when padw)
-template <bool PADW>
-static hipError_t hier_pack_impl(const momms_hier_layout& L, const double* A, int64_t rs,
-                                 int64_t cs, double* P, hipStream_t s) {
+template <typename T, bool PADW>
+static hipError_t hier_pack_impl(const momms_hier_layout& L, const T* A, int64_t rs,
+                                 int64_t cs, T* P, hipStream_t s) {
   if (L.capacity == 0 || (!PADW && (L.h == 0 || L.w == 0))) return hipSuccess;
   int64_t* t = nullptr;
   hipError_t e = hier_launch_tables(L, s, &t);
   if (e != hipSuccess) return e;
   const HierShape sh = hier_shape(L, rs, cs);
   if (sh.lx) {
-    if (sh.g == 16) launch_pack<16, true, PADW>(L, sh.log2r, t, A, rs, cs, P, s);
-    else if (sh.g == 12) launch_pack<12, true, PADW>(L, sh.log2r, t, A, rs, cs, P, s);
-    else launch_pack<8, true, PADW>(L, sh.log2r, t, A, rs, cs, P, s);
+    if (sh.g == 16) launch_pack<T, 16, true, PADW>(L, sh.log2r, t, A, rs, cs, P, s);
+    else if (sh.g == 12) launch_pack<T, 12, true, PADW>(L, sh.log2r, t, A, rs, cs, P, s);
+    else launch_pack<T, 8, true, PADW>(L, sh.log2r, t, A, rs, cs, P, s);
   } else {
-    if (sh.g == 16) launch_pack<16, false, PADW>(L, sh.log2r, t, A, rs, cs, P, s);
-    else if (sh.g == 12) launch_pack<12, false, PADW>(L, sh.log2r, t, A, rs, cs, P, s);
-    else launch_pack<8, false, PADW>(L, sh.log2r, t, A, rs, cs, P, s);
+    if (sh.g == 16) launch_pack<T, 16, false, PADW>(L, sh.log2r, t, A, rs, cs, P, s);
+    else if (sh.g == 12) launch_pack<T, 12, false, PADW>(L, sh.log2r, t, A, rs, cs, P, s);
+    else launch_pack<T, 8, false, PADW>(L, sh.log2r, t, A, rs, cs, P, s);
   }
   return hipGetLastError();
 }
 
 // layout -> strided matrix: C += P (ADD) or C := P at the matrix's places
-template <bool ADD>
-static hipError_t hier_unpack_impl(const momms_hier_layout& L, const double* P, double* C,
+template <typename T, bool ADD>
+static hipError_t hier_unpack_impl(const momms_hier_layout& L, const T* P, T* C,
                                    int64_t rs, int64_t cs, hipStream_t s) {
   if (L.h == 0 || L.w == 0) return hipSuccess;
   int64_t* t = nullptr;
@@ -270,13 +270,13 @@ static hipError_t hier_unpack_impl(const momms_hier_layout& L, const double* P, 
   if (e != hipSuccess) return e;
   const HierShape sh = hier_shape(L, rs, cs);
   if (sh.lx) {
-    if (sh.g == 16) launch_unpack<16, true, ADD>(L, sh.log2r, t, P, C, rs, cs, s);
-    else if (sh.g == 12) launch_unpack<12, true, ADD>(L, sh.log2r, t, P, C, rs, cs, s);
-    else launch_unpack<8, true, ADD>(L, sh.log2r, t, P, C, rs, cs, s);
+    if (sh.g == 16) launch_unpack<T, 16, true, ADD>(L, sh.log2r, t, P, C, rs, cs, s);
+    else if (sh.g == 12) launch_unpack<T, 12, true, ADD>(L, sh.log2r, t, P, C, rs, cs, s);
+    else launch_unpack<T, 8, true, ADD>(L, sh.log2r, t, P, C, rs, cs, s);
   } else {
-    if (sh.g == 16) launch_unpack<16, false, ADD>(L, sh.log2r, t, P, C, rs, cs, s);
-    else if (sh.g == 12) launch_unpack<12, false, ADD>(L, sh.log2r, t, P, C, rs, cs, s);
-    else launch_unpack<8, false, ADD>(L, sh.log2r, t, P, C, rs, cs, s);
+    if (sh.g == 16) launch_unpack<T, 16, false, ADD>(L, sh.log2r, t, P, C, rs, cs, s);
+    else if (sh.g == 12) launch_unpack<T, 12, false, ADD>(L, sh.log2r, t, P, C, rs, cs, s);
+    else launch_unpack<T, 8, false, ADD>(L, sh.log2r, t, P, C, rs, cs, s);
   }
   return hipGetLastError();
 }
@@ -350,7 +350,7 @@ extern "C" int momms_hip_hier_pack(const momms_hier_layout* L, const double* A, 
   if ((st = check_layout(fn, 1, L))) return st;
   if ((st = check_operand(fn, 2, A, L->h, L->w, rsA, csA))) return st;
   if (!P && L->capacity) return bad_arg(fn, 5, "P is null");
-  return hip_status(fn, hier_pack_impl<true>(*L, A, rsA, csA, P, (hipStream_t)stream));
+  return hip_status(fn, hier_pack_impl<double, true>(*L, A, rsA, csA, P, (hipStream_t)stream));
 }
 
 extern "C" int momms_hip_hier_unpack_add(const momms_hier_layout* L, const double* P, double* C,
@@ -360,7 +360,37 @@ extern "C" int momms_hip_hier_unpack_add(const momms_hier_layout* L, const doubl
   if ((st = check_layout(fn, 1, L))) return st;
   if (!P && L->capacity) return bad_arg(fn, 2, "P is null");
   if ((st = check_operand(fn, 3, C, L->h, L->w, rsC, csC))) return st;
-  return hip_status(fn, hier_unpack_impl<true>(*L, P, C, rsC, csC, (hipStream_t)stream));
+  return hip_status(fn, hier_unpack_impl<double, true>(*L, P, C, rsC, csC, (hipStream_t)stream));
+}
+
+extern "C" int momms_hip_hier_pack_f32(const momms_hier_layout* L, const float* A, int64_t rsA,
+                                       int64_t csA, float* P, void* stream) {
+  static const char* fn = "momms_hip_hier_pack_f32";
+  int st;
+  if ((st = check_layout(fn, 1, L))) return st;
+  if ((st = check_operand(fn, 2, A, L->h, L->w, rsA, csA))) return st;
+  if (!P && L->capacity) return bad_arg(fn, 5, "P is null");
+  return hip_status(fn, hier_pack_impl<float, true>(*L, A, rsA, csA, P, (hipStream_t)stream));
+}
+
+extern "C" int momms_hip_hier_unpack_add_f32(const momms_hier_layout* L, const float* P,
+                                             float* C, int64_t rsC, int64_t csC, void* stream) {
+  static const char* fn = "momms_hip_hier_unpack_add_f32";
+  int st;
+  if ((st = check_layout(fn, 1, L))) return st;
+  if (!P && L->capacity) return bad_arg(fn, 2, "P is null");
+  if ((st = check_operand(fn, 3, C, L->h, L->w, rsC, csC))) return st;
+  return hip_status(fn, hier_unpack_impl<float, true>(*L, P, C, rsC, csC, (hipStream_t)stream));
+}
+
+extern "C" int momms_hip_hier_unpack_f32(const momms_hier_layout* L, const float* P, float* C,
+                                         int64_t rsC, int64_t csC, void* stream) {
+  static const char* fn = "momms_hip_hier_unpack_f32";
+  int st;
+  if ((st = check_layout(fn, 1, L))) return st;
+  if (!P && L->capacity) return bad_arg(fn, 2, "P is null");
+  if ((st = check_operand(fn, 3, C, L->h, L->w, rsC, csC))) return st;
+  return hip_status(fn, hier_unpack_impl<float, false>(*L, P, C, rsC, csC, (hipStream_t)stream));
 }
 
 extern "C" int momms_hip_dgemm_hier(const momms_hier_layout* LA, const double* PA,
@@ -389,12 +419,12 @@ extern "C" int momms_hip_dgemm_hier(const momms_hier_layout* LA, const double* P
   hipError_t e = stream_scratch(s, (size_t)(m * k + k * n + m * n) * sizeof(double), (void**)&W, 1);
   if (e != hipSuccess) return hip_status(fn, e);
   double *A = W, *B = W + m * k, *C = W + m * k + k * n;
-  if ((e = hier_unpack_impl<false>(*LA, PA, A, 1, m, s)) != hipSuccess) return hip_status(fn, e);
-  if ((e = hier_unpack_impl<false>(*LB, PB, B, 1, k > 0 ? k : 1, s)) != hipSuccess)
+  if ((e = hier_unpack_impl<double, false>(*LA, PA, A, 1, m, s)) != hipSuccess) return hip_status(fn, e);
+  if ((e = hier_unpack_impl<double, false>(*LB, PB, B, 1, k > 0 ? k : 1, s)) != hipSuccess)
     return hip_status(fn, e);
-  if (*beta != 0.0 && (e = hier_unpack_impl<false>(*LC, PC, C, 1, m, s)) != hipSuccess)
+  if (*beta != 0.0 && (e = hier_unpack_impl<double, false>(*LC, PC, C, 1, m, s)) != hipSuccess)
     return hip_status(fn, e);
   if ((st = momms_hip_dgemm(m, n, k, alpha, A, 1, m, B, 1, k > 0 ? k : 1, beta, C, 1, m, stream)))
     return st;
-  return hip_status(fn, hier_pack_impl<false>(*LC, C, 1, m, PC, s));
+  return hip_status(fn, hier_pack_impl<double, false>(*LC, C, 1, m, PC, s));
 }

@@ -143,6 +143,35 @@ struct HipGemm {
   static std::vector<AlgorithmStep> hierarchy_description() { return {}; }
 };
 
+// The libxsmm leaves Xsmm / KernelXsmmA2 (src/kern/kernel_xsmm.rs:8-139) on
+// the GPU: a small block in one launch, no packing (momms_hip_dgemm_small).
+struct HipXsmm {
+  hipStream_t stream = nullptr;
+  detail::DevBuf da, db, dc;
+
+  template <class At, class Bt, class Ct>
+  void run(At& a, Bt& b, Ct& c, const ThreadInfo& thr) {
+    if (thr.thread_id() == 0) launch(a, b, c);
+    if (thr.num_threads() > 1) thr.barrier();
+  }
+  template <class At, class Bt, class Ct>
+  void launch(At& a, Bt& b, Ct& c) {
+    const size_t m = c.height(), n = c.width(), k = a.width();
+    if (m == 0 || n == 0) return;
+    const double alpha = a.get_scalar() * b.get_scalar();
+    const double beta = c.get_scalar();
+    auto A = detail::stage_in<double>(a, m, k, da, true, stream);
+    auto B = detail::stage_in<double>(b, k, n, db, true, stream);
+    auto C = detail::stage_in<double>(c, m, n, dc, beta != 0.0, stream);
+    abi_check("momms_hip_dgemm_small",
+              momms_hip_dgemm_small((int64_t)m, (int64_t)n, (int64_t)k, &alpha, A.ptr, A.rs, A.cs,
+                                    B.ptr, B.rs, B.cs, &beta, C.ptr, C.rs, C.cs, stream));
+    detail::stage_out<double>(c, m, n, C.ptr, stream);
+    hip_check("hipStreamSynchronize", hipStreamSynchronize(stream));
+  }
+  static std::vector<AlgorithmStep> hierarchy_description() { return {}; }
+};
+
 // FP64-accurate GEMM from Slices BF16 slices (momms_hip_dgemm_split_bf16).
 template <int Slices>
 struct HipSplitGemm {

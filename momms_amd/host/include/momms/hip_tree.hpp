@@ -18,6 +18,8 @@
 //                            (src/composables/unpack.rs:214-270)
 //   HipGemmHier<Desc>        leaf on three HierMatrix operands
 //                            (momms_hip_dgemm_hier)
+//   KnmBlock<Mr, Nr>         the KNM f32 tree's micro-kernel loops over its
+//                            packed f32 operands (src/exper/knm.rs:89-97)
 // As with HipGemm, under a team of threads only thread 0 launches and the
 // team meets at a barrier; host-resident views are staged to HBM.
 #pragma once
@@ -92,17 +94,18 @@ struct GpuSchedule {
   }
 };
 
-// A device-resident hierarchical buffer (Hierarch<f64, LH, LW, LRS, LCS>):
-// its layout, the device memory (grown, never shrunk: a ResizableBuffer,
-// src/matrix/hierarch.rs:318-356) and the scalar.
-class HierMatrix {
+// A device-resident hierarchical buffer (Hierarch<T, LH, LW, LRS, LCS>, T =
+// double or float): its layout, the device memory (grown, never shrunk: a
+// ResizableBuffer, src/matrix/hierarch.rs:318-356) and the scalar.
+template <class T>
+class HierMatrixT {
  public:
   momms_hier_layout layout{};
-  double* ptr() const { return static_cast<double*>(buf_.p); }
+  T* ptr() const { return static_cast<T*>(buf_.p); }
   size_t height() const { return (size_t)layout.h; }
   size_t width() const { return (size_t)layout.w; }
-  double get_scalar() const { return scalar_; }
-  void set_scalar(double s) { scalar_ = s; }
+  T get_scalar() const { return scalar_; }
+  void set_scalar(T s) { scalar_ = s; }
   // Hierarch::resize_to: lays out an h x w matrix along the steps.
   void resize_to(size_t h, size_t w, const std::vector<AlgorithmStep>& desc, int32_t ydim,
                  int32_t xdim, int64_t lh, int64_t lw, int64_t lrs, int64_t lcs) {
@@ -110,31 +113,33 @@ class HierMatrix {
     abi_check("momms_hier_layout_init",
               momms_hier_layout_init(&layout, (int64_t)h, (int64_t)w, steps.data(),
                                      (int64_t)steps.size(), ydim, xdim, lh, lw, lrs, lcs));
-    buf_.get(std::max<size_t>((size_t)layout.capacity * sizeof(double), 256));
+    buf_.get(std::max<size_t>((size_t)layout.capacity * sizeof(T), 256));
   }
 
  private:
   detail::DevBuf buf_;
-  double scalar_ = 1.0;
+  T scalar_ = T(1);
 };
+using HierMatrix = HierMatrixT<double>;
 
 namespace detail {
-inline int32_t step_kind(AlgorithmStep::Kind k) {
-  return k == AlgorithmStep::M ? MOMMS_STEP_M : k == AlgorithmStep::N ? MOMMS_STEP_N : MOMMS_STEP_K;
-}
 // Packs an h x w view (host or device) into P laid out by P.layout.
-template <class M>
-void pack_view(const M& x, size_t h, size_t w, HierMatrix& P, DevBuf& stage, hipStream_t st) {
-  auto X = stage_in<double>(x, h, w, stage, true, st);
-  abi_check("momms_hip_hier_pack",
-            momms_hip_hier_pack(&P.layout, X.ptr, X.rs, X.cs, P.ptr(), st));
+template <class T, class M>
+void pack_view(const M& x, size_t h, size_t w, HierMatrixT<T>& P, DevBuf& stage, hipStream_t st) {
+  auto X = stage_in<T>(x, h, w, stage, true, st);
+  if constexpr (std::is_same<T, double>::value)
+    abi_check("momms_hip_hier_pack",
+              momms_hip_hier_pack(&P.layout, X.ptr, X.rs, X.cs, P.ptr(), st));
+  else
+    abi_check("momms_hip_hier_pack_f32",
+              momms_hip_hier_pack_f32(&P.layout, X.ptr, X.rs, X.cs, P.ptr(), st));
 }
 }  // namespace detail
 
-template <size_t LH, size_t LW, size_t LRS, size_t LCS, class Child>
+template <size_t LH, size_t LW, size_t LRS, size_t LCS, class Child, class T = double>
 struct PackA {
   Child child;
-  HierMatrix a_pack;
+  HierMatrixT<T> a_pack;
   detail::DevBuf stage;
   hipStream_t stream = nullptr;
   template <class At, class Bt, class Ct>
@@ -142,7 +147,7 @@ struct PackA {
     if (thr.thread_id() == 0) {
       a_pack.resize_to(a.height(), a.width(), Child::hierarchy_description(), MOMMS_STEP_M,
                        MOMMS_STEP_K, LH, LW, LRS, LCS);
-      detail::pack_view(a, a.height(), a.width(), a_pack, stage, stream);
+      detail::pack_view<T>(a, a.height(), a.width(), a_pack, stage, stream);
       a_pack.set_scalar(a.get_scalar());
     }
     if (thr.num_threads() > 1) thr.barrier();
@@ -151,10 +156,10 @@ struct PackA {
   static std::vector<AlgorithmStep> hierarchy_description() { return Child::hierarchy_description(); }
 };
 
-template <size_t LH, size_t LW, size_t LRS, size_t LCS, class Child>
+template <size_t LH, size_t LW, size_t LRS, size_t LCS, class Child, class T = double>
 struct PackB {
   Child child;
-  HierMatrix b_pack;
+  HierMatrixT<T> b_pack;
   detail::DevBuf stage;
   hipStream_t stream = nullptr;
   template <class At, class Bt, class Ct>
@@ -162,7 +167,7 @@ struct PackB {
     if (thr.thread_id() == 0) {
       b_pack.resize_to(b.height(), b.width(), Child::hierarchy_description(), MOMMS_STEP_K,
                        MOMMS_STEP_N, LH, LW, LRS, LCS);
-      detail::pack_view(b, b.height(), b.width(), b_pack, stage, stream);
+      detail::pack_view<T>(b, b.height(), b.width(), b_pack, stage, stream);
       b_pack.set_scalar(b.get_scalar());
     }
     if (thr.num_threads() > 1) thr.barrier();
@@ -223,6 +228,50 @@ struct HipGemmHier {
   }
   static std::vector<AlgorithmStep> hierarchy_description() {
     return Desc::hierarchy_description();
+  }
+};
+
+// The KNM f32 tree's micro-kernel loops on the GPU: PartN<Nr, PartM<Mr,
+// KnmKernel>> of src/exper/knm.rs:89-97 over the packed A (Hierarch<f32,
+// Mr, Kr, 1, Mr>) and B (Hierarch<f32, Kr, Nr, 1, Kr>) of its PackA / PackB
+// nodes, C a plain matrix. Its hierarchy description is that subtree's,
+// [K 4 (knm_kernel.rs:33-37), M Mr, N Nr], so the pack nodes above lay the
+// operands out as the reference does. The packed operands are copied out
+// column-major (momms_hip_hier_unpack_f32) and multiplied on FP32 matrix
+// cores (momms_hip_sgemm).
+template <size_t Mr, size_t Nr>
+struct KnmBlock {
+  hipStream_t stream = nullptr;
+  detail::DevBuf da, db, dc;
+  template <class Ct>
+  void run(HierMatrixT<float>& a, HierMatrixT<float>& b, Ct& c, const ThreadInfo& thr) {
+    if (thr.thread_id() == 0) launch(a, b, c);
+    if (thr.num_threads() > 1) thr.barrier();
+  }
+  template <class Ct>
+  void launch(HierMatrixT<float>& a, HierMatrixT<float>& b, Ct& c) {
+    const size_t m = c.height(), n = c.width(), k = a.width();
+    if (m == 0 || n == 0) return;
+    const float alpha = a.get_scalar() * b.get_scalar();
+    const float beta = c.get_scalar();
+    float* A = static_cast<float*>(da.get(std::max<size_t>(m * k * 4, 256)));
+    float* B = static_cast<float*>(db.get(std::max<size_t>(k * n * 4, 256)));
+    abi_check("momms_hip_hier_unpack_f32",
+              momms_hip_hier_unpack_f32(&a.layout, a.ptr(), A, 1, (int64_t)std::max<size_t>(m, 1),
+                                        stream));
+    abi_check("momms_hip_hier_unpack_f32",
+              momms_hip_hier_unpack_f32(&b.layout, b.ptr(), B, 1, (int64_t)std::max<size_t>(k, 1),
+                                        stream));
+    auto C = detail::stage_in<float>(c, m, n, dc, beta != 0.0f, stream);
+    abi_check("momms_hip_sgemm",
+              momms_hip_sgemm((int64_t)m, (int64_t)n, (int64_t)k, &alpha, A, 1,
+                              (int64_t)std::max<size_t>(m, 1), B, 1,
+                              (int64_t)std::max<size_t>(k, 1), &beta, C.ptr, C.rs, C.cs, stream));
+    detail::stage_out<float>(c, m, n, C.ptr, stream);
+    hip_check("hipStreamSynchronize", hipStreamSynchronize(stream));
+  }
+  static std::vector<AlgorithmStep> hierarchy_description() {
+    return {{AlgorithmStep::K, 4}, {AlgorithmStep::M, Mr}, {AlgorithmStep::N, Nr}};
   }
 };
 

@@ -14,6 +14,8 @@
 //   SpawnThreads<PartM<PartN<Barrier<GpuSchedule>>>>  out of core: host blocks
 //                                            streamed through the GPU
 //   UnpackC<PackB<PackA<HipGemmHier>>>       device Hierarch operands
+//   PartN<PartK<PackB<PartM<PackA<KnmBlock>>>>> the KNM f32 tree
+//   HipXsmm, PartM<PartN<PartK<HipXsmm>>>    the small-GEMM leaf
 // Tolerance: |C - C_ref| <= 2*gamma_{k+1}*(|A||B| + |C0|) componentwise, with
 // the unit roundoff of T (TripleLoop in T is the reference), plus the
 // randomized residual of src/util.rs:55-85.
@@ -48,6 +50,13 @@ using OutOfCore = SpawnThreads<PartM<700, PartN<700, Barrier<GpuSchedule<
 // A, B and C packed into device Hierarch buffers laid out by the tree
 using GotoHier = UnpackC<4, 12, 12, 1, PackB<192, 12, 12, 1, PackA<4, 192, 1, 4,
                  HipGemmHier<GotoDesc>>>>;
+// the KNM sgemm tree (knm.rs:79-97: NC 14400, KC 336, MC 160, MR 16, NR 24,
+// KR 4): f32 operands packed into device Hierarch buffers per block, the
+// micro-kernel loops of each block on FP32 matrix cores
+using KnmTree = PartN<14400, PartK<336, PackB<4, 24, 1, 4, PartM<160, PackA<16, 4, 1, 16,
+                KnmBlock<16, 24>, float>>, float>>>;
+// small blocks straight from the operands (the libxsmm leaves)
+using XsmmTree = PartM<96, PartN<64, PartK<128, HipXsmm>>>;
 
 template <class Algo, class T = double>
 static void check_tree(const char* name, size_t m, size_t n, size_t k, size_t threads,
@@ -132,6 +141,10 @@ int main() {
   check_tree<GotoHier>("UnpackC<PackB<PackA<HipGemmHier<Goto>>>>", 700, 500, 450, 1, 2.0, false);
   check_tree<SpawnThreads<GotoHier>>("SpawnThreads<UnpackC<...HipGemmHier>> x3", 300, 260, 200, 3,
                                      2.0, false);
+  check_tree<KnmTree, float>("KNM f32 tree (PackB/PackA Hierarch<f32>)", 500, 420, 700, 1, 2.0,
+                             false);
+  check_tree<HipXsmm>("HipXsmm (device matrices)", 90, 70, 50, 1, 2.0, true);
+  check_tree<XsmmTree>("PartM<96,PartN<64,PartK<128,HipXsmm>>>", 300, 200, 260, 1, 2.0, true);
   std::printf("%s (%d failures)\n", fails ? "FAIL" : "PASS", fails);
   return fails ? 1 : 0;
 }

@@ -258,3 +258,27 @@ def test_tree_order_on_row_major_operands_and_an_empty_tree():
     capi.dgemm(A2, B2, C4, 1.0, 1.0, tile_cfg=0)
     torch.cuda.synchronize()
     assert torch.equal(C3, C4)
+
+
+@pytest.mark.parametrize("m,n,k", [(1, 1, 1), (32, 32, 32), (70, 33, 5), (96, 64, 128),
+                                   (129, 65, 300), (300, 200, 260), (17, 900, 41)])
+@pytest.mark.parametrize("layout", ["col", "row", "gen"])
+def test_small_gemm_leaf(m, n, k, layout):
+    """momms_hip_dgemm_small (the libxsmm-leaf counterpart): column-major and
+    row-major operands take the single-launch wavefront tiles, general strides
+    the ordinary path; within gamma_{k+1} of double-double either way, and
+    beta = 0 never reads C."""
+    rng = np.random.default_rng(SEED + m + 5 * n + 11 * k + len(layout))
+    A = layout_view(rng, m, k, layout)
+    B = layout_view(rng, k, n, layout)
+    C = layout_view(rng, m, n, layout)
+    C0 = C.clone()
+    capi.dgemm_small(A, B, C, -0.75, 1.25)
+    torch.cuda.synchronize()
+    check_against_oracle(A, B, C0, C, -0.75, 1.25, k)
+    Cz = layout_view(rng, m, n, layout)
+    Cz.fill_(float("nan"))
+    capi.dgemm_small(A, B, Cz, 1.0, 0.0)
+    torch.cuda.synchronize()
+    assert torch.isfinite(Cz).all()
+    check_against_oracle(A, B, torch.zeros_like(Cz), Cz, 1.0, 0.0, k)

@@ -152,3 +152,33 @@ def test_dgemm_hier_goto_8192_matches_dgemm():
     back = torch.zeros((n, n), dtype=torch.float64, device="cuda").T
     capi.hier_unpack_add(Ls[2], Ps[2], back)
     assert torch.equal(back, X[2])
+
+
+@pytest.mark.parametrize("name", sorted(FORMS))
+@pytest.mark.parametrize("layout", ["col", "row", "strided"])
+def test_f32_pack_unpack_match_oracle(name, layout):
+    """The float32 forms (Hierarch<f32>, the KNM tree's operands): the same
+    places as the oracle's layout, values moved bit for bit; the copy form of
+    the unpack gives the source back."""
+    _, steps, yd, xd, lh, lw, lrs, lcs = FORMS[name]
+    h, w = 203, 157
+    L = capi.hier_layout(h, w, steps, yd, xd, lh, lw, lrs, lcs)
+    rng = np.random.default_rng(h + w + len(layout))
+    A = layout_view(rng, h, w, layout).float()
+    if layout == "row":
+        A = A.contiguous()
+    elif layout == "col":
+        A = A.T.contiguous().T
+    P = torch.full((L.capacity,), float("nan"), dtype=torch.float32, device="cuda")
+    capi.hier_pack(L, A, P)
+    C = torch.rand((w, h), dtype=torch.float32, device="cuda").T
+    C0 = C.clone()
+    capi.hier_unpack_add(L, P, C)
+    R = torch.empty((w, h), dtype=torch.float32, device="cuda").T
+    capi.hier_unpack_f32(L, P, R)
+    torch.cuda.synchronize()
+    OL = oracle_layout(L)
+    want_p = po.hier_pack(OL, host(A).astype(np.float64)).astype(np.float32)
+    np.testing.assert_array_equal(P.cpu().numpy(), want_p)
+    assert torch.equal(R, A)
+    assert torch.equal(C, C0 + A)
