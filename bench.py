@@ -403,7 +403,8 @@ def main() -> int:
     ap.add_argument("--quick", action="store_true", help="headline only, no other configs")
     ap.add_argument("--tile", type=int, default=None, help="FP64 tile configuration (--only plain)")
     ap.add_argument("--moduli", type=int, default=14, help="moduli of the residue form (--only crt)")
-    ap.add_argument("--only", choices=["plain", "split", "multiply", "crt", "crt_i8", "batched",
+    ap.add_argument("--only", choices=["plain", "plain_sweep", "split", "multiply", "crt", "crt_i8",
+                                       "batched",
                                        "batched_plain", "copy", "hier", "fp32"])
     args = ap.parse_args()
     if args.gpus != 1:
@@ -437,6 +438,12 @@ def main() -> int:
         ms = time_loop(lambda: capi.dgemm(A, B, C, 1.0, 1.0, tile_cfg=args.tile), args.steps,
                        args.warmup)
         print(json.dumps({"only": "plain", "tile": args.tile, "ms": ms, "tflops": flops / ms / 1e9}))
+        return 0
+    if args.only == "plain_sweep":
+        # the plain FP64 sweep shapes at the built-in tile choice
+        del A, B, C0, C
+        print(json.dumps({"only": "plain_sweep",
+                          "sweep": sweep_plain(capi, torch, gen, args.steps)}))
         return 0
     if args.only in ("split", "multiply"):
         ws = capi.SplitWorkspace(n, n, n, s)
