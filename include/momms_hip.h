@@ -253,7 +253,10 @@ int momms_hip_batch_plan_create(const momms_gemm_desc* descs, int64_t count, int
                                 momms_batch_plan** out, void* stream);
 
 /* Enqueues the whole batch on `stream` (reset kernel, slicing in mode 1,
- * persistent multiply kernels). */
+ * persistent multiply kernels). Every run of a plan resets and uses the
+ * plan's own device queue counters and workspace, so runs and graph launches
+ * of one plan must be ordered on one stream; overlapping ones would skip or
+ * repeat tiles. */
 int momms_hip_batch_run(momms_batch_plan* plan, const double* alpha, const double* A,
                         const double* B, const double* beta, double* C, void* stream);
 
