@@ -243,7 +243,13 @@ typedef struct momms_batch_plan momms_batch_plan;
 /* Builds a plan for `count` problems (descs is a HOST array, copied).
  * mode 0: plain FP64 matrix cores; mode 1: split-BF16 FP64-accurate with
  * `slices` slices (0 = chosen by the autotuner: the smallest count whose
- * sampled error is within 2x plain FP64's, 7 without autotuning).
+ * sampled error is within 2x plain FP64's, 7 without autotuning);
+ * mode 2: the BF16 residue form with `slices` moduli (0 = chosen the same
+ * way, 13 without autotuning; at least the count momms_hip_crt_bf16_beta
+ * needs for the largest k of the batch), every problem's result bitwise
+ * equal to momms_hip_dgemm_crt_bf16 with that many moduli, in one fused
+ * launch (residues, products and reconstruction on chip per tile) after an
+ * exponent pre-pass.
  * autotune != 0 times the candidate tile shapes per size bucket on this
  * batch (reading A and B, writing only plan-owned scratch) and remembers the
  * choice for the process. A and B may be NULL when autotune == 0.
@@ -273,6 +279,53 @@ int momms_hip_batch_launch(momms_batch_plan* plan, void* stream);
 const char* momms_hip_batch_plan_describe(const momms_batch_plan* plan);
 
 int momms_hip_batch_plan_destroy(momms_batch_plan* plan);
+
+/* ---- the autotuner's table ---------------------------------------------------
+ * No reference counterpart: the reference fixes its blocking at compile time
+ * (Cargo features, Cargo.toml:59-66, and typenum block sizes,
+ * src/exper/l3.rs:78-93); the batched planner chooses at run time and keeps
+ * its choices in a per-process table, one entry per (kind, batch class).
+ * Exporting the table and importing it into a later process lets that
+ * process plan without timing candidates again; an imported entry is used by
+ * momms_hip_batch_plan_create with or without autotune.
+ *
+ * The class of a batch (momms_hip_batch_class) is
+ * 16 * min(15, floor(log2(1 + P))) + min(8, floor(log2(1 + K))), P being the
+ * number of problems with m, n > 0 and K their mean k.
+ *
+ * Entry values by kind:
+ *   MOMMS_TUNE_BATCH_FP64:  {T, T2, 0, 0}: problems whose smaller side is at
+ *     least T2 take 128 x 128 tiles, at least T 64 x 64 tiles, the others
+ *     32 x 32 one-wavefront tiles; 16 <= T <= T2 <= 2^30 (2^30: none).
+ *   MOMMS_TUNE_BATCH_SPLIT: {slices, 0, 0, 0}, 1 <= slices <= 8.
+ *   MOMMS_TUNE_BATCH_CRT:   {moduli, 0, 0, 0}, within the residue form's range.
+ * A table whose version differs from MOMMS_TUNING_VERSION, with count above
+ * MOMMS_TUNING_MAX, an unknown kind, a key outside 0..255 or a value outside
+ * its range is refused as a whole (-1, nothing imported). */
+#define MOMMS_TUNING_VERSION 1u
+#define MOMMS_TUNING_MAX 256
+enum { MOMMS_TUNE_BATCH_FP64 = 1, MOMMS_TUNE_BATCH_SPLIT = 2, MOMMS_TUNE_BATCH_CRT = 3 };
+
+typedef struct momms_tuning_entry {
+  int32_t kind, key;
+  int32_t value[4];
+} momms_tuning_entry;
+
+typedef struct momms_tuning {
+  uint32_t version; /* MOMMS_TUNING_VERSION */
+  uint32_t count;   /* entries in use */
+  momms_tuning_entry entry[MOMMS_TUNING_MAX];
+} momms_tuning;
+
+/* The class key of a batch table (host array), for entries written by hand. */
+int momms_hip_batch_class(const momms_gemm_desc* descs, int64_t count, int32_t* key);
+
+/* Copies the process's table into *out (sorted by kind, then key). */
+int momms_hip_tuning_export(momms_tuning* out);
+
+/* Adds the entries of *in to the process's table, an entry replacing one of
+ * the same kind and key; replace != 0 empties the table first. */
+int momms_hip_tuning_import(const momms_tuning* in, int replace);
 
 /* The comparison baseline: one momms_hip_dgemm launch per problem. */
 int momms_hip_dgemm_batched_loop(const momms_gemm_desc* descs, int64_t count,

@@ -53,6 +53,9 @@ _SIGS: dict[str, list] = {
     "momms_hip_dgemm_noalloc": [_c_i64, _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_vp,
                                 _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_vp],
     "momms_last_error_string": [],
+    "momms_hip_batch_class": [_c_vp, _c_i64, _c_vp],
+    "momms_hip_tuning_export": [_c_vp],
+    "momms_hip_tuning_import": [_c_vp, _c_int],
     "momms_hip_split_bf16_workspace_size": [_c_i64, _c_i64, _c_i64, _c_int],
     "momms_hip_dgemm_split_bf16": [_c_i64, _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_vp,
                                    _c_i64, _c_i64, _c_dp, _c_vp, _c_i64, _c_i64, _c_int, _c_vp,
@@ -358,6 +361,75 @@ def dgemm_batched_loop(rows_or_descs, A, B, C, alpha=1.0, beta=1.0, stream=None)
                                             A.data_ptr(), B.data_ptr(), ctypes.byref(be),
                                             C.data_ptr(), stream_handle(stream))
     check("momms_hip_dgemm_batched_loop", st)
+
+
+TUNING_VERSION = 1
+TUNING_MAX = 256
+TUNE_KINDS = {"batch_fp64": 1, "batch_split": 2, "batch_crt": 3}
+
+
+class _TuningEntry(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int32), ("key", ctypes.c_int32), ("value", ctypes.c_int32 * 4)]
+
+
+class _Tuning(ctypes.Structure):
+    """momms_tuning (include/momms_hip.h)."""
+    _fields_ = [("version", ctypes.c_uint32), ("count", ctypes.c_uint32),
+                ("entry", _TuningEntry * TUNING_MAX)]
+
+
+def batch_class(rows_or_descs) -> int:
+    """The class key the batched planner files a batch table's tuned choices under."""
+    d = rows_or_descs if isinstance(rows_or_descs, ctypes.Array) else desc_array(rows_or_descs)
+    key = ctypes.c_int32()
+    check("momms_hip_batch_class",
+          lib().momms_hip_batch_class(ctypes.cast(d, _c_vp), len(d), ctypes.byref(key)))
+    return key.value
+
+
+def tuning_export() -> dict:
+    """The process's tuning table: {"version": 1, "entries": [{"kind", "key", "value"}]},
+    kinds by name (TUNE_KINDS)."""
+    t = _Tuning()
+    check("momms_hip_tuning_export", lib().momms_hip_tuning_export(ctypes.byref(t)))
+    names = {v: k for k, v in TUNE_KINDS.items()}
+    return {"version": t.version,
+            "entries": [{"kind": names[t.entry[i].kind], "key": t.entry[i].key,
+                         "value": list(t.entry[i].value)} for i in range(t.count)]}
+
+
+def tuning_import(table: dict, replace: bool = False) -> None:
+    """Loads a table of the form tuning_export returns (the library checks every entry)."""
+    entries = table.get("entries", [])
+    if len(entries) > TUNING_MAX:
+        raise ValueError(f"tuning table has {len(entries)} entries, at most {TUNING_MAX}")
+    t = _Tuning()
+    t.version = int(table.get("version", -1)) & 0xFFFFFFFF
+    t.count = len(entries)
+    for i, e in enumerate(entries):
+        kind = e["kind"]
+        t.entry[i].kind = TUNE_KINDS[kind] if isinstance(kind, str) else int(kind)
+        t.entry[i].key = int(e["key"])
+        vals = list(e["value"]) + [0] * (4 - len(e["value"]))
+        if len(vals) != 4:
+            raise ValueError(f"entry {i} has {len(e['value'])} values, at most 4")
+        for j in range(4):
+            t.entry[i].value[j] = int(vals[j])
+    check("momms_hip_tuning_import", lib().momms_hip_tuning_import(ctypes.byref(t), int(bool(replace))))
+
+
+def save_tuning(path) -> None:
+    """Writes the process's tuning table to `path` as JSON."""
+    import json
+
+    Path(path).write_text(json.dumps(tuning_export(), indent=1) + "\n")
+
+
+def load_tuning(path, replace: bool = False) -> None:
+    """Imports a tuning table saved by save_tuning."""
+    import json
+
+    tuning_import(json.loads(Path(path).read_text()), replace)
 
 
 def uneven_batch(count: int, lo: int = 16, hi: int = 512, seed: int = 0x6D6F6D6D73):

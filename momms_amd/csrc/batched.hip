@@ -19,6 +19,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <cstdio>
 #include <cstring>
@@ -29,6 +30,8 @@
 
 #include "batch_tiles.hpp"
 #include "batched.hpp"
+#include "crt_bf16.hpp"
+#include "crt_recon.hpp"
 #include "dgemm.hpp"
 #include "dgemm_tile.hpp"
 #include "mfma.hpp"
@@ -39,12 +42,6 @@
 
 namespace momms {
 
-
-// Per-problem workspace of the split mode (byte offsets into the arena).
-struct SplitProb {
-  int64_t aq, bq, ea, eb;  // byte offsets
-  int32_t mpad, npad, kpad, pad;
-};
 
 // Slicing task: 64 vectors (rows of A: kind 0, columns of B: kind 1) of one
 // problem, and for the slice pass one 64-wide block of k.
@@ -210,11 +207,54 @@ static const int kBigThresholds[] = {128, 192, 256, 1 << 30};
 constexpr int kDefaultThreshold = 48;
 constexpr int kDefaultBigThreshold = 256;
 
-// Tuned thresholds, remembered for the process (keyed by a coarse class of
+// The autotuner's choices, remembered for the process per (kind, class of
 // the batch: problem count and mean k), so plans for similar batches reuse
-// them.
+// them; exported and imported as a momms_tuning table.
 static std::mutex g_tune_mu;
-static std::map<int, std::pair<int, int>> g_tuned;
+static std::map<std::pair<int, int>, std::array<int32_t, 4>> g_tuned;
+
+static bool tuned_lookup(int kind, int key, std::array<int32_t, 4>& v) {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  auto it = g_tuned.find({kind, key});
+  if (it == g_tuned.end()) return false;
+  v = it->second;
+  return true;
+}
+
+static void tuned_store(int kind, int key, std::array<int32_t, 4> v) {
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  g_tuned[{kind, key}] = v;
+}
+
+// The class key of a batch: 16 * min(15, floor(log2(1 + P))) +
+// min(8, floor(log2(1 + K))) for P problems with m, n > 0 of mean k K.
+static int batch_class(const momms_gemm_desc* d, int64_t count) {
+  int64_t ksum = 0, live = 0;
+  for (int64_t i = 0; i < count; ++i)
+    if (d[i].m && d[i].n) {
+      ksum += d[i].k;
+      ++live;
+    }
+  const int kcls = live ? (int)std::min<int64_t>(8, (int64_t)std::log2(1.0 + (double)ksum / live)) : 0;
+  const int ccls = (int)std::min<int64_t>(15, (int64_t)std::log2(1.0 + (double)live));
+  return ccls * 16 + kcls;
+}
+
+// Whether a table entry's values are in range for its kind.
+static bool tuning_entry_ok(const momms_tuning_entry& t) {
+  if (t.key < 0 || t.key > 255) return false;
+  const int32_t* v = t.value;
+  switch (t.kind) {
+    case MOMMS_TUNE_BATCH_FP64:
+      return v[0] >= 16 && v[0] <= v[1] && v[1] <= (1 << 30) && v[2] == 0 && v[3] == 0;
+    case MOMMS_TUNE_BATCH_SPLIT:
+      return v[0] >= 1 && v[0] <= kMaxSlices && v[1] == 0 && v[2] == 0 && v[3] == 0;
+    case MOMMS_TUNE_BATCH_CRT:
+      return v[0] >= 1 && v[0] <= kCrtMaxModuli && v[1] == 0 && v[2] == 0 && v[3] == 0;
+    default:
+      return false;
+  }
+}
 
 static int occupancy(const void* kernel, int threads, int cus) {
   int per = 0;
@@ -260,6 +300,11 @@ struct momms_batch_plan {
   momms::SliceTask* d_slice_tasks = nullptr;
   int n_slice_tasks = 0;
   momms::Launchable mul{};
+  // residue mode (and the exponent tables above)
+  int moduli = 0;
+  momms::Launchable crt{};
+  momms::QueueBounds qb_crt{};
+  int beta_min = 0, beta_max = 0;
   // graph
   hipStream_t cap_stream = nullptr;
   hipGraph_t graph = nullptr;
@@ -435,7 +480,7 @@ static hipError_t enqueue(const momms_batch_plan* P, double alpha, const double*
                           const double* B, double beta, double* C, hipStream_t st) {
   // plain mode: counters 0..7 the queues, 8 the wide problems, 9..16 the
   // queues of 128 x 128 tiles
-  const int nctr = P->mode == 0 ? 2 * kBatchQueues + 1 : 1;
+  const int nctr = P->mode == 0 ? 2 * kBatchQueues + 1 : (P->mode == 2 ? kBatchQueues : 1);
   hipLaunchKernelGGL(reset_counters, dim3(1), dim3(64), 0, st, P->d_counters, nctr);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return e;
@@ -450,6 +495,15 @@ static hipError_t enqueue(const momms_batch_plan* P, double alpha, const double*
     if (P->wide.nitems)
       launch_batched_wide(P->wide.grid, st, P->d_desc, P->wide.items, P->wide.nitems,
                           P->d_counters + kBatchQueues, alpha, A, B, beta, C);
+    return hipGetLastError();
+  }
+  if (P->mode == 2) {
+    if (P->n_exp_tasks)
+      hipLaunchKernelGGL(batched_exponents, dim3((unsigned)P->n_exp_tasks), dim3(256), 0, st,
+                         P->d_desc, P->d_sp, P->d_exp_tasks, A, B, P->d_ws);
+    if (P->crt.nitems)
+      launch_batched_crt(P->moduli, P->crt.grid, st, P->d_desc, P->d_sp, P->crt.items, P->qb_crt,
+                         P->d_counters, alpha, A, B, beta, C, P->d_ws);
     return hipGetLastError();
   }
   if (P->n_exp_tasks) {
@@ -490,6 +544,7 @@ static void free_plan(momms_batch_plan* P) {
   (void)hipFree(P->d_ws);
   (void)hipFree(P->d_exp_tasks);
   (void)hipFree(P->d_slice_tasks);
+  (void)hipFree(P->crt.items);
   if (P->exec) (void)hipGraphExecDestroy(P->exec);
   if (P->graph) (void)hipGraphDestroy(P->graph);
   if (P->cap_stream) (void)hipStreamDestroy(P->cap_stream);
@@ -557,6 +612,90 @@ static hipError_t build_split(momms_batch_plan* P, int s) {
   return hipSuccess;
 }
 
+// Builds the residue-mode tables for N moduli (replacing earlier ones):
+// the exponent workspace, beta of every problem and the per-XCD queues of
+// 64 x 64 tiles. Fails (hipErrorInvalidValue) if N moduli cannot hold some
+// problem's k-term products.
+static hipError_t build_crt(momms_batch_plan* P, int N) {
+  (void)hipFree(P->d_sp);
+  (void)hipFree(P->d_ws);
+  (void)hipFree(P->d_exp_tasks);
+  (void)hipFree(P->crt.items);
+  P->d_sp = nullptr;
+  P->d_ws = nullptr;
+  P->d_exp_tasks = nullptr;
+  P->crt = Launchable{};
+  P->moduli = N;
+  const auto& d = P->descs;
+  P->sp.assign(d.size(), SplitProb{});
+  size_t off = 0;
+  auto al = [](size_t x) { return (x + 255) & ~size_t(255); };
+  std::vector<SliceTask> et;
+  std::vector<int> probs;
+  P->beta_min = 1 << 30;
+  P->beta_max = 0;
+  for (size_t i = 0; i < d.size(); ++i) {
+    SplitProb& q = P->sp[i];
+    q.mpad = (int32_t)(ceil_div(d[i].m, 64) * 64);
+    q.npad = (int32_t)(ceil_div(d[i].n, 64) * 64);
+    q.kpad = (int32_t)ceil_div(d[i].k, 16) * 16;
+    q.ea = (int64_t)off;
+    off = al(off + (size_t)q.mpad * 4);
+    q.eb = (int64_t)off;
+    off = al(off + (size_t)q.npad * 4);
+    if (d[i].m == 0 || d[i].n == 0) continue;
+    q.beta = crt_beta(d[i].k, N);
+    if (q.beta < 1) return hipErrorInvalidValue;
+    P->beta_min = std::min(P->beta_min, (int)q.beta);
+    P->beta_max = std::max(P->beta_max, (int)q.beta);
+    probs.push_back((int)i);
+    for (int kind = 0; kind < 2; ++kind) {
+      const int vb = (kind == 0 ? q.mpad : q.npad) / 64;
+      for (int v = 0; v < vb; ++v) et.push_back({(int32_t)i, kind, v, 0});
+    }
+  }
+  if (probs.empty()) P->beta_min = 0;
+  P->ws_bytes = std::max<size_t>(off, 256);
+  hipError_t e;
+  if ((e = hipMalloc((void**)&P->d_ws, P->ws_bytes)) != hipSuccess) return e;
+  if ((e = upload((void**)&P->d_sp, P->sp.data(), P->sp.size() * sizeof(SplitProb))) != hipSuccess)
+    return e;
+  if ((e = upload((void**)&P->d_exp_tasks, et.data(), et.size() * sizeof(SliceTask))) != hipSuccess)
+    return e;
+  P->n_exp_tasks = (int)et.size();
+  // per queue: its problems' 64 x 64 tiles, by k
+  std::vector<int> qp[kBatchQueues];
+  assign_queues(d, probs, std::vector<int>(probs.size(), 64), qp);
+  std::vector<BatchItem> items;
+  for (int qi = 0; qi < kBatchQueues; ++qi) {
+    std::vector<int> ps;
+    for (int i : qp[qi]) ps.push_back(probs[i]);
+    std::sort(ps.begin(), ps.end());
+    const auto tiles = tile_items(d, ps, 64);
+    P->qb_crt.off[qi] = (int)items.size();
+    items.insert(items.end(), tiles.begin(), tiles.end());
+  }
+  P->qb_crt.off[kBatchQueues] = (int)items.size();
+  P->crt.nitems = (int)items.size();
+  if (!items.empty()) {
+    if ((e = upload((void**)&P->crt.items, items.data(), items.size() * sizeof(BatchItem))) != hipSuccess)
+      return e;
+    P->crt.grid = std::max(1, std::min(P->crt.nitems, occupancy(batched_crt_kernel_ptr(N), 512, P->cus)));
+  }
+  P->total_tiles = P->crt.nitems;
+  return hipSuccess;
+}
+
+// The smallest moduli count that can hold every problem's k-term products.
+static int crt_min_moduli(const std::vector<momms_gemm_desc>& d) {
+  int64_t kmax = 0;
+  for (const auto& p : d)
+    if (p.m && p.n) kmax = std::max(kmax, p.k);
+  for (int N = 1; N <= kCrtMaxModuli; ++N)
+    if (crt_beta(kmax, N) >= 1) return N;
+  return kCrtMaxModuli + 1;
+}
+
 static float time_enqueue(momms_batch_plan* P, const double* A, const double* B, double* C,
                           hipStream_t st, int reps) {
   hipEvent_t e0, e1;
@@ -595,9 +734,12 @@ extern "C" int momms_hip_batch_plan_create(const momms_gemm_desc* descs, int64_t
   static const char* fn = "momms_hip_batch_plan_create";
   if (descs == nullptr && count > 0) return bad_arg(fn, 1, "descs is null");
   if (count < 0 || count > (1 << 24)) return bad_arg(fn, 2, "count out of range");
-  if (mode != 0 && mode != 1) return bad_arg(fn, 3, "mode must be 0 (FP64) or 1 (split-BF16)");
+  if (mode < 0 || mode > 2)
+    return bad_arg(fn, 3, "mode must be 0 (FP64), 1 (split-BF16 slices) or 2 (BF16 residues)");
   if (mode == 1 && (slices < 0 || slices > kMaxSlices))
     return bad_arg(fn, 4, "slices must be 0 (tuned) or 1..8");
+  if (mode == 2 && (slices < 0 || slices > kCrtMaxModuli))
+    return bad_arg(fn, 4, "moduli must be 0 (tuned) or 1..16");
   if (autotune && (A == nullptr || B == nullptr)) return bad_arg(fn, 6, "autotune needs A and B");
   if (out == nullptr) return bad_arg(fn, 8, "out is null");
   for (int64_t i = 0; i < count; ++i) {
@@ -637,25 +779,15 @@ extern "C" int momms_hip_batch_plan_create(const momms_gemm_desc* descs, int64_t
 
   char line[384];
   if (mode == 0) {
-    int64_t ksum = 0, live = 0;
-    for (auto& p : P->descs)
-      if (p.m && p.n) {
-        ksum += p.k;
-        ++live;
-      }
-    const int kcls = live ? (int)std::min<int64_t>(8, (int64_t)std::log2(1.0 + (double)ksum / live)) : 0;
-    const int ccls = (int)std::min<int64_t>(15, (int64_t)std::log2(1.0 + (double)live));
-    const int key = ccls * 16 + kcls;
+    int64_t live = 0;
+    for (auto& p : P->descs) live += (p.m && p.n) ? 1 : 0;
+    const int key = batch_class(P->descs.data(), count);
     int T = kDefaultThreshold, T2 = kDefaultBigThreshold;
-    bool known = false;
-    {
-      std::lock_guard<std::mutex> g(g_tune_mu);
-      auto it = g_tuned.find(key);
-      if (it != g_tuned.end()) {
-        T = it->second.first;
-        T2 = it->second.second;
-        known = true;
-      }
+    std::array<int32_t, 4> tv{};
+    const bool known = tuned_lookup(MOMMS_TUNE_BATCH_FP64, key, tv);
+    if (known) {
+      T = tv[0];
+      T2 = tv[1];
     }
     if (!known && autotune && live) {
       float best = 1e30f;
@@ -674,8 +806,7 @@ extern "C" int momms_hip_batch_plan_create(const momms_gemm_desc* descs, int64_t
             T2 = cand2;
           }
         }
-      std::lock_guard<std::mutex> g(g_tune_mu);
-      g_tuned[key] = {T, T2};
+      tuned_store(MOMMS_TUNE_BATCH_FP64, key, {T, T2, 0, 0});
     }
     if ((e = build_plain(P, T, T2)) != hipSuccess) {
       (void)hipFree(scratchC);
@@ -691,10 +822,28 @@ extern "C" int momms_hip_batch_plan_create(const momms_gemm_desc* descs, int64_t
                   P->big.grid);
     P->info = line;
   } else {
+    // mode 1: slice count; mode 2: moduli count (the `slices` argument)
+    const bool crt = mode == 2;
+    const int kind = crt ? MOMMS_TUNE_BATCH_CRT : MOMMS_TUNE_BATCH_SPLIT;
+    const int cmin = crt ? crt_min_moduli(P->descs) : 1;
+    if (cmin > kCrtMaxModuli) {
+      (void)hipFree(scratchC);
+      free_plan(P);
+      return bad_arg(fn, 1, "a problem's k is too large for 16 moduli");
+    }
+    if (crt && slices != 0 && slices < cmin) {
+      (void)hipFree(scratchC);
+      free_plan(P);
+      return bad_arg(fn, 4, "too few moduli for the largest k of the batch");
+    }
+    auto build = [&](int c) { return crt ? build_crt(P, c) : build_split(P, c); };
     int s = slices;
+    const int key = batch_class(P->descs.data(), count);
+    std::array<int32_t, 4> tv{};
+    if (s == 0 && tuned_lookup(kind, key, tv) && tv[0] >= cmin) s = tv[0];
     if (s == 0 && autotune) {
-      // smallest slice count whose sampled error is within 2x plain FP64's
-      if ((e = build_split(P, 8)) != hipSuccess) MOMMS_FAIL(e);
+      // the smallest count whose sampled error is within 2x plain FP64's
+      const int first = crt ? std::max(cmin, 8) : 4, last = crt ? kCrtMaxModuli : kMaxSlices;
       momms_batch_plan T;  // the plain FP64 batch, for the reference error
       T.mode = 0;
       T.cus = P->cus;
@@ -725,9 +874,9 @@ extern "C" int momms_hip_batch_plan_create(const momms_gemm_desc* descs, int64_t
         return worst;
       };
       if (e == hipSuccess) plain_err = max_err(plainC);
-      s = 8;
-      for (int cand = 4; cand <= 8 && e == hipSuccess; ++cand) {
-        if ((e = build_split(P, cand)) != hipSuccess) break;
+      s = last;
+      for (int cand = first; cand <= last && e == hipSuccess; ++cand) {
+        if ((e = build(cand)) != hipSuccess) break;
         if ((e = enqueue(P, 1.0, A, B, 0.0, scratchC, st)) != hipSuccess) break;
         if ((e = hipStreamSynchronize(st)) != hipSuccess) break;
         if (max_err(scratchC) <= 2.0 * plain_err) {
@@ -750,17 +899,25 @@ extern "C" int momms_hip_batch_plan_create(const momms_gemm_desc* descs, int64_t
         (void)hipFree(scratchC);
         MOMMS_FAIL(e);
       }
+      tuned_store(kind, key, {s, 0, 0, 0});
     } else if (s == 0) {
-      s = 7;
+      s = crt ? std::max(cmin, 13) : 7;
     }
-    if ((e = build_split(P, s)) != hipSuccess) {
+    if ((e = build(s)) != hipSuccess) {
       (void)hipFree(scratchC);
       MOMMS_FAIL(e);
     }
-    P->total_tiles = P->mul.nitems;
-    std::snprintf(line, sizeof(line),
-                  "{\"mode\":\"split_bf16\",\"slices\":%d,\"tile\":%d,\"tiles\":%d,\"workspace_bytes\":%zu}",
-                  s, kBatchPanel, P->mul.nitems, P->ws_bytes);
+    if (crt) {
+      std::snprintf(line, sizeof(line),
+                    "{\"mode\":\"crt_bf16\",\"moduli\":%d,\"tile\":64,\"tiles\":%d,\"beta_min\":%d,"
+                    "\"beta_max\":%d,\"queues\":%d,\"workgroups\":%d,\"workspace_bytes\":%zu}",
+                    s, P->crt.nitems, P->beta_min, P->beta_max, kBatchQueues, P->crt.grid, P->ws_bytes);
+    } else {
+      P->total_tiles = P->mul.nitems;
+      std::snprintf(line, sizeof(line),
+                    "{\"mode\":\"split_bf16\",\"slices\":%d,\"tile\":%d,\"tiles\":%d,\"workspace_bytes\":%zu}",
+                    s, kBatchPanel, P->mul.nitems, P->ws_bytes);
+    }
     P->info = line;
   }
   (void)hipFree(scratchC);
@@ -847,6 +1004,51 @@ extern "C" int momms_hip_dgemm_batched_loop(const momms_gemm_desc* descs, int64_
     const int st = momms_hip_dgemm(p.m, p.n, p.k, alpha, A + p.a_off, 1, p.lda, B + p.b_off, 1,
                                    p.ldb, beta, C + p.c_off, 1, p.ldc, stream);
     if (st != 0) return st;
+  }
+  return 0;
+}
+
+extern "C" int momms_hip_batch_class(const momms_gemm_desc* descs, int64_t count, int32_t* key) {
+  static const char* fn = "momms_hip_batch_class";
+  if (descs == nullptr && count > 0) return bad_arg(fn, 1, "descs is null");
+  if (count < 0) return bad_arg(fn, 2, "count < 0");
+  if (key == nullptr) return bad_arg(fn, 3, "key is null");
+  *key = batch_class(descs, count);
+  return 0;
+}
+
+extern "C" int momms_hip_tuning_export(momms_tuning* out) {
+  static const char* fn = "momms_hip_tuning_export";
+  if (out == nullptr) return bad_arg(fn, 1, "out is null");
+  std::memset(out, 0, sizeof(*out));
+  out->version = MOMMS_TUNING_VERSION;
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  for (const auto& kv : g_tuned) {
+    if (out->count == MOMMS_TUNING_MAX) break;
+    momms_tuning_entry& t = out->entry[out->count++];
+    t.kind = kv.first.first;
+    t.key = kv.first.second;
+    for (int i = 0; i < 4; ++i) t.value[i] = kv.second[i];
+  }
+  return 0;
+}
+
+extern "C" int momms_hip_tuning_import(const momms_tuning* in, int replace) {
+  static const char* fn = "momms_hip_tuning_import";
+  if (in == nullptr) return bad_arg(fn, 1, "in is null");
+  if (in->version != MOMMS_TUNING_VERSION) return bad_arg(fn, 1, "table version differs from MOMMS_TUNING_VERSION");
+  if (in->count > MOMMS_TUNING_MAX) return bad_arg(fn, 1, "count exceeds MOMMS_TUNING_MAX");
+  for (uint32_t i = 0; i < in->count; ++i)
+    if (!tuning_entry_ok(in->entry[i])) {
+      char buf[96];
+      std::snprintf(buf, sizeof(buf), "entry[%u] has an unknown kind or a value out of range", i);
+      return bad_arg(fn, 1, buf);
+    }
+  std::lock_guard<std::mutex> g(g_tune_mu);
+  if (replace) g_tuned.clear();
+  for (uint32_t i = 0; i < in->count; ++i) {
+    const momms_tuning_entry& t = in->entry[i];
+    g_tuned[{t.kind, t.key}] = {t.value[0], t.value[1], t.value[2], t.value[3]};
   }
   return 0;
 }

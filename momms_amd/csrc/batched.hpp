@@ -11,6 +11,15 @@ struct BatchItem {
   int32_t prob, tm, tn, pad;
 };
 
+// Per-problem workspace of the split and residue modes (byte offsets into
+// the plan's arena): slices of A and B (split mode only), the exponents of
+// the rows of A and columns of B (mpad and npad entries), and in the residue
+// mode the problem's bits beta of the integer images.
+struct SplitProb {
+  int64_t aq, bq, ea, eb;
+  int32_t mpad, npad, kpad, beta;
+};
+
 // A persistent workgroup's loop over one queue of `count` items: body(it)
 // for every item index it claims from `counter`. The claim of the next item
 // (a device-scope atomic, a long round trip) is issued when an item starts
@@ -68,5 +77,13 @@ const void* batched_wide_kernel_ptr();
 void launch_batched_wide(int grid, hipStream_t st, const momms_gemm_desc* d, const BatchItem* items,
                          int nitems, unsigned* counter, double alpha, const double* A,
                          const double* B, double beta, double* C);
+
+// Residue mode (batched_crt.hip): one persistent kernel of 64 x 64 tiles
+// over per-XCD queues for N moduli (1 <= N <= 16).
+const void* batched_crt_kernel_ptr(int N);
+void launch_batched_crt(int N, int grid, hipStream_t st, const momms_gemm_desc* d,
+                        const SplitProb* sp, const BatchItem* items, QueueBounds qb,
+                        unsigned* counters, double alpha, const double* A, const double* B,
+                        double beta, double* C, const char* ws);
 
 }  // namespace momms

@@ -71,12 +71,15 @@ def test_batch_plan_rejects_bad_tables_before_gpu_work():
                                                 None, None, ctypes.byref(h), None)
     assert st == -1 and "descs[0]" in capi.last_error()
     good = capi.desc_array([(4, 4, 4, 0, 0, 0, 4, 4, 4)])
-    st = capi.lib().momms_hip_batch_plan_create(ctypes.cast(good, ctypes.c_void_p), 1, 2, 0, 0,
+    st = capi.lib().momms_hip_batch_plan_create(ctypes.cast(good, ctypes.c_void_p), 1, 3, 0, 0,
                                                 None, None, ctypes.byref(h), None)
     assert st == -3
     st = capi.lib().momms_hip_batch_plan_create(ctypes.cast(good, ctypes.c_void_p), 1, 1, 9, 0,
                                                 None, None, ctypes.byref(h), None)
     assert st == -4
+    st = capi.lib().momms_hip_batch_plan_create(ctypes.cast(good, ctypes.c_void_p), 1, 2, 17, 0,
+                                                None, None, ctypes.byref(h), None)
+    assert st == -4 and "moduli" in capi.last_error()
     st = capi.lib().momms_hip_batch_plan_create(ctypes.cast(good, ctypes.c_void_p), 1, 0, 0, 1,
                                                 None, None, ctypes.byref(h), None)
     assert st == -6
@@ -206,3 +209,64 @@ def test_dgemm_tree_rejects_bad_steps_before_gpu_work():
     assert lib.momms_hip_dgemm_tree(*args, None, 2, None) == -16
     assert lib.momms_hip_dgemm_tree(*args, None, -1, None) == -16
     assert "momms_hip_dgemm_tree" in capi.last_error()
+
+
+def test_batch_class_follows_its_formula():
+    import math
+
+    rows, _, _, _ = capi.uneven_batch(300)
+    rows = rows + [(0, 5, 7, 0, 0, 0, 1, 7, 1)]  # an empty problem does not count
+    live = [r for r in rows if r[0] and r[1]]
+    mean_k = sum(r[2] for r in live) / len(live)
+    want = 16 * min(15, int(math.log2(1 + len(live)))) + min(8, int(math.log2(1 + mean_k)))
+    assert capi.batch_class(rows) == want
+    assert capi.batch_class([]) == 0
+
+
+def test_tuning_table_round_trip_and_refusals(tmp_path):
+    saved = capi.tuning_export()
+    try:
+        table = {"version": capi.TUNING_VERSION, "entries": [
+            {"kind": "batch_fp64", "key": 200, "value": [48, 256, 0, 0]},
+            {"kind": "batch_split", "key": 200, "value": [6, 0, 0, 0]},
+            {"kind": "batch_crt", "key": 17, "value": [13, 0, 0, 0]}]}
+        capi.tuning_import(table, replace=True)
+        got = capi.tuning_export()
+        assert got["version"] == capi.TUNING_VERSION
+        assert sorted((e["kind"], e["key"], tuple(e["value"])) for e in got["entries"]) == sorted(
+            (e["kind"], e["key"], tuple(e["value"])) for e in table["entries"])
+        # merge: same (kind, key) replaced, others kept
+        capi.tuning_import({"version": 1, "entries": [
+            {"kind": "batch_fp64", "key": 200, "value": [32, 1 << 30, 0, 0]}]})
+        got = {(e["kind"], e["key"]): e["value"] for e in capi.tuning_export()["entries"]}
+        assert got[("batch_fp64", 200)] == [32, 1 << 30, 0, 0] and len(got) == 3
+        # through a file
+        path = tmp_path / "tuning.json"
+        capi.save_tuning(path)
+        capi.tuning_import({"version": 1, "entries": []}, replace=True)
+        assert capi.tuning_export()["entries"] == []
+        capi.load_tuning(path)
+        assert len(capi.tuning_export()["entries"]) == 3
+        before = capi.tuning_export()
+        bad = [
+            {"version": 2, "entries": []},
+            {"version": 1, "entries": [{"kind": 9, "key": 0, "value": [1, 0, 0, 0]}]},
+            {"version": 1, "entries": [{"kind": "batch_fp64", "key": 256, "value": [48, 256, 0, 0]}]},
+            {"version": 1, "entries": [{"kind": "batch_fp64", "key": 3, "value": [300, 256, 0, 0]}]},
+            {"version": 1, "entries": [{"kind": "batch_fp64", "key": 3, "value": [8, 256, 0, 0]}]},
+            {"version": 1, "entries": [{"kind": "batch_split", "key": 3, "value": [9, 0, 0, 0]}]},
+            {"version": 1, "entries": [{"kind": "batch_crt", "key": 3, "value": [17, 0, 0, 0]}]},
+            {"version": 1, "entries": [{"kind": "batch_crt", "key": 3, "value": [13, 1, 0, 0]}]},
+            # one bad entry refuses the whole table
+            {"version": 1, "entries": [{"kind": "batch_split", "key": 4, "value": [5, 0, 0, 0]},
+                                       {"kind": "batch_split", "key": 5, "value": [0, 0, 0, 0]}]},
+        ]
+        for t in bad:
+            with pytest.raises(capi.MommsError):
+                capi.tuning_import(t, replace=True)
+            assert capi.tuning_export() == before
+        with pytest.raises(ValueError):
+            capi.tuning_import({"version": 1, "entries": [
+                {"kind": "batch_split", "key": i % 256, "value": [5]} for i in range(257)]})
+    finally:
+        capi.tuning_import(saved, replace=True)

@@ -4,8 +4,8 @@ Bars: plain FP64 batch within gamma_{k+1} of the double-double oracle, and so
 within 2 gamma_{k+1} of the loop of per-problem momms_hip_dgemm launches
 (the two take different summation orders: one-wavefront 32 x 32 tiles,
 128 x 128 tiles, split k); the same batch twice gives the same bits;
-split-BF16 batch == the oracle's restatement bit for bit; graph replays ==
-eager runs.
+split-BF16 batch and residue batch == the oracle's restatements bit for bit;
+graph replays == eager runs.
 """
 import numpy as np
 import pytest
@@ -83,7 +83,7 @@ def test_plain_batch_within_gamma_of_oracle(table, autotune):
     plan.close()
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("mode", [0, 1, 2])
 def test_batch_reads_only_its_problems(mode):
     """Arenas whose padding (lda > m, ldb > k, gaps) is NaN and whose last
     problem ends at the last element: shapes where tiles reach past k (k < 16,
@@ -110,7 +110,8 @@ def test_batch_reads_only_its_problems(mode):
         if k and n:
             view(B, b0, k, n, ldb).copy_(torch.rand(k, n, generator=g, device="cuda", dtype=torch.float64) - 0.5)
     assert rows[-1][3] + rows[-1][6] * rows[-1][2] == al and rows[-1][4] + rows[-1][7] * rows[-1][1] == bl
-    plan = capi.BatchPlan(rows, mode=mode, slices=7 if mode else 0, autotune=mode == 0, A=A, B=B)
+    plan = capi.BatchPlan(rows, mode=mode, slices={0: 0, 1: 7, 2: 13}[mode], autotune=mode == 0,
+                          A=A, B=B)
     C = C0.clone()
     plan.run(A, B, C, 1.0, 1.0)
     torch.cuda.synchronize()
@@ -124,7 +125,10 @@ def test_batch_reads_only_its_problems(mode):
             assert (np.abs(got - ref) <= gamma(k + 1) * mag).all(), (m, n, k)
         else:
             want = Cm0.copy(order="F")
-            O.ozaki_gemm(Am, Bm, want, 7, 1.0, 1.0)
+            if mode == 1:
+                O.ozaki_gemm(Am, Bm, want, 7, 1.0, 1.0)
+            else:
+                O.crt_gemm(Am, Bm, want, 13, 1.0, 1.0)
             assert np.array_equal(got.view(np.int64), want.view(np.int64)), (m, n, k)
     plan.close()
 
@@ -189,11 +193,124 @@ def test_split_batch_bitwise_against_restatement(s):
     plan.close()
 
 
-@pytest.mark.parametrize("mode", [0, 1])
+@pytest.mark.parametrize("moduli", [9, 13, 16])
+def test_crt_batch_bitwise_against_restatement(moduli):
+    """The residue batch equals the CPU restatement of the residue form
+    (oracle/crt.c) problem by problem, bit for bit: ragged and degenerate
+    shapes, padded leading dimensions, k = 0, k > 256 (the mid-loop modular
+    reduction), alpha and beta other than 1, and rows of A / columns of B
+    holding Inf and NaN (their entries take the fma-chain path)."""
+    rows, al, bl, cl = edge_rows()
+    A, B, C0 = arenas(rows, al, bl, cl, seed=SEED + moduli)
+    # Inf in row 3 of problem 3's A, NaN in column 5 of problem 4's B
+    m3, n3, k3, ao3, _, _, lda3, _, _ = rows[3]
+    A[ao3 + 3 + 7 * lda3] = float("inf")
+    m4, n4, k4, _, bo4, _, _, ldb4, _ = rows[4]
+    B[bo4 + 5 * ldb4 + 11] = float("nan")
+    plan = capi.BatchPlan(rows, mode=2, slices=moduli)
+    info = plan.describe()
+    assert info["mode"] == "crt_bf16" and info["moduli"] == moduli
+    C = C0.clone()
+    plan.run(A, B, C, -0.5, 1.5)
+    torch.cuda.synchronize()
+    a, b, c0, c = (x.cpu().numpy() for x in (A, B, C0, C))
+    for (m, n, k, ao, bo, co, lda, ldb, ldc) in rows:
+        if m == 0 or n == 0:
+            continue
+        want = np_view(c0, co, m, n, ldc)
+        O.crt_gemm(np_view(a, ao, m, k, lda), np_view(b, bo, k, n, ldb), want, moduli, -0.5, 1.5)
+        got = np_view(c, co, m, n, ldc)
+        assert np.array_equal(got.view(np.int64), want.view(np.int64)), (m, n, k)
+    # the fma-chain entries are there: non-finite in problem 3's row 3
+    assert not np.isfinite(np_view(c, rows[3][5], m3, n3, rows[3][8])[3]).all()
+    plan.close()
+
+
+def test_crt_batch_beta_zero_never_reads_c():
+    rows, al, bl, cl = edge_rows()
+    A, B, _ = arenas(rows, al, bl, cl, seed=SEED + 77)
+    C = torch.full((cl,), float("nan"), device="cuda", dtype=torch.float64)
+    plan = capi.BatchPlan(rows, mode=2, slices=13)
+    plan.run(A, B, C, 1.0, 0.0)
+    torch.cuda.synchronize()
+    a, b, c = (x.cpu().numpy() for x in (A, B, C))
+    for (m, n, k, ao, bo, co, lda, ldb, ldc) in rows:
+        if m == 0 or n == 0:
+            continue
+        want = np.zeros((m, n), order="F")
+        O.crt_gemm(np_view(a, ao, m, k, lda), np_view(b, bo, k, n, ldb), want, 13, 1.0, 0.0)
+        assert np.array_equal(np_view(c, co, m, n, ldc).view(np.int64), want.view(np.int64)), (m, n, k)
+    plan.close()
+
+
+def test_crt_moduli_autotune_meets_accuracy_and_is_remembered():
+    """The autotuner picks the smallest moduli count whose sampled error is
+    within 2x plain FP64's; the choice lands in the tuning table under the
+    batch's class, and a later plan without autotuning takes it from there."""
+    rows, al, bl, cl = capi.uneven_batch(64, 16, 256, seed=SEED + 19)
+    A, B, C0 = arenas(rows, al, bl, cl)
+    saved = capi.tuning_export()
+    try:
+        capi.tuning_import({"version": 1, "entries": []}, replace=True)
+        plan = capi.BatchPlan(rows, mode=2, slices=0, autotune=True, A=A, B=B)
+        N = plan.describe()["moduli"]
+        assert 8 <= N <= 16
+        entry = {(e["kind"], e["key"]): e["value"] for e in capi.tuning_export()["entries"]}
+        assert entry[("batch_crt", capi.batch_class(rows))] == [N, 0, 0, 0]
+        C = C0.clone()
+        plan.run(A, B, C, 1.0, 0.0)
+        torch.cuda.synchronize()
+        a, b, c = (x.cpu().numpy() for x in (A, B, C))
+        worst = 0.0
+        for (m, n, k, ao, bo, co, lda, ldb, ldc) in rows[:20]:
+            ref, mag = O.dd_gemm(np_view(a, ao, m, k, lda), np_view(b, bo, k, n, ldb),
+                                 np.zeros((m, n), order="F"), 1.0, 0.0)
+            worst = max(worst, float(np.max(np.abs(np_view(c, co, m, n, ldc) - ref) / mag)))
+        assert worst <= 4 * gamma(256)
+        plan.close()
+        again = capi.BatchPlan(rows, mode=2, slices=0)
+        assert again.describe()["moduli"] == N
+        again.close()
+    finally:
+        capi.tuning_import(saved, replace=True)
+
+
+def test_tuned_plain_thresholds_are_used_from_an_imported_table():
+    rows, al, bl, cl = capi.uneven_batch(128, 16, 300, seed=SEED + 23)
+    A, B, C0 = arenas(rows, al, bl, cl)
+    key = capi.batch_class(rows)
+    saved = capi.tuning_export()
+    try:
+        outs = []
+        for T, T2 in [(16, 1 << 30), (16, 16)]:
+            capi.tuning_import({"version": 1, "entries": [
+                {"kind": "batch_fp64", "key": key, "value": [T, T2, 0, 0]}]}, replace=True)
+            plan = capi.BatchPlan(rows, mode=0)
+            info = plan.describe()
+            assert (info["threshold"], info["threshold_128"]) == (T, T2)
+            assert info["tiles_32x32_wavefront"] == 0
+            assert (info["tiles_128x128"] == 0) == (T2 == 1 << 30)
+            C = C0.clone()
+            plan.run(A, B, C, 1.0, 1.0)
+            outs.append(C)
+            plan.close()
+        torch.cuda.synchronize()
+        a, b, c0 = (x.cpu().numpy() for x in (A, B, C0))
+        for C in outs:
+            c = C.cpu().numpy()
+            for (m, n, k, ao, bo, co, lda, ldb, ldc) in rows[:40]:
+                ref, mag = O.dd_gemm(np_view(a, ao, m, k, lda), np_view(b, bo, k, n, ldb),
+                                     np_view(c0, co, m, n, ldc))
+                assert (np.abs(np_view(c, co, m, n, ldc) - ref) <= gamma(k + 1) * mag).all()
+    finally:
+        capi.tuning_import(saved, replace=True)
+
+
+@pytest.mark.parametrize("mode", [0, 1, 2])
 def test_graph_replays_match_eager(mode):
     rows, al, bl, cl = capi.uneven_batch(200, 16, 256, seed=SEED + mode)
     A, B, C0 = arenas(rows, al, bl, cl)
-    plan = capi.BatchPlan(rows, mode=mode, slices=5)
+    plan = capi.BatchPlan(rows, mode=mode, slices=12 if mode == 2 else 5)
     Ce = C0.clone()
     plan.run(A, B, Ce, 1.0, 1.0)
     Cg = C0.clone()
@@ -282,4 +399,34 @@ def test_plan_rejects_bad_tables():
     with pytest.raises(capi.MommsError):
         capi.BatchPlan([(4, 4, 4, 0, 0, 0, 2, 4, 4)])   # lda < m
     with pytest.raises(capi.MommsError):
-        capi.BatchPlan([(4, 4, 4, 0, 0, 0, 4, 4, 4)], mode=2)
+        capi.BatchPlan([(4, 4, 4, 0, 0, 0, 4, 4, 4)], mode=3)
+    # one modulus cannot hold the products of a k = 4096 problem
+    assert O.crt_beta(4096, 1) < 1
+    with pytest.raises(capi.MommsError, match="too few moduli"):
+        capi.BatchPlan([(4, 4, 4096, 0, 0, 0, 4, 4096, 4)], mode=2, slices=1)
+
+
+def test_config5_crt_graph_bitwise_on_sampled_problems():
+    """The 4096-problem uneven batch through the residue plan at 13 moduli,
+    captured and launched as one graph: 64 problems drawn with a fixed seed
+    equal the oracle's restatement bit for bit, and the whole output is
+    finite."""
+    rows, al, bl, cl = capi.uneven_batch(4096)
+    A, B, C0 = arenas(rows, al, bl, cl, seed=SEED + 13)
+    plan = capi.BatchPlan(rows, mode=2, slices=13)
+    assert plan.describe()["moduli"] == 13
+    C = C0.clone()
+    plan.capture(A, B, C, 1.0, 1.0)
+    plan.launch()
+    torch.cuda.synchronize()
+    assert torch.isfinite(C).all()
+    pick = np.random.default_rng(SEED + 4097).choice(len(rows), 64, replace=False)
+    for i in sorted(pick):
+        m, n, k, ao, bo, co, lda, ldb, ldc = rows[i]
+        a = view(A, ao, m, k, lda).cpu().numpy()
+        b = view(B, bo, k, n, ldb).cpu().numpy()
+        want = np.asfortranarray(view(C0, co, m, n, ldc).cpu().numpy())
+        O.crt_gemm(np.asfortranarray(a), np.asfortranarray(b), want, 13, 1.0, 1.0)
+        got = np.asfortranarray(view(C, co, m, n, ldc).cpu().numpy())
+        assert np.array_equal(got.view(np.int64), want.view(np.int64)), (i, m, n, k)
+    plan.close()
