@@ -21,7 +21,7 @@ moduli, N residue GEMMs). Prints ONE JSON line:
                pack / unpack-add) unless --quick
   cpu_baseline the restated Goto + 6x8 CPU path on this host's cores
 
---only {plain,split,multiply,crt,batched,batched_plain,copy,hier,fp32} runs one kernel family in a loop
+--only {plain,split,multiply,crt,batched,batched_plain,batched_crt,copy,hier,fp32} runs one kernel family in a loop
 (for profilers) and prints nothing else but its timing line.
 """
 from __future__ import annotations
@@ -346,6 +346,14 @@ def batched(capi, torch, gen, steps):
     res["split7_graph"] = {"ms": round(ms, 4), "tflops_fp64_equiv": round(flops / ms / 1e9, 3),
                            "plan": splan.describe()}
     splan.close()
+    # the residue form, its moduli count chosen by the autotuner (the smallest
+    # whose sampled error is within 2x plain FP64's on this batch)
+    cplan = capi.BatchPlan(rows, mode=2, slices=0, autotune=True, A=A, B=B)
+    cplan.capture(A, B, C, 1.0, 1.0)
+    ms = time_loop(cplan.launch, max(steps, 10), 3)
+    res["crt_graph"] = {"ms": round(ms, 4), "tflops_fp64_equiv": round(flops / ms / 1e9, 3),
+                        "plan": cplan.describe()}
+    cplan.close()
     return res
 
 
@@ -405,7 +413,7 @@ def main() -> int:
     ap.add_argument("--moduli", type=int, default=14, help="moduli of the residue form (--only crt)")
     ap.add_argument("--only", choices=["plain", "plain_sweep", "split", "multiply", "crt", "crt_i8",
                                        "batched",
-                                       "batched_plain", "copy", "hier", "fp32"])
+                                       "batched_plain", "batched_crt", "copy", "hier", "fp32"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported. This engine "
@@ -480,6 +488,19 @@ def main() -> int:
     if args.only == "batched":
         del A, B, C0, C
         print(json.dumps({"only": "batched", **batched(capi, torch, gen, args.steps)}))
+        return 0
+    if args.only == "batched_crt":
+        # the uneven batch through the residue plan at --moduli moduli, eager launches
+        del A, B, C0, C
+        rows, al, bl, cl = capi.uneven_batch(4096)
+        flops = sum(2.0 * r[0] * r[1] * r[2] for r in rows)
+        Ab = torch.rand(al, generator=gen, device="cuda", dtype=torch.float64) * 2 - 1
+        Bb = torch.rand(bl, generator=gen, device="cuda", dtype=torch.float64) * 2 - 1
+        Cb = torch.rand(cl, generator=gen, device="cuda", dtype=torch.float64) * 2 - 1
+        plan = capi.BatchPlan(rows, mode=2, slices=args.moduli)
+        ms = time_loop(lambda: plan.run(Ab, Bb, Cb, 1.0, 1.0), args.steps, args.warmup)
+        print(json.dumps({"only": "batched_crt", "ms": ms, "tflops_fp64_equiv": flops / ms / 1e9,
+                          "plan": plan.describe()}))
         return 0
     if args.only == "batched_plain":
         # the uneven batch through the tuned plain plan only, eager launches
