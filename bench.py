@@ -21,7 +21,8 @@ moduli, N residue GEMMs). Prints ONE JSON line:
                pack / unpack-add) unless --quick
   cpu_baseline the restated Goto + 6x8 CPU path on this host's cores
 
---only {plain,split,multiply,crt,batched,batched_plain,batched_crt,copy,hier,fp32} runs one kernel family in a loop
+--only {plain,split,multiply,crt,batched,batched_plain,batched_split,batched_crt,copy,hier,fp32}
+    runs one kernel family in a loop
 (for profilers) and prints nothing else but its timing line.
 """
 from __future__ import annotations
@@ -413,7 +414,8 @@ def main() -> int:
     ap.add_argument("--moduli", type=int, default=14, help="moduli of the residue form (--only crt)")
     ap.add_argument("--only", choices=["plain", "plain_sweep", "split", "multiply", "crt", "crt_i8",
                                        "batched",
-                                       "batched_plain", "batched_crt", "copy", "hier", "fp32"])
+                                       "batched_plain", "batched_split", "batched_crt", "copy", "hier",
+                                       "fp32"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported. This engine "
@@ -488,6 +490,19 @@ def main() -> int:
     if args.only == "batched":
         del A, B, C0, C
         print(json.dumps({"only": "batched", **batched(capi, torch, gen, args.steps)}))
+        return 0
+    if args.only == "batched_split":
+        # the uneven batch through the slice plan at --slices slices, eager launches
+        del A, B, C0, C
+        rows, al, bl, cl = capi.uneven_batch(4096)
+        flops = sum(2.0 * r[0] * r[1] * r[2] for r in rows)
+        Ab = torch.rand(al, generator=gen, device="cuda", dtype=torch.float64) * 2 - 1
+        Bb = torch.rand(bl, generator=gen, device="cuda", dtype=torch.float64) * 2 - 1
+        Cb = torch.rand(cl, generator=gen, device="cuda", dtype=torch.float64) * 2 - 1
+        plan = capi.BatchPlan(rows, mode=1, slices=s)
+        ms = time_loop(lambda: plan.run(Ab, Bb, Cb, 1.0, 1.0), args.steps, args.warmup)
+        print(json.dumps({"only": "batched_split", "ms": ms, "tflops_fp64_equiv": flops / ms / 1e9,
+                          "plan": plan.describe()}))
         return 0
     if args.only == "batched_crt":
         # the uneven batch through the residue plan at --moduli moduli, eager launches
