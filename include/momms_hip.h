@@ -249,7 +249,7 @@ typedef struct momms_batch_plan momms_batch_plan;
  * needs for the largest k of the batch), every problem's result bitwise
  * equal to momms_hip_dgemm_crt_bf16 with that many moduli, in one fused
  * launch (residues, products and reconstruction on chip per tile) after an
- * exponent pre-pass.
+ * exponent pre-pass; lda and ldb of every problem at most 2^21 (-1 otherwise).
  * autotune != 0 times the candidate tile shapes per size bucket on this
  * batch (reading A and B, writing only plan-owned scratch) and remembers the
  * choice for the process. A and B may be NULL when autotune == 0.

@@ -751,6 +751,9 @@ extern "C" int momms_hip_batch_plan_create(const momms_gemm_desc* descs, int64_t
       std::snprintf(buf, sizeof(buf), "descs[%lld] is invalid", (long long)i);
       return bad_arg(fn, 1, buf);
     }
+    // the residue mode's fused kernel takes 32-bit byte offsets from a slab's corner
+    if (mode == 2 && p.m && p.n && std::max(p.lda, p.ldb) > kBatchLeanLd)
+      return bad_arg(fn, 1, "residue mode: lda and ldb must be at most 2^21");
   }
   hipStream_t st = (hipStream_t)stream;
   auto* P = new momms_batch_plan();

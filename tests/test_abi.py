@@ -80,6 +80,10 @@ def test_batch_plan_rejects_bad_tables_before_gpu_work():
     st = capi.lib().momms_hip_batch_plan_create(ctypes.cast(good, ctypes.c_void_p), 1, 2, 17, 0,
                                                 None, None, ctypes.byref(h), None)
     assert st == -4 and "moduli" in capi.last_error()
+    wide = capi.desc_array([(4, 4, 4, 0, 0, 0, (1 << 21) + 1, 4, 4)])
+    st = capi.lib().momms_hip_batch_plan_create(ctypes.cast(wide, ctypes.c_void_p), 1, 2, 13, 0,
+                                                None, None, ctypes.byref(h), None)
+    assert st == -1 and "2^21" in capi.last_error()
     st = capi.lib().momms_hip_batch_plan_create(ctypes.cast(good, ctypes.c_void_p), 1, 0, 0, 1,
                                                 None, None, ctypes.byref(h), None)
     assert st == -6
