@@ -58,7 +58,12 @@ int momms_hip_dgemm(int64_t m, int64_t n, int64_t k, const double* alpha, const 
  * added in k order by a second kernel (6: 64x64 tiles, three workgroups per
  * CU; 7: 128x64, two per CU; 8: 64x64, two per CU with a deeper slab ring;
  * 9: 64x64, one per CU with a six-slab ring). 5-9 use a device workspace the
- * library allocates once per stream and keeps. Argument 15 is tile_cfg.
+ * library allocates once per stream and keeps. 10-11 = k split over the four
+ * wavefronts of a workgroup, each multiplying a quarter of k into the whole
+ * tile straight from global memory, the four partial tiles added in LDS in
+ * wavefront order (10: 64x64 tiles, one workgroup per CU; 11: 32x32 tiles,
+ * two per CU); no workspace; A and B column-major with leading dimensions
+ * <= 2^21 (otherwise the register-staged tiles run). Argument 15 is tile_cfg.
  * Same semantics as momms_hip_dgemm. */
 int momms_hip_dgemm_tiled(int64_t m, int64_t n, int64_t k, const double* alpha,
                           const double* A, int64_t rsA, int64_t csA, const double* B,

@@ -18,6 +18,7 @@
 // consecutive rows of one column: stores to column-major C are 128-byte
 // coalesced.
 #include "dgemm.hpp"
+#include "dgemm_ksplit.hpp"
 #include "dgemm_streamk.hpp"
 #include "dgemm_tile.hpp"
 #include "mfma.hpp"
@@ -195,7 +196,8 @@ static void splitk_plan(const DgemmArgs& g, int64_t& kper, int64_t& used) {
 static int resolve_cfg(const DgemmArgs& g, int tile_cfg, uint32_t& a_bytes, uint32_t& b_bytes) {
   if (tile_cfg < 0) tile_cfg = dgemm_pick_tile(g.m, g.n, g.k);
   const bool dma = dma_ok(g, a_bytes, b_bytes);
-  if (tile_cfg >= 5 && !dma) {
+  if (tile_cfg >= 10 && ksplit_ok(g)) return tile_cfg;
+  if (tile_cfg >= 5 && (!dma || tile_cfg >= 10)) {
     // operands the buffer->LDS path cannot take: the register-staged tiles
     const int64_t big = ceil_div(g.m, 128) * ceil_div(g.n, 128);
     const int64_t mid = ceil_div(g.m, 128) * ceil_div(g.n, 64);
@@ -229,6 +231,7 @@ hipError_t dgemm_launch(const DgemmArgs& g, int tile_cfg, const DgemmWorkspace* 
   const int lb = layout_of(g.rsB, g.csB, g.k, g.n);
   uint32_t a_bytes = 0, b_bytes = 0;
   tile_cfg = resolve_cfg(g, tile_cfg, a_bytes, b_bytes);
+  if (tile_cfg >= 10) return ksplit_launch(g, tile_cfg - 10);
   if (tile_cfg >= 6 && tile_cfg <= 9) {
     const int cfg = tile_cfg - 6;
     hipError_t e = hipSuccess;
@@ -295,9 +298,14 @@ int dgemm_pick_tile(int64_t m, int64_t n, int64_t k) {
   // tiles at three workgroups per CU when there are fewer big tiles than
   // CUs, 128 x 64 tiles at two per CU when the last round of big tiles
   // would be less than 85 % full (3000^3: 47 -> 55 TF).
+  // Problems of at most 16 such tiles take 32 x 32 tiles with k split over
+  // the four wavefronts of each workgroup (512^3: 12.9 -> 19.8 TF, 256^3:
+  // 2.3 -> 3.9 TF, 256 x 1024 x 1024: 14.3 -> 23.0 TF; from 640^3 on the
+  // stream-K form is faster).
   (void)k;
   constexpr int64_t kCus = 256;
   const int64_t big = ceil_div(m, 128) * ceil_div(n, 128);
+  if (big <= 16) return 11;
   if (big < kCus) return 6;
   const int64_t rounds = ceil_div(big, kCus);
   if (big * 100 >= rounds * kCus * 85) return 0;

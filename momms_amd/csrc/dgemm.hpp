@@ -27,8 +27,12 @@ struct DgemmArgs {
 // fixed-order reduction, 6-9 = persistent stream-K with the tile forms of
 // dgemm_streamk.hpp (6: 64x64 three per CU, 7: 128x64, 8: 64x64 two per CU,
 // 9: 64x64 one per CU) (5-9 fall back to the built-in register-staged
-// choice when the buffer->LDS path does not apply).
-constexpr int kDgemmTileConfigs = 10;
+// choice when the buffer->LDS path does not apply), 10-11 = k split over the
+// four wavefronts of a workgroup, partial tiles added in LDS in wavefront
+// order (10: 64x64, one workgroup per CU; 11: 32x32, two per CU;
+// dgemm_ksplit.hpp; the register-staged choice when A or B is not
+// column-major with leading dimension <= 2^21).
+constexpr int kDgemmTileConfigs = 12;
 
 int dgemm_splitk_parts(int64_t m, int64_t n, int64_t k);
 

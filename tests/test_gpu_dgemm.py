@@ -78,7 +78,7 @@ def test_ragged_shapes_and_layouts(m, n, k, la, lb, lc):
     check_against_oracle(A, B, C0, C, 1.0, 1.0, k)
 
 
-@pytest.mark.parametrize("tile", list(range(10)))
+@pytest.mark.parametrize("tile", list(range(12)))
 def test_every_tile_config(tile):
     rng = np.random.default_rng(SEED + tile)
     m, n, k = 300, 250, 99
@@ -282,3 +282,33 @@ def test_small_gemm_leaf(m, n, k, layout):
     torch.cuda.synchronize()
     assert torch.isfinite(Cz).all()
     check_against_oracle(A, B, torch.zeros_like(Cz), Cz, 1.0, 0.0, k)
+
+
+@pytest.mark.parametrize("tile", [10, 11])
+@pytest.mark.parametrize("m,n,k", [(512, 512, 512), (1024, 1024, 1024), (333, 77, 1000),
+                                   (64, 64, 7), (100, 1, 300), (1, 130, 64), (40, 50, 0)])
+def test_k_split_in_workgroup_is_deterministic_and_accurate(tile, m, n, k):
+    """The form with k split over the four wavefronts of a workgroup (partial
+    tiles added in LDS in wavefront order): ragged edges, k not a multiple of
+    the 8-deep chunks or of 4, one row or one column, k = 0. Repeat calls give
+    the same bits; sampled entries are within gamma_{k+1} of double-double;
+    beta = 0 never reads C."""
+    rng = np.random.default_rng(SEED + m + 3 * n + 7 * k + tile)
+    a, b, c0 = (uniform_colmajor(rng, *s) for s in ((m, k), (k, n), (m, n)))
+    A = to_dev(a) if k else torch.empty((m, 0), dtype=torch.float64, device="cuda")
+    B = to_dev(b) if k else torch.empty((0, n), dtype=torch.float64, device="cuda")
+    C1, C2 = to_dev(c0), to_dev(c0)
+    capi.dgemm(A, B, C1, -0.75, 1.25, tile_cfg=tile)
+    capi.dgemm(A, B, C2, -0.75, 1.25, tile_cfg=tile)
+    C3 = torch.full((n, m), float("nan"), dtype=torch.float64, device="cuda").T
+    capi.dgemm(A, B, C3, -0.75, 0.0, tile_cfg=tile)
+    torch.cuda.synchronize()
+    assert torch.equal(C1, C2)
+    assert torch.isfinite(C3).all()
+    rows = rng.integers(0, m, 2048)
+    cols = rng.integers(0, n, 2048)
+    ref, mag = O.dd_entries(a, b, c0, rows, cols, -0.75, 1.25)
+    assert (np.abs(host(C1)[rows, cols] - ref) <= gamma(k + 1) * mag).all()
+    ref0, mag0 = O.dd_entries(a, b, c0, rows, cols, -0.75, 0.0)
+    assert (np.abs(host(C3)[rows, cols] - ref0) <= gamma(k + 1) * mag0).all()
+
