@@ -6,6 +6,7 @@
 //   HipGemm                                  host Matrix<T> views, staged to HBM
 //   PartN<3000, PartK<192, HipGemm>>         beta := 1 after the first k block
 //   SpawnThreads<HipGemm> on 4 threads       only thread 0 launches
+//   SpawnThreads<ParallelN<.., HipGemm>>     every sub-team launches, at once
 //   HipGemm on device-resident matrices      no staging
 //   HipSplitGemm<7>                          FP64-accurate from BF16 slices
 //   HipCrtGemm<13>                           FP64-accurate from BF16 residues
@@ -125,6 +126,12 @@ int main() {
                                                         3100, 400, 1, 2.0, false);
   check_tree<SpawnThreads<HipGemm<double>>>("SpawnThreads<HipGemm> x4", 129, 300, 250, 4, 2.0,
                                             false);
+  // sub-teams of one thread each launch on the NULL stream at once; their
+  // shapes take the stream-K form, whose workspace is per calling thread
+  check_tree<SpawnThreads<ParallelN<64, Target<2>, HipGemm<double>>>>(
+      "SpawnThreads<ParallelN<64,Target<2>,HipGemm>> x2", 1024, 1024, 2048, 2, 2.0, true);
+  check_tree<SpawnThreads<ParallelN<64, TheRest, HipGemm<double>>>>(
+      "SpawnThreads<ParallelN<64,TheRest,HipGemm>> x3", 700, 1500, 1800, 3, 2.0, true);
   check_tree<HipGemm<double>>("HipGemm (device matrices)", 512, 384, 640, 1, 2.0, true);
   check_tree<HipSplitGemm<7>>("HipSplitGemm<7>", 300, 200, 700, 1, 2.0, false);
   check_tree<HipCrtGemm<13>>("HipCrtGemm<13>", 300, 200, 700, 1, 2.0, false);
