@@ -21,7 +21,15 @@ moduli, N residue GEMMs). Prints ONE JSON line:
                pack / unpack-add) unless --quick
   cpu_baseline the restated Goto + 6x8 CPU path on this host's cores
 
---only {plain,split,multiply,crt,batched,batched_plain,batched_split,batched_crt,copy,hier,fp32}
+roofline.traffic (HBM bytes per launch of crt_multiply) is measured in the
+run: two rocprofv3 --pmc runs (FETCH_SIZE, then WRITE_SIZE) of this script's
+--only traffic probe in a child process, each counter scaled by its reading
+for a 1 GiB device copy in the same probe. --no-traffic skips them and uses
+the reading stored in profiles/roofline_traffic.json only when it was taken
+on the current kernel sources (sha256 over crt_bf16.hip and its headers).
+
+--only {plain,split,multiply,crt,batched,batched_plain,batched_split,batched_crt,copy,hier,fp32,
+        traffic}
     runs one kernel family in a loop
 (for profilers) and prints nothing else but its timing line.
 """
@@ -399,6 +407,121 @@ def hier(capi, torch, gen, steps, n=8192):
     return out
 
 
+def sources_digest(name: str) -> str:
+    """sha256 over a kernel's source file under momms_amd/csrc and every local
+    header it includes, transitively: the files whose change can change the
+    kernel, so a stored counter reading is used only for the sources it was
+    taken on."""
+    import hashlib
+    import re
+
+    dirs = [ROOT / "momms_amd" / "csrc", ROOT / "include"]
+    seen, todo = {}, [ROOT / "momms_amd" / "csrc" / name]
+    while todo:
+        p = todo.pop()
+        rel = str(p.relative_to(ROOT))
+        if rel in seen:
+            continue
+        seen[rel] = p.read_bytes()
+        for inc in re.findall(rb'#include\s+"([^"]+)"', seen[rel]):
+            for d in [p.parent] + dirs:
+                q = d / inc.decode()
+                if q.exists():
+                    todo.append(q.resolve())
+                    break
+    h = hashlib.sha256()
+    for rel in sorted(seen):
+        h.update(rel.encode() + b"\0" + seen[rel])
+    return h.hexdigest()
+
+
+COPY_BYTES = 1 << 30  # the calibration copy of the traffic probe
+
+
+def traffic_probe(capi, torch, gen, n: int, moduli: int) -> dict:
+    """The launches whose HBM counters give the headline kernel's traffic:
+    five copies of a 1 GiB device buffer (a known 1 GiB read + 1 GiB write
+    each, which scales the counters), then three launches of the residue
+    products (crt_multiply) at m=n=k=n with `moduli` moduli."""
+    src = torch.empty(COPY_BYTES, dtype=torch.uint8, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(5):
+        dst.copy_(src)
+    torch.cuda.synchronize()
+    del src, dst
+    A, B = rand_colmajor(n, n, gen), rand_colmajor(n, n, gen)
+    ws = capi.CrtWorkspace(n, n, n, moduli)
+    ws.prepare(A, B)
+    for _ in range(3):
+        ws.products()
+    torch.cuda.synchronize()
+    return {"only": "traffic", "copies": 5, "products": 3, "moduli": moduli, "size": n}
+
+
+def _pmc_dispatches(counter: str, args: list, out: Path, limit: int) -> dict:
+    """KiB per dispatch of one rocprofv3 --pmc counter over this script run
+    with `args` in a child process, grouped by kernel name."""
+    import csv
+    import subprocess
+
+    d = out / counter
+    cmd = ["timeout", "-s", "KILL", str(limit), "rocprofv3", "--pmc", counter, "-d", str(d),
+           "-o", "p", "--output-format", "csv", "--", sys.executable, str(ROOT / "bench.py"),
+           *args]
+    log("bench: traffic pass: " + " ".join(cmd[4:]))
+    r = subprocess.run(cmd, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"),
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=limit + 30)
+    if r.returncode != 0:
+        tail = r.stdout.decode(errors="replace").strip().splitlines()[-3:]
+        raise RuntimeError(f"rocprofv3 --pmc {counter} exited {r.returncode}: {tail}")
+    files = sorted(d.rglob("*counter_collection.csv"))
+    if not files:
+        raise RuntimeError(f"rocprofv3 --pmc {counter} wrote no counter_collection.csv")
+    per: dict = {}
+    with open(files[0], newline="") as f:
+        for row in csv.DictReader(f):
+            if row["Counter_Name"] != counter:
+                continue
+            key = (row["Kernel_Name"], int(row["Dispatch_Id"]))
+            per[key] = per.get(key, 0.0) + float(row["Counter_Value"])
+    by_kernel: dict = {}
+    for (name, _), v in sorted(per.items(), key=lambda kv: kv[0][1]):
+        by_kernel.setdefault(name, []).append(v)
+    return by_kernel
+
+
+def _median_of(by_kernel: dict, part: str) -> float:
+    vals = sorted(v for name, vs in by_kernel.items() if part in name for v in vs)
+    if not vals:
+        raise RuntimeError(f"no dispatch of a kernel named *{part}*")
+    return vals[len(vals) // 2]
+
+
+def measure_traffic(n: int, moduli: int, limit: int = 180) -> dict:
+    """HBM bytes per launch of crt_multiply at m=n=k=n, `moduli` moduli,
+    measured now: FETCH_SIZE and WRITE_SIZE (KiB per dispatch), each from its
+    own rocprofv3 --pmc run of the traffic probe in a child process, each
+    scaled by what the same counter reads for the probe's 1 GiB device copy
+    (the medians over the dispatches of each kernel)."""
+    import shutil
+    import tempfile
+
+    out = Path(tempfile.mkdtemp(prefix="momms_traffic_", dir="/tmp"))
+    try:
+        probe = ["--only", "traffic", "--size", str(n), "--moduli", str(moduli), "--no-cpu"]
+        kib = {}
+        for counter in ("FETCH_SIZE", "WRITE_SIZE"):
+            by_kernel = _pmc_dispatches(counter, probe, out, limit)
+            kib[counter] = (_median_of(by_kernel, "copyBuffer"), _median_of(by_kernel, "crt_multiply"))
+    finally:
+        shutil.rmtree(out, ignore_errors=True)
+    ff = COPY_BYTES / 1024 / kib["FETCH_SIZE"][0]
+    wf = COPY_BYTES / 1024 / kib["WRITE_SIZE"][0]
+    fetch, write = kib["FETCH_SIZE"][1], kib["WRITE_SIZE"][1]
+    return {"fetch_kib": fetch, "write_kib": write, "fetch_factor": round(ff, 4),
+            "write_factor": round(wf, 4), "traffic_bytes": int(round((fetch * ff + write * wf) * 1024))}
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -412,10 +535,15 @@ def main() -> int:
     ap.add_argument("--quick", action="store_true", help="headline only, no other configs")
     ap.add_argument("--tile", type=int, default=None, help="FP64 tile configuration (--only plain)")
     ap.add_argument("--moduli", type=int, default=14, help="moduli of the residue form (--only crt)")
+    ap.add_argument("--no-traffic", action="store_true",
+                    help="skip the two rocprofv3 counter runs that measure roofline.traffic")
+    ap.add_argument("--traffic-out", default=None,
+                    help="also store the measured traffic, keyed by workload and kernel sources, "
+                         "in this JSON file (the format of profiles/roofline_traffic.json)")
     ap.add_argument("--only", choices=["plain", "plain_sweep", "split", "multiply", "crt", "crt_i8",
                                        "batched",
                                        "batched_plain", "batched_split", "batched_crt", "copy", "hier",
-                                       "fp32"])
+                                       "fp32", "traffic"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported. This engine "
@@ -430,6 +558,9 @@ def main() -> int:
     s = args.slices if args.slices else 7
     gen = torch.Generator(device="cuda").manual_seed(SEED & 0xFFFFFFFF)
     flops = 2.0 * n ** 3
+    if args.only == "traffic":
+        print(json.dumps(traffic_probe(capi, torch, gen, n, args.moduli)))
+        return 0
     log(f"bench: inputs {n}^3 on {torch.cuda.get_device_name(0)}")
     A = rand_colmajor(n, n, gen)
     B = rand_colmajor(n, n, gen)
@@ -624,23 +755,45 @@ def main() -> int:
     crt_tf = flops / crt_ms / 1e9
     i8_tf = flops / i8_ms / 1e9
 
-    # HBM bytes per launch of the dominant kernel, measured with rocprofv3
-    # FETCH_SIZE / WRITE_SIZE (calibrated on a plain copy) for this exact
-    # workload; null for a workload that was not profiled
-    traffic = {}
-    tpath = ROOT / "profiles" / "roofline_traffic.json"
-    if tpath.exists():
-        traffic = json.loads(tpath.read_text())
-
     use_crt = crt_tf >= split_tf
     if use_crt:
         value, step_ms = crt_tf, crt_ms
         workload = f"FP64-accurate GEMM C += A*B, m=n=k={n}, split-BF16 residues, {nm} moduli"
         dtype = "f64 in/out; bf16 residues x bf16 residues -> fp32 (exact, mod p), 128-bit CRT"
+        # HBM bytes per launch: measured now by two rocprofv3 --pmc runs of a
+        # child process (FETCH_SIZE, WRITE_SIZE, scaled on a 1 GiB copy); with
+        # --no-traffic or when a run fails, the stored reading of
+        # profiles/roofline_traffic.json if it was taken on these kernel
+        # sources, else null
+        digest = sources_digest("crt_bf16.hip")
+        key = f"crt_multiply moduli={nm} m=n=k={n}"
+        traffic, source = None, "not measured"
+        if not args.no_traffic:
+            try:
+                t = measure_traffic(n, nm)
+                traffic, source = t["traffic_bytes"], {"measured": "in this run", **t}
+                if args.traffic_out:
+                    out = Path(args.traffic_out)
+                    table = json.loads(out.read_text()) if out.exists() else {}
+                    table[key] = {**t, "sources_sha256": digest,
+                                  "command": f"python bench.py --size {n} (moduli {nm} chosen)"}
+                    out.write_text(json.dumps(table, indent=1) + "\n")
+            except Exception as exc:  # the profiler is optional for the bench line
+                log(f"bench: traffic measurement failed: {exc}")
+                source = f"measurement failed: {exc}"
+        if traffic is None:
+            tpath = ROOT / "profiles" / "roofline_traffic.json"
+            stored = json.loads(tpath.read_text()).get(key, {}) if tpath.exists() else {}
+            if stored.get("sources_sha256") == digest:
+                traffic = stored["traffic_bytes"]
+                source = {"measured": "stored reading for these kernel sources",
+                          **{k: v for k, v in stored.items() if k != "sources_sha256"}}
         roof = {"bound": "mfma", "achieved": round(nm * flops / prod_ms / 1e9, 2),
                 "peak": PEAK_BF16, "unit": "TFLOP/s",
                 "frac": round(nm * flops / prod_ms / 1e9 / PEAK_BF16, 4),
-                "traffic": traffic.get(f"crt_multiply moduli={nm} m=n=k={n}", {}).get("traffic_bytes"),
+                "traffic": traffic, "traffic_source": source,
+                # BF16 residues of A and B in, one 16-bit residue of C out, per modulus
+                "algorithmic_bytes_per_launch": 2 * nm * 3 * n * n,
                 "kernel": "crt_multiply (all moduli, one launch)", "ms_per_launch": round(prod_ms, 4),
                 "algorithmic_flops_per_launch": nm * flops}
     else:
