@@ -29,9 +29,11 @@ the reading stored in profiles/roofline_traffic.json only when it was taken
 on the current kernel sources (sha256 over crt_bf16.hip and its headers).
 
 --only {plain,split,multiply,crt,batched,batched_plain,batched_split,batched_crt,copy,hier,fp32,
-        traffic}
+        bf16,vendor,traffic}
     runs one kernel family in a loop
-(for profilers) and prints nothing else but its timing line.
+(for profilers) and prints nothing else but its timing line; vendor times the
+vendor library's plain BF16 and INT8 GEMMs (torch.matmul, torch._int_mm) on
+the same GPU as reference points for the residue GEMMs.
 """
 from __future__ import annotations
 
@@ -543,7 +545,7 @@ def main() -> int:
     ap.add_argument("--only", choices=["plain", "plain_sweep", "split", "multiply", "crt", "crt_i8",
                                        "batched",
                                        "batched_plain", "batched_split", "batched_crt", "copy", "hier",
-                                       "fp32", "traffic"])
+                                       "fp32", "bf16", "vendor", "traffic"])
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: --gpus {args.gpus} is not supported. This engine "
@@ -613,6 +615,32 @@ def main() -> int:
         del A, B, C0, C
         ms = time_loop(lambda: capi.sgemm(A32, B32, C32, 1.0, 1.0), args.steps, args.warmup)
         print(json.dumps({"only": "fp32", "ms": ms, "tflops": flops / ms / 1e9}))
+        return 0
+    if args.only == "bf16":
+        # the BF16 -> FP32 GEMM alone (bf16_panel_gemm), C := A*B
+        Ah, Bh = A.to(torch.bfloat16), B.to(torch.bfloat16)
+        C32 = torch.zeros((n, n), device="cuda", dtype=torch.float32).T
+        del A, B, C0, C
+        ms = time_loop(lambda: capi.gemm_bf16(Ah, Bh, C32, 1.0, 0.0), args.steps, args.warmup)
+        print(json.dumps({"only": "bf16", "ms": ms, "tflops": flops / ms / 1e9,
+                          "frac_peak": flops / ms / 1e9 / PEAK_BF16}))
+        return 0
+    if args.only == "vendor":
+        # reference points on the same GPU: the vendor library's plain BF16 GEMM
+        # (torch.matmul, hipBLASLt) and INT8 GEMM (torch._int_mm), n^3
+        del A, B, C0, C
+        Ah = (torch.rand((n, n), generator=gen, device="cuda") * 2 - 1).to(torch.bfloat16)
+        Bh = (torch.rand((n, n), generator=gen, device="cuda") * 2 - 1).to(torch.bfloat16)
+        ms_bf = time_loop(lambda: torch.matmul(Ah, Bh), args.steps, args.warmup)
+        del Ah, Bh
+        Ai = torch.randint(-128, 128, (n, n), generator=gen, device="cuda", dtype=torch.int8)
+        Bi = torch.randint(-128, 128, (n, n), generator=gen, device="cuda", dtype=torch.int8)
+        ms_i8 = time_loop(lambda: torch._int_mm(Ai, Bi), args.steps, args.warmup)
+        print(json.dumps({"only": "vendor",
+                          "bf16_matmul": {"ms": ms_bf, "tflops": flops / ms_bf / 1e9,
+                                          "frac_peak": flops / ms_bf / 1e9 / PEAK_BF16},
+                          "int8_int_mm": {"ms": ms_i8, "tops": flops / ms_i8 / 1e9,
+                                          "frac_peak": flops / ms_i8 / 1e9 / PEAK_INT8}}))
         return 0
     if args.only == "hier":
         del A, B, C0, C

@@ -385,7 +385,7 @@ __device__ __forceinline__ void crt_batch_tile(CrtBatchSmem<N>& sm, const momms_
         CrtSums a;
         crt_sums_init(a);
 #pragma unroll
-        for (int l = 0; l < N; ++l) crt_sums_add(a, (float)rr[l], l, K);
+        for (int l = 0; l < N; ++l) crt_sums_add(a, (int)rr[l], l, K);
         v = alpha * ldexp(crt_i128_to_f64(crt_sums_value(a, K)), ei + ej - 2 * bits);
       }
       Crow[j * ldc] = beta == 0.0 ? v : __builtin_fma(beta, c0, v);

@@ -182,17 +182,17 @@ crt_reconstruct(int64_t m, int64_t n, int64_t k, int beta_bits, double alpha, do
       for (int e = 0; e < kCrtRows; ++e) crt_sums_init(a[e]);
 #pragma unroll
       for (int l = 0; l < N; ++l) {
-        float rr[kCrtRows];
+        int rr[kCrtRows];
         if constexpr (I8) {
           const int8_t* src = (const int8_t*)RCv + (int64_t)l * mpad * npad + j * mpad + i0;
           const uint16_t h = *reinterpret_cast<const uint16_t*>(src);  // two int8 residues
 #pragma unroll
-          for (int e = 0; e < kCrtRows; ++e) rr[e] = (float)(int8_t)(h >> (8 * e));
+          for (int e = 0; e < kCrtRows; ++e) rr[e] = (int8_t)(h >> (8 * e));
         } else {
           const int16_t* src = (const int16_t*)RCv + (int64_t)l * mpad * npad + j * mpad + i0;
           const uint32_t h = *reinterpret_cast<const uint32_t*>(src);  // two int16 residues
 #pragma unroll
-          for (int e = 0; e < kCrtRows; ++e) rr[e] = (float)(int16_t)(h >> (16 * e));
+          for (int e = 0; e < kCrtRows; ++e) rr[e] = (int16_t)(h >> (16 * e));
         }
 #pragma unroll
         for (int e = 0; e < kCrtRows; ++e) crt_sums_add(a[e], rr[e], l, K);

@@ -5,18 +5,23 @@
 // ends of the balanced range (-P/2, P/2).
 //
 // For N moduli p_1..p_N (P = p_1...p_N, M_l = P / p_l, y_l = M_l^-1 mod p_l)
-// and residues r_l of an integer C' with |C'| < P/2 and |C'| < 2^126:
-//   C' = sum_l t_l M_l - q P,  t_l = r_l y_l mod p_l,  q = rint(sum_l t_l / p_l).
-// M_l and P are held as six 24-bit limbs in fp64: with |t_l| < 2^9 every
-// limb sum S_k = sum_l t_l mu_lk (< 2^37) and T_k = S_k - q pi_k is an exact
-// fp64 integer; C' = sum_k T_k 2^(24k) is then assembled modulo 2^128
-// (wrapping), exact because C' fits.
-// sum_l t_l / p_l is within 10^-10 of C'/P + q_true, so q is exact unless
-// |C'| is within 10^-10 P of P/2. That needs P < 2 k 2^(2 beta) (1 + 10^-10),
-// which only an uncapped beta can give. When P < 2^127 (BF16 moduli: N <= 14;
-// INT8 moduli: every N), C' +- P still fits and one compare against
-// floor(P/2) fixes it. Otherwise (BF16, N >= 15) beta is capped, |C'| <=
-// 2^125 < 2^-8 P and q is always exact.
+// and balanced residues r_l (|r_l| <= 255) of an integer C' with |C'| < P/2
+// and |C'| < 2^126:
+//   X = sum_l r_l y_l M_l = C' + q P,  q = rint(X / P) = rint(sum_l r_l y_l / p_l),
+// so C' = X - q P, and it is computed modulo 2^128 (exact, because C' fits in
+// 128 bits signed): u_l = y_l M_l mod 2^128 and P mod 2^128 are held as four
+// 32-bit limbs c_lk, pi_k in fp64. Every limb sum S_k = sum_l r_l c_lk
+// (|r c| < 2^40, N <= 16: |S_k| < 2^44) and T_k = S_k - q pi_k (|q| <= 16 *
+// 255 + 1 < 2^12: |T_k| < 2^45) is an exact fp64 integer, and C' = sum_k T_k
+// 2^(32k) modulo 2^128. Per residue: one conversion, one fma for the
+// quotient sum and four for the limbs (r y is never reduced modulo p).
+// The quotient sum f = sum_l r_l fl(y_l / p_l) is within 10^-11 of X / P =
+// q + C'/P, so q is exact unless |C'| is within 10^-11 P of P/2. That needs
+// P < 2 k 2^(2 beta) (1 + 10^-11), which only an uncapped beta can give.
+// When P < 2^127 (BF16 moduli: N <= 14; INT8 moduli: every N), C' +- P then
+// has magnitude about P/2 and still fits, and one compare against floor(P/2)
+// fixes it. Otherwise (BF16, N >= 15) beta is capped, |C'| <= 2^125 <
+// 2^-8 P and q is always exact.
 #pragma once
 #include <stdint.h>
 
@@ -52,17 +57,14 @@ constexpr int mod_inverse(int a, int m) {  // extended Euclid, gcd(a, m) = 1
   return t < 0 ? t + m : t;
 }
 
-constexpr int kCrtLimbs = 6;  // 24-bit limbs: 144 bits hold M_l and P for N <= 16
+constexpr int kCrtLimbs = 4;  // 32-bit limbs: everything is computed modulo 2^128
 
 struct CrtRecon {
-  double mu[kCrtMaxModuli][kCrtLimbs];                 // 24-bit limbs of M_l
-  double pi[kCrtLimbs];                                // 24-bit limbs of P
-  uint64_t p_lo, p_hi, h_lo, h_hi;                     // P and floor(P/2) mod 2^128
-  float y[kCrtMaxModuli];                              // M_l^-1 mod p_l
-  float pf[kCrtMaxModuli], invpf[kCrtMaxModuli];      // p_l, fp32(1/p_l)
-  double invpd[kCrtMaxModuli];                         // fp64(1/p_l)
-  int fix;                                             // P < 2^127: the +-P fix
-  int limbs;                                           // 24-bit limbs that P (and so every M_l) needs
+  double c[kCrtMaxModuli][kCrtLimbs];  // 32-bit limbs of y_l M_l mod 2^128
+  double pi[kCrtLimbs];                // 32-bit limbs of P mod 2^128
+  double yp[kCrtMaxModuli];            // fp64(y_l / p_l)
+  uint64_t p_lo, p_hi, h_lo, h_hi;     // P and floor(P/2) mod 2^128
+  int fix;                             // P < 2^127: the +-P fix
 };
 
 // Small unsigned big integers (32-bit limbs) for the constants.
@@ -87,12 +89,7 @@ constexpr uint32_t crt_big_div(const CrtBig& a, uint32_t b, CrtBig& q) {  // ret
   return (uint32_t)rem;
 }
 
-constexpr double crt_big_limb24(const CrtBig& a, int k) {  // bits 24k .. 24k+23
-  const int bit = 24 * k, w = bit / 32, sh = bit % 32;
-  uint64_t v = a.w[w] >> sh;
-  if (sh > 8 && w + 1 < 6) v |= (uint64_t)a.w[w + 1] << (32 - sh);
-  return (double)(v & 0xFFFFFF);
-}
+constexpr double crt_big_limb32(const CrtBig& a, int k) { return (double)a.w[k]; }
 
 constexpr uint64_t crt_big_lo(const CrtBig& a) { return (uint64_t)a.w[0] | ((uint64_t)a.w[1] << 32); }
 constexpr uint64_t crt_big_hi(const CrtBig& a) { return (uint64_t)a.w[2] | ((uint64_t)a.w[3] << 32); }
@@ -109,8 +106,7 @@ constexpr int crt_big_bitlen(const CrtBig& a) {
 
 // The constants for the first N moduli of a table (kCrtModuli or
 // kCrtI8Moduli); constexpr, so the kernel for a given N holds them as
-// instruction literals (a 24-bit limb as a double has a zero low word: one
-// 32-bit literal).
+// instruction literals.
 constexpr CrtRecon crt_recon_constants(int N, const int* mods = kCrtModuli) {
   CrtRecon K{};
   CrtBig P;
@@ -120,15 +116,14 @@ constexpr CrtRecon crt_recon_constants(int N, const int* mods = kCrtModuli) {
     CrtBig M, unused;
     (void)crt_big_div(P, (uint32_t)mods[l], M);
     const uint32_t mr = crt_big_div(M, (uint32_t)mods[l], unused);
-    for (int k = 0; k < kCrtLimbs; ++k) K.mu[l][k] = crt_big_limb24(M, k);
-    K.y[l] = (float)mod_inverse((int)mr, mods[l]);
-    K.pf[l] = (float)mods[l];
-    K.invpf[l] = 1.0f / (float)mods[l];
-    K.invpd[l] = 1.0 / (double)mods[l];
+    const int y = mod_inverse((int)mr, mods[l]);
+    crt_big_mul(M, (uint32_t)y);  // y_l M_l < 2^9 P < 2^153: within the 192 bits
+    for (int k = 0; k < kCrtLimbs; ++k) K.c[l][k] = crt_big_limb32(M, k);
+    K.yp[l] = (double)y / (double)mods[l];
   }
   CrtBig H;
   (void)crt_big_div(P, 2, H);
-  for (int k = 0; k < kCrtLimbs; ++k) K.pi[k] = crt_big_limb24(P, k);
+  for (int k = 0; k < kCrtLimbs; ++k) K.pi[k] = crt_big_limb32(P, k);
   K.p_lo = crt_big_lo(P);
   K.p_hi = crt_big_hi(P);
   K.h_lo = crt_big_lo(H);
@@ -137,8 +132,6 @@ constexpr CrtRecon crt_recon_constants(int N, const int* mods = kCrtModuli) {
   // (BF16 moduli: N <= 14; above, beta is capped and q is always exact;
   // INT8 moduli: every N <= 16)
   K.fix = crt_big_bitlen(P) <= 127 ? 1 : 0;
-  K.limbs = kCrtLimbs;  // the higher limbs of P are 0: fewer suffice, and M_l < P
-  while (K.limbs > 1 && K.pi[K.limbs - 1] == 0.0) --K.limbs;
   return K;
 }
 
@@ -146,8 +139,7 @@ MOMMS_HD inline unsigned __int128 crt_u128(uint64_t lo, uint64_t hi) {
   return ((unsigned __int128)hi << 64) | lo;
 }
 
-// C' from its balanced residues r[0..N-1] (integers in fp32).
-// Running sums of one entry: f = sum_l t_l / p_l and the limb sums S_k.
+// Running sums of one entry: f = sum_l r_l y_l / p_l and the limb sums S_k.
 struct CrtSums {
   double f, S[kCrtLimbs];
 };
@@ -158,27 +150,24 @@ MOMMS_HD inline void crt_sums_init(CrtSums& a) {
   for (int k = 0; k < kCrtLimbs; ++k) a.S[k] = 0.0;
 }
 
-// Adds the balanced residue r (an integer in fp32) modulo p_l.
-MOMMS_HD inline void crt_sums_add(CrtSums& a, float r, int l, const CrtRecon& K) {
-  // t = r y mod p: |r y| < 2^17 is exact in fp32; |t| < 0.51 p < 2^9
-  float t = r * K.y[l];
-  t = __builtin_fmaf(-__builtin_rintf(t * K.invpf[l]), K.pf[l], t);
-  const double td = (double)t;
-  a.f = __builtin_fma(td, K.invpd[l], a.f);
+// Adds the balanced residue r (an integer, |r| <= 255, as int or as an
+// integer-valued float) modulo p_l.
+template <typename R>
+MOMMS_HD inline void crt_sums_add(CrtSums& a, R r, int l, const CrtRecon& K) {
+  const double rd = (double)r;
+  a.f = __builtin_fma(rd, K.yp[l], a.f);
 #pragma unroll
-  for (int k = 0; k < kCrtLimbs; ++k)
-    if (k < K.limbs) a.S[k] = __builtin_fma(td, K.mu[l][k], a.S[k]);
+  for (int k = 0; k < kCrtLimbs; ++k) a.S[k] = __builtin_fma(rd, K.c[l][k], a.S[k]);
 }
 
 // C' from the sums over all N residues.
 MOMMS_HD inline __int128 crt_sums_value(const CrtSums& a, const CrtRecon& K) {
-  const double q = __builtin_rint(a.f);  // |q| <= 9
+  const double q = __builtin_rint(a.f);  // |q| < 2^12
   unsigned __int128 x = 0;
 #pragma unroll
   for (int k = 0; k < kCrtLimbs; ++k) {
-    if (k >= K.limbs) break;  // limbs of P and of every M_l above are 0
-    const double T = __builtin_fma(-q, K.pi[k], a.S[k]);  // exact integer, |T| < 2^38
-    x += (unsigned __int128)(__int128)(int64_t)T << (24 * k);
+    const double T = __builtin_fma(-q, K.pi[k], a.S[k]);  // exact integer, |T| < 2^45
+    x += (unsigned __int128)(__int128)(int64_t)T << (32 * k);
   }
   __int128 c = (__int128)x;
   if (K.fix) {
