@@ -113,14 +113,17 @@ crt_pack(const double* __restrict__ X, int64_t count, int64_t len, int64_t sv, i
     uint16_t* dst = R + ((pv * panel_stride(nslabs) + ks) * (2 * PANEL) + (int64_t)kh * PANEL +
                          (gv % PANEL)) * 8;
     for (int l = 0; l < nmod; ++l) {
-      const double p = T.pd[l], invp = T.invpd[l], h = 0.5 * (p - 1.0);
+      const double p = T.pd[l], invp = T.invpd[l];
+      const float pf = T.pf[l], invpf = T.invpf[l];
       uint32_t bits[G];
 #pragma unroll
       for (int e = 0; e < G; ++e) {
-        // q - p*rint(q/p) is an integer below 0.51 p in magnitude: exact by fma
-        double rr = __builtin_fma(-rint(q[e] * invp), p, q[e]);
-        rr = rr > h ? rr - p : rr;
-        rr = rr < -h ? rr + p : rr;
+        // x = q - p*rint(q/p) is an integer below 0.51 p in magnitude (exact
+        // by fma); the same step once more in fp32 on x (|x| < 2^9, and x/p
+        // is at least 1/(2p) away from +-1/2, so rint(x/p) is exact) gives the
+        // balanced residue, |rr| <= (p-1)/2
+        const float x = (float)__builtin_fma(-rint(q[e] * invp), p, q[e]);
+        const float rr = __builtin_fmaf(-__builtin_rintf(x * invpf), pf, x);
         // BF16: |rr| <= 255 is exact in bf16 (the high half of the float);
         // INT8: |rr| <= 127, its two's complement byte
         bits[e] = I8 ? ((uint32_t)(int)rr & 0xFFu) : (__float_as_uint((float)rr) >> 16);
