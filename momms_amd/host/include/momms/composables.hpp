@@ -5,6 +5,7 @@
 //   static std::vector<AlgorithmStep> hierarchy_description(); (innermost step first)
 // and is default-constructible. Nodes here:
 //   PartM/PartN/PartK<Bsz, Child>   src/composables/part.rs:7-120
+//   FirstDiffPartM/N/K<Bsz, Child1, Child2>  src/composables/part.rs:122-264
 //   ParallelM/ParallelN<Iota, Nway, Child>, Target<N>, TheRest
 //                                   src/composables/parallel_range.rs:8-163
 //   SpawnThreads<Child>             src/composables/spawn.rs:37-145
@@ -95,6 +96,86 @@ struct PartK {
   }
   static std::vector<AlgorithmStep> hierarchy_description() {
     auto d = Child::hierarchy_description();
+    d.push_back({AlgorithmStep::K, Bsz});
+    return d;
+  }
+};
+
+// The loops of PartM/PartN/PartK with a different child for the first block
+// (src/composables/part.rs:122-264): the overlap trees give the first M
+// block the leaf that packs B as it multiplies and the later blocks the one
+// that reads the packed B. The description is Child1's plus this step.
+template <size_t Bsz, class Child1, class Child2>
+struct FirstDiffPartM {
+  Child1 child1;
+  Child2 child2;
+  template <class At, class Bt, class Ct>
+  void run(At& a, Bt& b, Ct& c, const ThreadInfo& thr) {
+    const size_t m_save = a.push_y_view(Bsz);
+    c.push_y_view(Bsz);
+    for (size_t i = 0; i < m_save; i += Bsz) {
+      a.slide_y_view_to(i, Bsz);
+      c.slide_y_view_to(i, Bsz);
+      if (i == 0) child1.run(a, b, c, thr);
+      else child2.run(a, b, c, thr);
+    }
+    a.pop_y_view();
+    c.pop_y_view();
+  }
+  static std::vector<AlgorithmStep> hierarchy_description() {
+    auto d = Child1::hierarchy_description();
+    d.push_back({AlgorithmStep::M, Bsz});
+    return d;
+  }
+};
+
+template <size_t Bsz, class Child1, class Child2>
+struct FirstDiffPartN {
+  Child1 child1;
+  Child2 child2;
+  template <class At, class Bt, class Ct>
+  void run(At& a, Bt& b, Ct& c, const ThreadInfo& thr) {
+    const size_t n_save = b.push_x_view(Bsz);
+    c.push_x_view(Bsz);
+    for (size_t i = 0; i < n_save; i += Bsz) {
+      b.slide_x_view_to(i, Bsz);
+      c.slide_x_view_to(i, Bsz);
+      if (i == 0) child1.run(a, b, c, thr);
+      else child2.run(a, b, c, thr);
+    }
+    b.pop_x_view();
+    c.pop_x_view();
+  }
+  static std::vector<AlgorithmStep> hierarchy_description() {
+    auto d = Child1::hierarchy_description();
+    d.push_back({AlgorithmStep::N, Bsz});
+    return d;
+  }
+};
+
+// As PartK, C's scalar becomes 1 after the first block and is restored.
+template <size_t Bsz, class Child1, class Child2>
+struct FirstDiffPartK {
+  Child1 child1;
+  Child2 child2;
+  template <class At, class Bt, class Ct>
+  void run(At& a, Bt& b, Ct& c, const ThreadInfo& thr) {
+    const size_t k_save = a.push_x_view(Bsz);
+    b.push_y_view(Bsz);
+    const auto beta_save = c.get_scalar();
+    for (size_t i = 0; i < k_save; i += Bsz) {
+      a.slide_x_view_to(i, Bsz);
+      b.slide_y_view_to(i, Bsz);
+      if (i == 0) child1.run(a, b, c, thr);
+      else child2.run(a, b, c, thr);
+      c.set_scalar(1);
+    }
+    a.pop_x_view();
+    b.pop_y_view();
+    c.set_scalar(beta_save);
+  }
+  static std::vector<AlgorithmStep> hierarchy_description() {
+    auto d = Child1::hierarchy_description();
     d.push_back({AlgorithmStep::K, Bsz});
     return d;
   }

@@ -16,6 +16,9 @@
 //   UnpackC<LH,LW,LRS,LCS, Child>  Child computes alpha*A*B into a zeroed
 //                            HierMatrix, which is then added to C
 //                            (src/composables/unpack.rs:214-270)
+//   DelayedPackA/B<LH,LW,LRS,LCS, Child>, PackPair, UnpairA/B
+//                            packing left to the first multiply, for the
+//                            overlap trees (src/composables/fused_pack.rs)
 //   HipGemmHier<Desc>        leaf on three HierMatrix operands
 //                            (momms_hip_dgemm_hier)
 //   KnmBlock<Mr, Nr>         the KNM f32 tree's micro-kernel loops over its
@@ -176,6 +179,109 @@ struct PackB {
   static std::vector<AlgorithmStep> hierarchy_description() { return Child::hierarchy_description(); }
 };
 
+// ---------------------------------------------------------------------------
+// Packing that waits for the first multiply (src/composables/fused_pack.rs,
+// src/matrix/pack_pair.rs): DelayedPackA/B lay the pack buffer out for the
+// current view but do not fill it; the operand travels down as a PackPair of
+// the view and its buffer. The first leaf that multiplies with the pair fills
+// the buffer (the reference packs each leaf panel when its micro-kernel first
+// reaches it, pack_pair.rs establish_leaf); UnpairA/B pass the filled buffer
+// on, so the nodes below them read the packed operand. The overlap tree
+// (src/exper/overlap.rs:71-79) is
+//   PartN<Nc, PartK<Kc, DelayedPackB<.., FirstDiffPartM<Mc, Leaf, UnpairB<Leaf>>>>>:
+// the first M block multiplies while B is packed, the others reuse it.
+template <class Xt, class T = double>
+struct PackPair {
+  Xt& x;                   // the operand's current view
+  HierMatrixT<T>& packed;  // its pack buffer, laid out for that view
+  hipStream_t stream;
+  detail::DevBuf& stage;
+  bool filled = false;
+  size_t height() const { return x.height(); }
+  size_t width() const { return x.width(); }
+  T get_scalar() const { return x.get_scalar(); }
+  // The packed operand, filled on first use (thread 0 of a team only, as
+  // every device call here).
+  HierMatrixT<T>& establish() {
+    if (!filled) {
+      detail::pack_view<T>(x, x.height(), x.width(), packed, stage, stream);
+      packed.set_scalar(x.get_scalar());
+      filled = true;
+    }
+    return packed;
+  }
+};
+
+namespace detail {
+template <class T>
+HierMatrixT<T>& established(HierMatrixT<T>& x) { return x; }
+template <class Xt, class T>
+HierMatrixT<T>& established(PackPair<Xt, T>& p) { return p.establish(); }
+}  // namespace detail
+
+template <size_t LH, size_t LW, size_t LRS, size_t LCS, class Child, class T = double>
+struct DelayedPackA {
+  Child child;
+  HierMatrixT<T> a_pack;
+  detail::DevBuf stage;
+  hipStream_t stream = nullptr;
+  template <class At, class Bt, class Ct>
+  void run(At& a, Bt& b, Ct& c, const ThreadInfo& thr) {
+    if (thr.thread_id() == 0)
+      a_pack.resize_to(a.height(), a.width(), Child::hierarchy_description(), MOMMS_STEP_M,
+                       MOMMS_STEP_K, LH, LW, LRS, LCS);
+    if (thr.num_threads() > 1) thr.barrier();
+    PackPair<At, T> pair{a, a_pack, stream, stage};
+    child.run(pair, b, c, thr);
+  }
+  static std::vector<AlgorithmStep> hierarchy_description() { return Child::hierarchy_description(); }
+};
+
+template <size_t LH, size_t LW, size_t LRS, size_t LCS, class Child, class T = double>
+struct DelayedPackB {
+  Child child;
+  HierMatrixT<T> b_pack;
+  detail::DevBuf stage;
+  hipStream_t stream = nullptr;
+  template <class At, class Bt, class Ct>
+  void run(At& a, Bt& b, Ct& c, const ThreadInfo& thr) {
+    if (thr.thread_id() == 0)
+      b_pack.resize_to(b.height(), b.width(), Child::hierarchy_description(), MOMMS_STEP_K,
+                       MOMMS_STEP_N, LH, LW, LRS, LCS);
+    if (thr.num_threads() > 1) thr.barrier();
+    PackPair<Bt, T> pair{b, b_pack, stream, stage};
+    child.run(a, pair, c, thr);
+  }
+  static std::vector<AlgorithmStep> hierarchy_description() { return Child::hierarchy_description(); }
+};
+
+// The packed side of a pair for the nodes below (fused_pack.rs UnpairA/B):
+// the buffer is filled here if no leaf has done it yet, on thread 0, and the
+// team meets before the children read it.
+template <class Child>
+struct UnpairA {
+  Child child;
+  template <class Xt, class T, class Bt, class Ct>
+  void run(PackPair<Xt, T>& a, Bt& b, Ct& c, const ThreadInfo& thr) {
+    if (thr.thread_id() == 0) a.establish();
+    if (thr.num_threads() > 1) thr.barrier();
+    child.run(a.packed, b, c, thr);
+  }
+  static std::vector<AlgorithmStep> hierarchy_description() { return Child::hierarchy_description(); }
+};
+
+template <class Child>
+struct UnpairB {
+  Child child;
+  template <class At, class Xt, class T, class Ct>
+  void run(At& a, PackPair<Xt, T>& b, Ct& c, const ThreadInfo& thr) {
+    if (thr.thread_id() == 0) b.establish();
+    if (thr.num_threads() > 1) thr.barrier();
+    child.run(a, b.packed, c, thr);
+  }
+  static std::vector<AlgorithmStep> hierarchy_description() { return Child::hierarchy_description(); }
+};
+
 // C := C + alpha*A*B: Child writes alpha*A*B into a hierarchical buffer with
 // scalar 0 (not read), whose elements are then added to C's (Unpacker::add,
 // unpack.rs:61-98 and :174-213; C's own scalar is not applied, as there).
@@ -212,11 +318,16 @@ struct UnpackC {
 
 // GemmNode<f64, Hierarch, Hierarch, Hierarch> on the GPU (the l3
 // experiment's trees on hierarchical operands, src/exper/l3.rs:46-72).
+// An operand may also come as a PackPair of a DelayedPackA/B above: the leaf
+// fills its buffer first (the packing of the overlap trees' first block).
 template <class Desc>
 struct HipGemmHier {
   hipStream_t stream = nullptr;
-  void run(HierMatrix& a, HierMatrix& b, HierMatrix& c, const ThreadInfo& thr) {
+  template <class At, class Bt>
+  void run(At& a_in, Bt& b_in, HierMatrix& c, const ThreadInfo& thr) {
     if (thr.thread_id() == 0) {
+      HierMatrix& a = detail::established(a_in);
+      HierMatrix& b = detail::established(b_in);
       const double alpha = a.get_scalar() * b.get_scalar();
       const double beta = c.get_scalar();
       abi_check("momms_hip_dgemm_hier",

@@ -43,6 +43,19 @@ static bool same_as_triple_loop(size_t m, size_t n, size_t k, size_t threads, bo
   return true;
 }
 
+// TripleLoop that counts its calls (which child of a FirstDiff node ran).
+template <int Id>
+struct CountedLoop {
+  static inline int calls = 0;
+  TripleLoop loop;
+  template <class At, class Bt, class Ct>
+  void run(At& a, Bt& b, Ct& c, const ThreadInfo& thr) {
+    if (thr.thread_id() == 0) ++calls;
+    loop.run(a, b, c, thr);
+  }
+  static std::vector<AlgorithmStep> hierarchy_description() { return {}; }
+};
+
 int main() {
   // --- views (src/matrix/view.rs, general_stride.rs) ---
   {
@@ -116,6 +129,28 @@ int main() {
   CHECK((same_as_triple_loop<PartK<16, ParallelN<8, TheRest, TripleLoop>>>(20, 70, 33, 4, false)));
   CHECK((same_as_triple_loop<ParallelM<6, Target<2>, ParallelN<8, TheRest, Barrier<TripleLoop>>>>(
       50, 41, 9, 4, false)));
+  // --- FirstDiffPartM/N/K (part.rs:122-264): the first block takes Child1,
+  // the others Child2, the loops and the beta rule as PartM/N/K ---
+  {
+    using M = FirstDiffPartM<5, CountedLoop<1>, CountedLoop<2>>;
+    CHECK((same_as_triple_loop<M>(37, 29, 23, 1, false)));
+    CHECK(CountedLoop<1>::calls == 1 && CountedLoop<2>::calls == 7);  // ceil(37/5) blocks
+    using N = FirstDiffPartN<13, CountedLoop<3>, CountedLoop<4>>;
+    CHECK((same_as_triple_loop<N>(37, 29, 23, 1, true)));
+    CHECK(CountedLoop<3>::calls == 1 && CountedLoop<4>::calls == 2);  // ceil(29/13)
+    using K = FirstDiffPartK<7, CountedLoop<5>, CountedLoop<6>>;
+    CHECK((same_as_triple_loop<K>(37, 29, 23, 1, false)));
+    CHECK(CountedLoop<5>::calls == 1 && CountedLoop<6>::calls == 3);  // ceil(23/7)
+    CHECK((same_as_triple_loop<PartN<13, FirstDiffPartK<7, PartM<5, TripleLoop>,
+                                                        FirstDiffPartM<5, TripleLoop, TripleLoop>>>>(
+        40, 31, 30, 1, false)));
+    CHECK((same_as_triple_loop<ParallelN<8, TheRest, FirstDiffPartK<7, PartM<5, TripleLoop>,
+                                                                    PartM<5, TripleLoop>>>>(
+        40, 31, 30, 3, false)));
+    auto d = FirstDiffPartM<120, PartK<192, TripleLoop>, TripleLoop>::hierarchy_description();
+    CHECK(d.size() == 2 && d[0].kind == AlgorithmStep::K && d[1].kind == AlgorithmStep::M &&
+          d[1].bsz == 120);
+  }
   std::printf("%s (%d failures)\n", fails ? "FAIL" : "PASS", fails);
   return fails ? 1 : 0;
 }

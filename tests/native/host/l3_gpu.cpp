@@ -15,6 +15,9 @@
 //   SpawnThreads<PartM<PartN<Barrier<GpuSchedule>>>>  out of core: host blocks
 //                                            streamed through the GPU
 //   UnpackC<PackB<PackA<HipGemmHier>>>       device Hierarch operands
+//   PartN<PartK<DelayedPackB<FirstDiffPartM<Bottom, UnpairB<Bottom>>>>>
+//                                            the overlap tree: B packed by the
+//                                            first M block's leaf
 //   PartN<PartK<PackB<PartM<PackA<KnmBlock>>>>> the KNM f32 tree
 //   HipXsmm, PartM<PartN<PartK<HipXsmm>>>    the small-GEMM leaf
 // Tolerance: |C - C_ref| <= 2*gamma_{k+1}*(|A||B| + |C0|) componentwise, with
@@ -56,6 +59,13 @@ using GotoHier = UnpackC<4, 12, 12, 1, PackB<192, 12, 12, 1, PackA<4, 192, 1, 4,
 // micro-kernel loops of each block on FP32 matrix cores
 using KnmTree = PartN<14400, PartK<336, PackB<4, 24, 1, 4, PartM<160, PackA<16, 4, 1, 16,
                 KnmBlock<16, 24>, float>>, float>>>;
+// the overlap tree (overlap.rs:71-79, MR 4, NR 12, over the l3 experiment's
+// hierarchical operands): DelayedPackB lays B's pack out per (N, K) block,
+// the first M block's leaf fills it as it multiplies, the later M blocks
+// read it through UnpairB
+using OverlapBottom = UnpackC<4, 12, 12, 1, PackA<4, 192, 1, 4, HipGemmHier<GotoDesc>>>;
+using GotoOverlap = SpawnThreads<PartN<3000, PartK<192, DelayedPackB<192, 12, 12, 1,
+                    FirstDiffPartM<120, OverlapBottom, UnpairB<OverlapBottom>>>>>>;
 // small blocks straight from the operands (the libxsmm leaves)
 using XsmmTree = PartM<96, PartN<64, PartK<128, HipXsmm>>>;
 
@@ -150,6 +160,8 @@ int main() {
                                      2.0, false);
   check_tree<KnmTree, float>("KNM f32 tree (PackB/PackA Hierarch<f32>)", 500, 420, 700, 1, 2.0,
                              false);
+  check_tree<GotoOverlap>("SpawnThreads<PartN<PartK<DelayedPackB<FirstDiffPartM<..UnpairB>>>>> x2",
+                          700, 3100, 450, 2, 2.0, false);
   check_tree<HipXsmm>("HipXsmm (device matrices)", 90, 70, 50, 1, 2.0, true);
   check_tree<XsmmTree>("PartM<96,PartN<64,PartK<128,HipXsmm>>>", 300, 200, 260, 1, 2.0, true);
   std::printf("%s (%d failures)\n", fails ? "FAIL" : "PASS", fails);
