@@ -17,7 +17,8 @@
 //   UnpackC<PackB<PackA<HipGemmHier>>>       device Hierarch operands
 //   PartN<PartK<DelayedPackB<FirstDiffPartM<Bottom, UnpairB<Bottom>>>>>
 //                                            the overlap tree: B packed by the
-//                                            first M block's leaf
+//                                            first M block's leaf (and A by the
+//                                            first N block's, DelayedPackA)
 //   PartN<PartK<PackB<PartM<PackA<KnmBlock>>>>> the KNM f32 tree
 //   HipXsmm, PartM<PartN<PartK<HipXsmm>>>    the small-GEMM leaf
 // Tolerance: |C - C_ref| <= 2*gamma_{k+1}*(|A||B| + |C0|) componentwise, with
@@ -66,6 +67,10 @@ using KnmTree = PartN<14400, PartK<336, PackB<4, 24, 1, 4, PartM<160, PackA<16, 
 using OverlapBottom = UnpackC<4, 12, 12, 1, PackA<4, 192, 1, 4, HipGemmHier<GotoDesc>>>;
 using GotoOverlap = SpawnThreads<PartN<3000, PartK<192, DelayedPackB<192, 12, 12, 1,
                     FirstDiffPartM<120, OverlapBottom, UnpairB<OverlapBottom>>>>>>;
+// the mirror image: A's pack delayed to the first N block's leaf
+using OverlapBottomA = UnpackC<4, 12, 12, 1, PackB<192, 12, 12, 1, HipGemmHier<GotoDesc>>>;
+using GotoOverlapA = PartM<600, PartK<192, DelayedPackA<4, 192, 1, 4,
+                     FirstDiffPartN<500, OverlapBottomA, UnpairA<OverlapBottomA>>>>>;
 // small blocks straight from the operands (the libxsmm leaves)
 using XsmmTree = PartM<96, PartN<64, PartK<128, HipXsmm>>>;
 
@@ -162,6 +167,8 @@ int main() {
                              false);
   check_tree<GotoOverlap>("SpawnThreads<PartN<PartK<DelayedPackB<FirstDiffPartM<..UnpairB>>>>> x2",
                           700, 3100, 450, 2, 2.0, false);
+  check_tree<GotoOverlapA>("PartM<PartK<DelayedPackA<FirstDiffPartN<..UnpairA>>>>", 900, 1300, 400,
+                           1, 2.0, false);
   check_tree<HipXsmm>("HipXsmm (device matrices)", 90, 70, 50, 1, 2.0, true);
   check_tree<XsmmTree>("PartM<96,PartN<64,PartK<128,HipXsmm>>>", 300, 200, 260, 1, 2.0, true);
   std::printf("%s (%d failures)\n", fails ? "FAIL" : "PASS", fails);

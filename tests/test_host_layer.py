@@ -39,4 +39,4 @@ def test_host_selftest_under_thread_sanitizer():
 @pytest.mark.gpu
 def test_hip_nodes_in_gemmnode_trees():
     out = _run("l3_gpu", 300)
-    assert out.count("PASS") == 23 and "FAIL" not in out
+    assert out.count("PASS") == 24 and "FAIL" not in out
