@@ -19,6 +19,9 @@ moduli, N residue GEMMs). Prints ONE JSON line:
   configs      the other configurations' lines (sweep, slices/moduli x
                exponent spread, FP32/BF16, batched, hierarchical layout
                pack / unpack-add) unless --quick
+  mfma_ceiling the same GPU's BF16 / INT8 matrix-core rate with operands in
+               registers (random: the power-limited ceiling; zero: full
+               clock), and the residue GEMMs' fractions of it
   cpu_baseline the restated Goto + 6x8 CPU path on this host's cores
 
 roofline.traffic (HBM bytes per launch of crt_multiply) is measured in the
@@ -33,7 +36,8 @@ on the current kernel sources (sha256 over crt_bf16.hip and its headers).
     runs one kernel family in a loop
 (for profilers) and prints nothing else but its timing line; vendor times the
 vendor library's plain BF16 and INT8 GEMMs (torch.matmul, torch._int_mm) on
-the same GPU as reference points for the residue GEMMs.
+the same GPU as reference points for the residue GEMMs, and its FP64 GEMM
+(torch.addmm) beside this library's over the plain sweep's shapes.
 """
 from __future__ import annotations
 
@@ -177,9 +181,9 @@ def _best_of(run, c0, reps: int, flush) -> float:
 def cpu_baseline(a, b, c0, a1, b1, c1, threads: int, info: dict) -> dict:
     """The restated reference path (Goto tree, Nc/Kc/Mc = 3000/192/120, 6x8
     FMA micro-kernel, one pinned thread per physical core) on the same inputs
-    as the GPU legs: the 8192^3 workload's A with the first n columns of B and
-    C (a bounded sample), and the m=n=k=1024 configuration; all threads and
-    one thread; best of 5 (1024^3 at one thread: best of 3)."""
+    as the GPU legs: the 8192^3 workload (A with the first n columns of B and
+    C, all of them by default), and the m=n=k=1024 configuration; all threads
+    and one thread; best of 5 (1024^3 at one thread: best of 3)."""
     import numpy as np
 
     sys.path.insert(0, str(ROOT / "oracle"))
@@ -203,10 +207,11 @@ def cpu_baseline(a, b, c0, a1, b1, c1, threads: int, info: dict) -> dict:
     # (its boost clock can be higher: the one-thread rate is the better yardstick)
     peak_core = 16.0 * info["max_mhz"] / 1e3 if info["max_mhz"] else None
     return {"value": round(tf, 4), "unit": "TFLOP/s", "cores": threads, "kind": "port",
-            "sample": f"plain FP64 Goto+6x8 GEMM (the reference path restated in C) on the "
-                      f"8192^3 workload's A and its first {n} of 8192 columns of B and C "
-                      f"(m={m} n={n} k={k}), {threads} threads pinned one per physical core, "
-                      f"best of 5 with fresh C and a 256 MiB flush, {t_all:.2f} s",
+            "sample": f"plain FP64 Goto+6x8 GEMM (the reference path restated in C) on "
+                      + ("the whole workload's A, B and C" if n == m else
+                         f"the workload's A and its first {n} of {m} columns of B and C")
+                      + f" (m={m} n={n} k={k}), {threads} threads pinned one per physical "
+                      f"core, best of 5 with fresh C and a 256 MiB flush, {t_all:.2f} s a run",
             "cpu_model": info["model"], "physical_cores": info["physical_cores"],
             "logical_cpus": info["logical_cpus"], "cgroup_cpu_share": info["cpu_share"],
             "gflops_per_core": round(gf_core, 2),
@@ -524,6 +529,25 @@ def measure_traffic(n: int, moduli: int, limit: int = 180) -> dict:
             "write_factor": round(wf, 4), "traffic_bytes": int(round((fetch * ff + write * wf) * 1024))}
 
 
+def mfma_ceiling(limit: int = 60) -> dict:
+    """The matrix cores' sustained rate on this GPU with the operands in
+    registers (tests/native/mfma_ceiling.hip, run as a child process): the
+    BF16 and INT8 MFMA forms of the residue GEMMs with random operands, which
+    bound any GEMM kernel at the clock the chip holds under full matrix-core
+    load, and with zero operands, which run at the full clock."""
+    import subprocess
+
+    exe = ROOT / "tests" / "native" / "bin" / "mfma_ceiling"
+    if not exe.exists():
+        raise RuntimeError(f"{exe} is not built")
+    r = subprocess.run(["timeout", "-k", "5", str(limit), str(exe)], stdout=subprocess.PIPE,
+                       stderr=subprocess.STDOUT, timeout=limit + 15)
+    text = r.stdout.decode(errors="replace").strip()
+    if r.returncode != 0 or not text.startswith("{"):
+        raise RuntimeError(f"mfma_ceiling exited {r.returncode}: {text[-300:]}")
+    return json.loads(text.splitlines()[-1])
+
+
 def main() -> int:
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -636,7 +660,23 @@ def main() -> int:
         Ai = torch.randint(-128, 128, (n, n), generator=gen, device="cuda", dtype=torch.int8)
         Bi = torch.randint(-128, 128, (n, n), generator=gen, device="cuda", dtype=torch.int8)
         ms_i8 = time_loop(lambda: torch._int_mm(Ai, Bi), args.steps, args.warmup)
-        print(json.dumps({"only": "vendor",
+        del Ai, Bi
+        # the vendor library's FP64 GEMM (torch.addmm on column-major operands,
+        # C := C + A*B) over the plain sweep's shapes, beside this library's
+        fp64 = []
+        for m, nn, k in [(s, s, s) for s in (512, 1024, 2048, 8192)] + [(8192, 8192, 256),
+                                                                         (8192, 64, 8192)]:
+            A, B, C = rand_colmajor(m, k, gen), rand_colmajor(k, nn, gen), rand_colmajor(m, nn, gen)
+            f = 2.0 * m * nn * k
+            ms_v = time_loop(lambda: torch.addmm(C, A, B), max(args.steps, 20), args.warmup)
+            ms_o = time_loop(lambda: capi.dgemm(A, B, C, 1.0, 1.0), max(args.steps, 20), args.warmup)
+            fp64.append({"m": m, "n": nn, "k": k,
+                         "vendor_ms": round(ms_v, 4),
+                         "vendor_frac_fp64_peak": round(f / ms_v / 1e9 / PEAK_FP64, 4),
+                         "momms_ms": round(ms_o, 4),
+                         "momms_frac_fp64_peak": round(f / ms_o / 1e9 / PEAK_FP64, 4)})
+            del A, B, C
+        print(json.dumps({"only": "vendor", "fp64_addmm": fp64,
                           "bf16_matmul": {"ms": ms_bf, "tflops": flops / ms_bf / 1e9,
                                           "frac_peak": flops / ms_bf / 1e9 / PEAK_BF16},
                           "int8_int_mm": {"ms": ms_i8, "tops": flops / ms_i8 / 1e9,
@@ -879,12 +919,27 @@ def main() -> int:
         "accuracy_rule": f"max scaled and max relative error over {CHECK_SAMPLES} sampled "
                          "entries (double-double reference) within 2x plain FP64's",
     }
+    # the same GPU's matrix-core rate with operands in registers, measured now:
+    # the residue GEMMs against what the chip sustains at its power limit
+    log("bench: matrix-core ceiling with operands in registers (BF16, INT8)")
+    try:
+        ceil = mfma_ceiling()
+        bf_r, i8_r = ceil["bf16_32x32x16"]["random_tflops"], ceil["i8_32x32x32"]["random_tops"]
+        ceil["crt_multiply_frac_of_bf16_random"] = round(nm * flops / prod_ms / 1e9 / bf_r, 4)
+        ceil["crt_i8_products_frac_of_i8_random"] = round(n8 * flops / i8_prod_ms / 1e9 / i8_r, 4)
+        ceil["note"] = ("MFMA chains with operands in registers on every CU (no memory traffic): "
+                        "random operands give the rate the chip sustains at its power limit, "
+                        "zero operands the full-clock rate")
+        result["mfma_ceiling"] = ceil
+    except Exception as exc:  # a reported reference point, not part of the metric
+        log(f"bench: matrix-core ceiling not measured: {exc}")
+        result["mfma_ceiling"] = {"error": str(exc)}
     cpu_inputs, c1_gpu_tf = None, None
     if not args.no_cpu:
-        # host copies for the CPU leg: the workload's A, and B and C0 cut to
-        # their first 1024 columns (A, B, C0 are transposed views of
-        # row-major tensors, so .T is contiguous)
-        ncols = 1024
+        # host copies for the CPU leg: the workload's A, B and C0 whole (A, B,
+        # C0 are transposed views of row-major tensors, so .T is contiguous;
+        # ncols below n would time a column sample of the same product)
+        ncols = n
         cpu_inputs = [A.T.cpu().numpy().T, B.T[:ncols].cpu().numpy().T,
                       C0.T[:ncols].cpu().numpy().T]
         # configuration 1 (m=n=k=1024): drawn on the GPU, timed there too
