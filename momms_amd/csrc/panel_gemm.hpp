@@ -33,9 +33,14 @@
 // Fragments are software-pipelined: a wavefront holds two register sets and
 // reads the fragments of slab s+1 between the MFMAs of slab s (one read after
 // each MFMA), so LDS latency hides behind the matrix cores. The stage barrier
-// sits before the last slab's MFMAs, and the first fragments of the next
-// stage are read while the last slab multiplies. The MFMA order per
-// accumulator is slab order, as in any k loop over the same panels.
+// sits before the last slab's MFMAs; after it the first fragments of the
+// next stage are read, and the DMA of stage KS + 4 is issued one load at a
+// time between the last slab's final MFMAs, so the matrix cores hold queued
+// work while the loads go out (issued all at once after the barrier by every
+// wavefront, they left the matrix cores idle: 3-5 % of the residue GEMMs'
+// time). The slab that carries a block's modular reduction places its vector
+// instructions between the MFMAs explicitly (panel_slab_order_valu). The MFMA
+// order per accumulator is slab order, as in any k loop over the same panels.
 // The MFMA takes the B block as its "A" operand, so lane l holds row
 // i = l%32 of a block and columns 8(r/4) + 4(l/32) + r%4: column-major
 // stores along i are coalesced.
@@ -131,6 +136,25 @@ __device__ __forceinline__ void panel_slab_order() {
     __builtin_amdgcn_sched_group_barrier(0x008, FM * FN - 1 - FM - FN, 0);
 }
 
+// The same with up to VPER vector ALU instructions after each MFMA (the slab
+// that carries a block's modular reduction: its 48 instructions spread over
+// the shadows of the other blocks' MFMAs instead of bunching where the
+// compiler puts them; the reduced block's MFMA comes last in the slab).
+template <int FM, int FN, int VPER>
+__device__ __forceinline__ void panel_slab_order_valu() {
+#pragma unroll
+  for (int q = 0; q < FM + FN; ++q) {
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);     // MFMA
+    __builtin_amdgcn_sched_group_barrier(0x100, 1, 0);     // DS read
+    __builtin_amdgcn_sched_group_barrier(0x002, VPER, 0);  // VALU
+  }
+#pragma unroll
+  for (int q = FM + FN; q < FM * FN; ++q) {
+    __builtin_amdgcn_sched_group_barrier(0x008, 1, 0);     // MFMA
+    __builtin_amdgcn_sched_group_barrier(0x002, VPER, 0);  // VALU
+  }
+}
+
 // The k loop: acc += A_panel * B_panel over nslabs slabs (a multiple of SPS).
 // With MOD, each accumulator x is kept an exact FP32 integer by replacing it
 // with x - p rint(x / p) (exact by fma for |x| < 2^24) once every 256
@@ -155,22 +179,25 @@ __device__ __forceinline__ void panel_mainloop(typename PanelOps<I8>::acc (&acc)
                 "a chunk is whole stages and whole turns of the four-slot ring");
   static_assert(!MOD || NQ >= CHUNK_STAGES, "every stage of a chunk reduces some block");
   static_assert(!(MOD && I8), "INT8 sums are exact without reductions");
+  static_assert(G::LW <= NQ, "a stage's loads go between the last slab's MFMAs");
   typedef typename PanelOps<I8>::frag Frag;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wm = wave % WM, wn = wave / WM;
   const int64_t nst = nslabs / SPS;
 
-  auto issue = [&](int64_t kst, uint16_t* S) {
+  // load i (of LW) of this wavefront's part of stage kst, into ring slot S
+  auto issue_one = [&](int64_t kst, uint16_t* S, int i) {
     if (kst >= nst) kst = 0;  // keeps the ring full: stage 0 again, into a slot no one reads
+    const int c = wave + G::NW * i;
+    if (c < G::LA)
+      lds_dma16_untracked(Abase + kst * SPS * G::ABLK + c * 512 + lane * 8, &S[c * 512]);
+    else
+      lds_dma16_untracked(Bbase + kst * SPS * G::BBLK + (c - G::LA) * 512 + lane * 8,
+                          &S[G::BOFF + (c - G::LA) * 512]);
+  };
+  auto issue = [&](int64_t kst, uint16_t* S) {
 #pragma unroll
-    for (int i = 0; i < G::LW; ++i) {
-      const int c = wave + G::NW * i;
-      if (c < G::LA)
-        lds_dma16_untracked(Abase + kst * SPS * G::ABLK + c * 512 + lane * 8, &S[c * 512]);
-      else
-        lds_dma16_untracked(Bbase + kst * SPS * G::BBLK + (c - G::LA) * 512 + lane * 8,
-                            &S[G::BOFF + (c - G::LA) * 512]);
-    }
+    for (int i = 0; i < G::LW; ++i) issue_one(kst, S, i);
   };
 
   const int kh = lane >> 5;
@@ -212,17 +239,35 @@ __device__ __forceinline__ void panel_mainloop(typename PanelOps<I8>::acc (&acc)
     }
 #pragma unroll
     for (int ss = 0; ss < SPS; ++ss) {
+      const int f = ss == 0 ? first : 0;
       if (ss + 1 < SPS) {
         rd(Sc, ss + 1, an, bn);
+        mm(f);
+        if (MOD && ss == 0)
+          panel_slab_order_valu<FM, FN, 7>();
+        else
+          panel_slab_order<FM, FN>();
       } else {
         __builtin_amdgcn_sched_barrier(0);
         split_wait_vmcnt<2 * G::LW>();  // this wavefront's part of stage KS + 1 landed
         split_barrier();                // everyone's has; everyone is done reading slot KS
-        issue(KS + 4, Sc);
-        rd(Sn, 0, an, bn);  // past the last stage: unused
+        rd(Sn, 0, an, bn);              // past the last stage: unused
+        __builtin_amdgcn_sched_barrier(0);
+        // the slab's MFMAs with the DMA of stage KS + 4 (into slot KS, free
+        // since the barrier) after the last LW of them, one load each: the
+        // matrix cores have queued work while the loads are issued, instead
+        // of every wavefront issuing its loads at once after the barrier
+#pragma unroll
+        for (int t = 0; t < NQ; ++t) {
+          const int q = (f + t) % NQ, a = q / FN, b = q % FN;
+          acc[a][b] = PanelOps<I8>::mma(bf[b], af[a], acc[a][b]);
+          if (t >= NQ - G::LW) {
+            __builtin_amdgcn_sched_barrier(0);
+            issue_one(KS + 4, Sc, t - (NQ - G::LW));
+            __builtin_amdgcn_sched_barrier(0);
+          }
+        }
       }
-      mm(ss == 0 ? first : 0);
-      panel_slab_order<FM, FN>();
       __builtin_amdgcn_sched_barrier(0);
 #pragma unroll
       for (int a = 0; a < FM; ++a) af[a] = an[a];
