@@ -22,6 +22,8 @@ moduli, N residue GEMMs). Prints ONE JSON line:
   mfma_ceiling the same GPU's BF16 / INT8 matrix-core rate with operands in
                registers (random: the power-limited ceiling; zero: full
                clock), and the residue GEMMs' fractions of it
+  vendor_reference  the vendor library's BF16 and INT8 GEMMs at the same
+               size on the same GPU, and the residue GEMMs' rates against them
   cpu_baseline the restated Goto + 6x8 CPU path on this host's cores
 
 roofline.traffic (HBM bytes per launch of crt_multiply) is measured in the
@@ -529,6 +531,21 @@ def measure_traffic(n: int, moduli: int, limit: int = 180) -> dict:
             "write_factor": round(wf, 4), "traffic_bytes": int(round((fetch * ff + write * wf) * 1024))}
 
 
+def vendor_lowp_gemms(torch, gen, n: int, steps: int, warmup: int) -> tuple:
+    """Milliseconds per call of the vendor library's plain BF16 GEMM
+    (torch.matmul, hipBLASLt) and INT8 GEMM (torch._int_mm) at n^3 on this
+    GPU: the same-box reference points for the residue GEMMs."""
+    Ah = (torch.rand((n, n), generator=gen, device="cuda") * 2 - 1).to(torch.bfloat16)
+    Bh = (torch.rand((n, n), generator=gen, device="cuda") * 2 - 1).to(torch.bfloat16)
+    ms_bf = time_loop(lambda: torch.matmul(Ah, Bh), steps, warmup)
+    del Ah, Bh
+    Ai = torch.randint(-128, 128, (n, n), generator=gen, device="cuda", dtype=torch.int8)
+    Bi = torch.randint(-128, 128, (n, n), generator=gen, device="cuda", dtype=torch.int8)
+    ms_i8 = time_loop(lambda: torch._int_mm(Ai, Bi), steps, warmup)
+    del Ai, Bi
+    return ms_bf, ms_i8
+
+
 def mfma_ceiling(limit: int = 60) -> dict:
     """The matrix cores' sustained rate on this GPU with the operands in
     registers (tests/native/mfma_ceiling.hip, run as a child process): the
@@ -653,14 +670,7 @@ def main() -> int:
         # reference points on the same GPU: the vendor library's plain BF16 GEMM
         # (torch.matmul, hipBLASLt) and INT8 GEMM (torch._int_mm), n^3
         del A, B, C0, C
-        Ah = (torch.rand((n, n), generator=gen, device="cuda") * 2 - 1).to(torch.bfloat16)
-        Bh = (torch.rand((n, n), generator=gen, device="cuda") * 2 - 1).to(torch.bfloat16)
-        ms_bf = time_loop(lambda: torch.matmul(Ah, Bh), args.steps, args.warmup)
-        del Ah, Bh
-        Ai = torch.randint(-128, 128, (n, n), generator=gen, device="cuda", dtype=torch.int8)
-        Bi = torch.randint(-128, 128, (n, n), generator=gen, device="cuda", dtype=torch.int8)
-        ms_i8 = time_loop(lambda: torch._int_mm(Ai, Bi), args.steps, args.warmup)
-        del Ai, Bi
+        ms_bf, ms_i8 = vendor_lowp_gemms(torch, gen, n, args.steps, args.warmup)
         # the vendor library's FP64 GEMM (torch.addmm on column-major operands,
         # C := C + A*B) over the plain sweep's shapes, beside this library's
         fp64 = []
@@ -934,6 +944,19 @@ def main() -> int:
     except Exception as exc:  # a reported reference point, not part of the metric
         log(f"bench: matrix-core ceiling not measured: {exc}")
         result["mfma_ceiling"] = {"error": str(exc)}
+    # the vendor library's plain BF16 and INT8 GEMMs at the same size on the
+    # same GPU: what its own kernels reach under the same power limit
+    log("bench: vendor BF16 / INT8 GEMMs at the same size")
+    v_bf, v_i8 = vendor_lowp_gemms(torch, gen, n, args.steps, args.warmup)
+    result["vendor_reference"] = {
+        "bf16_matmul": {"ms": round(v_bf, 4), "tflops": round(flops / v_bf / 1e9, 2),
+                        "frac_peak": round(flops / v_bf / 1e9 / PEAK_BF16, 4)},
+        "int8_int_mm": {"ms": round(v_i8, 4), "tops": round(flops / v_i8 / 1e9, 2),
+                        "frac_peak": round(flops / v_i8 / 1e9 / PEAK_INT8, 4)},
+        "crt_multiply_rate_vs_bf16_matmul": round(nm * flops / prod_ms / (flops / v_bf), 4),
+        "crt_i8_products_rate_vs_int_mm": round(n8 * flops / i8_prod_ms / (flops / v_i8), 4),
+        "note": "torch.matmul on bf16 and torch._int_mm on int8, m=n=k as the workload; "
+                "rates are per GEMM, the residue kernels' per modulus"}
     cpu_inputs, c1_gpu_tf = None, None
     if not args.no_cpu:
         # host copies for the CPU leg: the workload's A, B and C0 whole (A, B,
