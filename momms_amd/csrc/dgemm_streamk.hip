@@ -127,11 +127,14 @@ __device__ __forceinline__ void sk_tile_range(SkSmem<BM, BN, STAGES>& sm, int64_
       for (int e = tid; e < (BK - kv) * BM; e += 256) sm.A[cur][kv + e / BM][e % BM] = 0.0;
       lds_barrier();
     }
-    if (j + STAGES - 1 < nk) issue(kt + STAGES - 1, (int)((j + STAGES - 1) % STAGES));
     // MFMA step 2u + t gives lane group lk the k-value 8u + 2lk + t; a lane's
-    // B values for two steps are its column's k-pair 4u + lk, one 16-byte read
+    // B values for two steps are its column's k-pair 4u + lk, one 16-byte read.
+    // The loads of slab kt + STAGES - 1 (into the slot every wave finished
+    // with before the barrier) go out after the first half of the MFMAs, so
+    // the matrix cores hold work while they are issued.
 #pragma unroll
     for (int u = 0; u < BK / 8; ++u) {
+      if (u == 1 && j + STAGES - 1 < nk) issue(kt + STAGES - 1, (int)((j + STAGES - 1) % STAGES));
       const int kp = 4 * u + lk;
       v2f64 bp[FN];
 #pragma unroll
