@@ -410,13 +410,16 @@ __device__ __forceinline__ void dgemm_tile_dma(DgemmDmaSmem<STAGES>& sm, int64_t
       for (int e = tid; e < (BK - kv) * BM; e += 256) sm.A[cur][kv + e / BM][e % BM] = 0.0;
       lds_barrier();
     }
-    if (kt + STAGES - 1 < ktiles) issue(kt + STAGES - 1, (int)((kt + STAGES - 1) % STAGES));
     // k order inside the slab: MFMA step 2u + t gives lane group lk the
     // k-value 8u + 2lk + t, so a lane's B values for two steps are its
     // column's k-pair 4u + lk, one 16-byte read (the 8 columns of a group of
-    // 8 lanes sit in distinct banks)
+    // 8 lanes sit in distinct banks). The loads of slab kt + STAGES - 1 (into
+    // the slot every wave finished with before the barrier) go out after the
+    // first half of the MFMAs, so the matrix cores hold work meanwhile.
 #pragma unroll
     for (int u = 0; u < BK / 8; ++u) {
+      if (u == 1 && kt + STAGES - 1 < ktiles)
+        issue(kt + STAGES - 1, (int)((kt + STAGES - 1) % STAGES));
       const int kp = 4 * u + lk;
       v2f64 bp[FN];
 #pragma unroll
