@@ -213,11 +213,13 @@ __device__ __forceinline__ void sgemm_clear_tail(float* As, float* Bs, int kv) {
   }
 }
 
+// k-groups [G0, G1) (of four, 8 k-values each) of a landed 32-deep slab
+template <int G0, int G1>
 __device__ __forceinline__ void sgemm_slab(v16f32 (&acc)[2][2], const float* As, const float* Bs) {
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int wm = wave & 1, wn = wave >> 1, h = lane >> 5, lr = lane & 31;
 #pragma unroll
-  for (int g = 0; g < 4; ++g) {
+  for (int g = G0; g < G1; ++g) {
     v4f32 bq[2];
 #pragma unroll
     for (int b = 0; b < 2; ++b) {
@@ -264,8 +266,9 @@ sgemm_dma_kernel(int64_t m, int64_t n, int64_t k, float alpha, const float* __re
 #pragma unroll
       for (int r = 0; r < 16; ++r) acc[a][b][r] = 0.0f;
 
-  // slab kt: wait for it, then (everyone done with the other stage) start
-  // slab kt + 1 into the other stage and multiply this one
+  // slab kt: wait for it, then (everyone done with the other stage) multiply
+  // its first half, start slab kt + 1 into the other stage (the matrix cores
+  // hold work while the loads are issued) and multiply the second half
 #define MOMMS_SG_STAGE(KT, Ac, Bc, An, Bn)                             \
   {                                                                    \
     wait_vmcnt<0>();                                                   \
@@ -274,9 +277,10 @@ sgemm_dma_kernel(int64_t m, int64_t n, int64_t k, float alpha, const float* __re
       sgemm_clear_tail(Ac, Bc, (int)(k - (KT) * 32));                  \
       lds_barrier();                                                   \
     }                                                                  \
+    sgemm_slab<0, 2>(acc, Ac, Bc);                                     \
     if ((KT) + 1 < ktiles)                                             \
       sgemm_issue(ra, lda, rb, ldb, row0, col0, ((KT) + 1) * 32, An, Bn); \
-    sgemm_slab(acc, Ac, Bc);                                           \
+    sgemm_slab<2, 4>(acc, Ac, Bc);                                     \
   }
   if (ktiles > 0) sgemm_issue(ra, lda, rb, ldb, row0, col0, 0, As0, Bs0);
   for (int64_t kt = 0; kt < ktiles; kt += 2) {
