@@ -17,6 +17,8 @@ The slice form folds its levels in FP64, so it keeps the transpose and
 power-of-two properties but not the k-permutation one. Each property was
 checked on the oracle (oracle/crt.c, oracle/ozaki.c) before being asserted
 here; a dropped term, a wrong exponent or an inexact partial sum breaks it.
+The plain FP64 kernels (data-parallel, stream-K, k-split) scale exactly by
+powers of two too, and repeat their bits from run to run.
 """
 import pytest
 import torch
@@ -105,3 +107,24 @@ def test_power_of_two_scaling(inputs, form, count):
     c = run(form, A, B, count)
     cs = run(form, colmajor(A * 32.0), colmajor(B * 0.125), count)
     assert same_bits(c * 4.0, cs)
+
+
+@pytest.mark.parametrize("m,n,k", [(8192, 8192, 8192), (1024, 1024, 1024), (8192, 64, 8192),
+                                   (512, 512, 512), (8192, 8192, 256)])
+def test_plain_fp64_power_of_two_scaling_and_repeatability(m, n, k):
+    """Every product and sum of the FP64 kernels (the data-parallel tile, the
+    stream-K and in-workgroup k-split forms with their fixed-order sums of
+    parts) scales exactly by a power of two, so the result must too; a second
+    run gives the same bits."""
+    gen = torch.Generator(device="cuda").manual_seed((SEED + m + n + k) & 0xFFFFFFFF)
+    A, B = rand_colmajor(m, k, gen, 8), rand_colmajor(k, n, gen, 8)
+
+    def plain(X, Y):
+        C = torch.empty((n, m), dtype=torch.float64, device="cuda").T
+        capi.dgemm(X, Y, C, 1.0, 0.0)
+        torch.cuda.synchronize()
+        return C
+
+    c = plain(A, B)
+    assert same_bits(c, plain(A, B))
+    assert same_bits(c * 4.0, plain(colmajor(A * 32.0), colmajor(B * 0.125)))
